@@ -1,0 +1,192 @@
+"""Benchmark: sample-point updates/s of the edge-bundling hot path on one MI355X.
+
+Workload (plain GPU case): 1,715 nodes from a 20-component Gaussian mixture,
+9,800 straight edges (seed 2), resampled at spacing 0.002 (~1.33M samples),
+30 iterations with the neighbor radius decaying 0.06 -> 0.01 geometrically,
+1024 x 1024 FP32 density with sigma 8 -> 2. One step = the whole 30-iteration
+schedule (the scope of 30 Gpu::iterate calls) replayed as one hipGraph from
+the post-upload state; upload/resampling is timed separately.
+
+Prints one JSON line (see the keys below). Extra objects: roofline (dominant
+kernel vs HBM peak, from HIP events), valu (the same kernel's FP32 work vs the
+vector peak), cpu_baseline (the CPU oracle on a bounded sample), latency.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+ITERS = 30
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def candidate_flops(counts2d):
+    """8 FP32 ops per candidate pair; candidates of a sample = samples in the
+    3 x 3 block of cells (side = radius) around its cell."""
+    c = counts2d.astype(np.int64)
+    p = np.pad(c, 1)
+    s = sum(p[1 + dy:1 + dy + c.shape[0], 1 + dx:1 + dx + c.shape[1]]
+            for dy in (-1, 0, 1) for dx in (-1, 0, 1))
+    return 8 * int((c * s).sum())
+
+
+def cpu_baseline(graph, opts, target_s=15.0):
+    """The CPU oracle (all host threads) on a bounded sample of the same
+    schedule: neighbor means of every `stride`-th sample for all 30 radii at
+    the initial positions, scaled by `stride`, plus the full density stages."""
+    from oracle import oracle as orc
+    p0 = opts.iter_params(0)
+    pl = orc.Pipeline(graph.points, graph.starts, graph.bbox, p0)
+    pts, fixed = pl.points, pl.fixed
+    threads = os.cpu_count() or 1
+    # calibrate the stride on the heaviest (first) radius
+    g0 = orc.grid(p0)
+    stride = 4096
+    t = time.perf_counter()
+    orc.neighbor_mean_strided(pts, fixed, g0, stride, threads)
+    t1 = time.perf_counter() - t
+    est = t1 * ITERS  # upper bound: later radii are smaller
+    stride = int(max(1, min(4096, np.ceil(stride * est / target_s))))
+    total = 0.0
+    for i in range(ITERS):
+        p = opts.iter_params(i)
+        g = orc.grid(p)
+        t = time.perf_counter()
+        out = orc.neighbor_mean_strided(pts, fixed, g, stride, threads)
+        tm = time.perf_counter() - t
+        t = time.perf_counter()
+        c = orc.splat(out, p.density_w, p.density_h)
+        orc.density(c, p.sigma)
+        td = time.perf_counter() - t
+        total += tm * stride + td
+    n = len(pts)
+    return {"value": ITERS * n / total, "unit": "sample-point updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle neighbor means of every {stride}th sample for all {ITERS} "
+                      f"radii at the initial positions (x{stride}), full splat/blur/normalize; "
+                      f"est. {total:.1f} s per schedule"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cell-split", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
+              "single-GPU build (jobs shard across GPUs without exchange; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import bundling_thingy_amd as bt
+    from bundling_thingy_amd import workloads
+
+    graph = workloads.plain_gpu_graph()
+    opts = workloads.plain_gpu_options(ITERS, cell_split=args.cell_split)
+    gpu = bt.Gpu(opts)
+    t = time.perf_counter()
+    n = gpu.upload(graph)
+    upload_ms = (time.perf_counter() - t) * 1e3
+    log(f"upload: {n} samples in {upload_ms:.1f} ms")
+
+    # warmup (also builds the hipGraph of the schedule)
+    t = time.perf_counter()
+    for _ in range(max(args.warmup, 1)):
+        gpu.rewind()
+        gpu.iterate(ITERS, graph=True)
+    log(f"warmup done in {time.perf_counter() - t:.2f} s")
+
+    times = []
+    for s in range(args.steps):
+        gpu.rewind()
+        t = time.perf_counter()
+        gpu.iterate(ITERS, graph=True)   # synchronizes the stream before returning
+        times.append(time.perf_counter() - t)
+        log(f"step {s}: {times[-1] * 1e3:.2f} ms")
+    total = float(sum(times))
+    ms = total / args.steps * 1e3
+    value = ITERS * n * args.steps / total
+
+    # Per-stage device time from HIP events on the engine's stream (eager run of
+    # the same schedule), plus the grid counts for the FP32-work figure.
+    gpu.rewind()
+    stage_ms = {k: 0.0 for k in bt.api.STAGES}
+    flops = 0
+    for i in range(ITERS):
+        st, _ = gpu.profile(1)
+        for k, v in st.items():
+            stage_ms[k] += v
+        _, counts = gpu.grid()
+        flops += candidate_flops(counts)
+    dom = max(stage_ms, key=stage_ms.get)
+    mean_ms = stage_ms["neighbor_mean"] / ITERS
+    P = 1024 * 1024
+    alg_bytes = 24 * n + 4 * n  # read + write 12 B records, splat 4 B per sample
+    roof = {"kernel": "bt_neighbor_mean", "bound": "hbm",
+            "achieved": alg_bytes / (mean_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "traffic": None}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    valu = {"kernel": "bt_neighbor_mean", "bound": "valu",
+            "achieved": flops / (stage_ms["neighbor_mean"] * 1e-3) / 1e12,
+            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "flops_per_schedule": flops}
+    valu["frac"] = valu["achieved"] / valu["peak"]
+    iter_bytes = 24 * n + 28 * P
+    path = {"bytes_per_iteration": iter_bytes,
+            "achieved_GBs": iter_bytes * ITERS / (total / args.steps) / 1e9}
+
+    cpu = None
+    if not args.no_cpu:
+        t = time.perf_counter()
+        cpu = cpu_baseline(graph, opts)
+        log(f"cpu baseline: {cpu['value']:.3e} updates/s ({time.perf_counter() - t:.1f} s)")
+
+    out = {
+        "metric": "sample-point updates/s (plain, 1 GPU)",
+        "value": value,
+        "unit": "sample-point updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator; reference datasets are LFS stubs)",
+        "config": {"workload": "plain GPU case: 1,715-node 20-component GMM, 9,800 edges, "
+                               "spacing 0.002, 30 iterations radius 0.06->0.01, 1024^2 density "
+                               "sigma 8->2",
+                   "samples": n, "iterations": ITERS, "density": [1024, 1024]},
+        "latency": {"p50_ms": float(np.percentile(np.array(times) * 1e3, 50)),
+                    "p99_ms": float(np.percentile(np.array(times) * 1e3, 99)),
+                    "upload_ms": upload_ms},
+        "stage_ms_per_schedule": stage_ms,
+        "dominant_stage": dom,
+        "roofline": roof,
+        "valu": valu,
+        "path": path,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out))
+    gpu.close()
+
+
+if __name__ == "__main__":
+    main()

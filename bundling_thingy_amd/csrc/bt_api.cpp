@@ -1,0 +1,528 @@
+// The C ABI (include/bundling.h): argument checks, error capture, and the
+// served form's scheduler.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+
+#include "bt_common.h"
+#include "bt_engine.h"
+
+struct bt_ctx {
+  std::unique_ptr<bt::Engine> engine;
+  std::string err;
+};
+
+namespace {
+
+template <class F>
+int guarded(std::string* err, F&& f) {
+  try {
+    f();
+    return BT_OK;
+  } catch (const bt::Error& e) {
+    if (err) *err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (err) *err = "host allocation failed";
+    return BT_E_CAPACITY;
+  } catch (const std::exception& e) {
+    if (err) *err = e.what();
+    return BT_E_DEVICE;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void bt_options_default(bt_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->map_w = 1600;
+  o->map_h = 1600;
+  o->flags = BT_FLAG_DENSITY;
+  o->max_graph_memory_bytes = 1024ull << 20;
+}
+
+int bt_create(const bt_options* opts, bt_ctx** out) {
+  if (!out) return BT_E_INVALID;
+  *out = nullptr;
+  auto ctx = std::make_unique<bt_ctx>();
+  std::string err;
+  int rc = guarded(&err, [&] { ctx->engine = std::make_unique<bt::Engine>(bt::resolve_options(opts)); });
+  if (rc != BT_OK) {
+    // No context to hold the message: keep it in a static slot.
+    static thread_local std::string last;
+    last = err;
+    return rc;
+  }
+  *out = ctx.release();
+  return BT_OK;
+}
+
+void bt_destroy(bt_ctx* ctx) {
+  if (!ctx) return;
+  try {
+    ctx->engine.reset();
+  } catch (...) {
+  }
+  delete ctx;
+}
+
+const char* bt_last_error(const bt_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+#define CTX_CHECK()                      \
+  if (!ctx || !ctx->engine) return BT_E_INVALID; \
+  ctx->err.clear();
+
+int bt_upload(bt_ctx* ctx, const float* xy, uint32_t n_points, const uint32_t* starts,
+              uint32_t n_starts, uint32_t* samples) {
+  CTX_CHECK();
+  return guarded(&ctx->err, [&] {
+    std::vector<bt::GraphIn> gs = {{xy, n_points, starts, n_starts, ctx->engine->options()}};
+    std::vector<uint32_t> s;
+    ctx->engine->upload(gs, &s);
+    if (samples) *samples = s[0];
+  });
+}
+
+int bt_upload_batch(bt_ctx* ctx, const bt_graph_desc* graphs, uint32_t n_graphs,
+                    uint32_t* samples) {
+  CTX_CHECK();
+  if (!graphs || n_graphs == 0) return BT_E_INVALID;
+  return guarded(&ctx->err, [&] {
+    std::vector<bt::GraphIn> gs;
+    for (uint32_t i = 0; i < n_graphs; ++i) {
+      const bt_graph_desc& g = graphs[i];
+      gs.push_back({g.xy, g.n_points, g.starts, g.n_starts,
+                    g.opts ? *g.opts : ctx->engine->options()});
+    }
+    std::vector<uint32_t> s;
+    ctx->engine->upload(gs, &s);
+    if (samples) std::copy(s.begin(), s.end(), samples);
+  });
+}
+
+int bt_iterate(bt_ctx* ctx) {
+  CTX_CHECK();
+  return guarded(&ctx->err, [&] { ctx->engine->iterate(1, false); });
+}
+
+int bt_iterate_n(bt_ctx* ctx, uint32_t n) {
+  CTX_CHECK();
+  return guarded(&ctx->err, [&] { ctx->engine->iterate(n, true); });
+}
+
+int bt_rewind(bt_ctx* ctx) {
+  CTX_CHECK();
+  return guarded(&ctx->err, [&] { ctx->engine->rewind(); });
+}
+
+int bt_read_paths(bt_ctx* ctx, uint32_t* rec3, size_t n_paths) {
+  return bt_read_paths_job(ctx, 0, rec3, n_paths);
+}
+
+int bt_read_paths_job(bt_ctx* ctx, uint32_t job, uint32_t* rec3, size_t n_paths) {
+  CTX_CHECK();
+  if (!rec3) return BT_E_INVALID;
+  return guarded(&ctx->err, [&] { ctx->engine->read_paths(job, rec3, n_paths); });
+}
+
+int bt_read_points(bt_ctx* ctx, void* rec12, size_t n_samples) {
+  return bt_read_points_job(ctx, 0, rec12, n_samples);
+}
+
+int bt_read_points_job(bt_ctx* ctx, uint32_t job, void* rec12, size_t n_samples) {
+  CTX_CHECK();
+  if (!rec12) return BT_E_INVALID;
+  return guarded(&ctx->err, [&] { ctx->engine->read_points(job, rec12, n_samples); });
+}
+
+int bt_read_density(bt_ctx* ctx, uint32_t kind, void* out, size_t n_pixels) {
+  return bt_read_density_job(ctx, 0, kind, out, n_pixels);
+}
+
+int bt_read_density_job(bt_ctx* ctx, uint32_t job, uint32_t kind, void* out, size_t n_pixels) {
+  CTX_CHECK();
+  if (!out) return BT_E_INVALID;
+  return guarded(&ctx->err, [&] { ctx->engine->read_density(job, kind, out, n_pixels); });
+}
+
+int bt_read_grid(bt_ctx* ctx, uint32_t* cell_of, size_t n_samples, uint32_t* counts,
+                 size_t n_cells) {
+  CTX_CHECK();
+  if (!cell_of || !counts) return BT_E_INVALID;
+  return guarded(&ctx->err,
+                 [&] { ctx->engine->read_grid(0, cell_of, n_samples, counts, n_cells); });
+}
+
+int bt_job_samples(bt_ctx* ctx, uint32_t job, uint32_t* samples, uint32_t* iteration) {
+  CTX_CHECK();
+  return guarded(&ctx->err, [&] {
+    bt::Job& j = ctx->engine->job(job);
+    if (samples) *samples = j.n;
+    if (iteration) *iteration = j.iteration;
+  });
+}
+
+int bt_get_info(bt_ctx* ctx, bt_info* info) {
+  CTX_CHECK();
+  if (!info) return BT_E_INVALID;
+  std::memset(info, 0, sizeof *info);
+  info->jobs = (uint32_t)ctx->engine->njobs();
+  info->device_bytes = ctx->engine->device_bytes();
+  if (info->jobs) {
+    bt::Job& j = ctx->engine->job(0);
+    info->samples = j.n;
+    info->paths = j.n_paths;
+    info->iteration = j.iteration;
+    info->capacity = j.capacity;
+    info->reserved_bytes = j.reserved;
+  }
+  return BT_OK;
+}
+
+int bt_iter_params_for(const bt_options* opts, uint32_t iteration, bt_iter_params* out) {
+  if (!out) return BT_E_INVALID;
+  bt_options o = bt::resolve_options(opts);
+  bt::iter_params(o, iteration, o.cell_split ? o.cell_split : 1, out);
+  return BT_OK;
+}
+
+int bt_profile(bt_ctx* ctx, uint32_t n, double* stage_ms, uint32_t n_stages, double* total_ms) {
+  CTX_CHECK();
+  if (!stage_ms && n_stages) return BT_E_INVALID;
+  return guarded(&ctx->err, [&] { ctx->engine->profile(n, stage_ms, n_stages, total_ms); });
+}
+
+int bt_trl_parse(const char* text, size_t len, float** xy, uint32_t* n_points, uint32_t** starts,
+                 uint32_t* n_starts, float* bbox, char* err, size_t errlen) {
+  if (!text || !xy || !n_points || !starts || !n_starts || !bbox) return BT_E_INVALID;
+  std::vector<float> p;
+  std::vector<uint32_t> s;
+  std::string e;
+  int rc = bt::parse_trl(text, len, p, s, bbox, &e);
+  if (rc != BT_OK) {
+    if (err && errlen) {
+      std::strncpy(err, e.c_str(), errlen - 1);
+      err[errlen - 1] = 0;
+    }
+    return rc;
+  }
+  *xy = (float*)std::malloc(p.size() * sizeof(float));
+  *starts = (uint32_t*)std::malloc(s.size() * sizeof(uint32_t));
+  if (!*xy || !*starts) {
+    std::free(*xy);
+    std::free(*starts);
+    return BT_E_CAPACITY;
+  }
+  std::memcpy(*xy, p.data(), p.size() * sizeof(float));
+  std::memcpy(*starts, s.data(), s.size() * sizeof(uint32_t));
+  *n_points = (uint32_t)(p.size() / 2);
+  *n_starts = (uint32_t)s.size();
+  return BT_OK;
+}
+
+void bt_free(void* p) { std::free(p); }
+
+int bt_normalization(const float* bbox, float* out) {
+  if (!bbox || !out) return BT_E_INVALID;
+  return bt::normalization(bbox, out) ? BT_OK : BT_E_INVALID;
+}
+
+int bt_capacity(const bt_options* opts, uint32_t n_points, uint32_t n_starts, uint64_t* capacity,
+                uint64_t* reserved_bytes) {
+  if (!capacity || !reserved_bytes) return BT_E_INVALID;
+  std::string e;
+  return bt::capacity_model(bt::resolve_options(opts), n_points, n_starts, capacity,
+                            reserved_bytes, &e);
+}
+
+uint32_t bt_blur_weights(float sigma, float* w, uint32_t max_taps) {
+  if (!w) return 0;
+  return bt::blur_weights(sigma, w, max_taps);
+}
+
+int bt_distortion(const float* original_xy, const float* current_xy, const uint32_t* rec3,
+                  uint32_t n_paths, uint32_t n_samples, uint32_t map_size, double* out) {
+  if (!original_xy || !current_xy || !rec3 || !out) return BT_E_INVALID;
+  std::string e;
+  return bt::distortion(original_xy, current_xy, rec3, n_paths, n_samples, map_size, out, &e);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ server
+
+namespace {
+
+struct ServedJob {
+  uint64_t ticket = 0;
+  uint32_t state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  std::vector<float> xy;
+  std::vector<uint32_t> starts;
+  bt_options opts{};
+  uint32_t iterations = 0;
+  uint64_t estimate = 0;  // device bytes expected before admission
+  std::unique_ptr<bt::Job> job;
+  std::vector<unsigned char> points;  // result records (12 B each)
+  std::vector<float> density;         // normalized density at completion
+  uint32_t samples = 0, dw = 0, dh = 0;
+  double submit_ms = 0, admit_ms = 0, done_ms = 0;
+  std::string err;
+};
+
+}  // namespace
+
+struct bt_server {
+  bt_server_options o{};
+  std::unique_ptr<bt::Engine> engine;
+  std::chrono::steady_clock::time_point t0;
+  uint64_t next_ticket = 1;
+  uint64_t used = 0;
+  std::deque<ServedJob*> queue;
+  std::vector<ServedJob*> running;
+  std::map<uint64_t, std::unique_ptr<ServedJob>> all;
+  std::string err;
+  double now() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+        .count();
+  }
+};
+
+namespace {
+
+// Device bytes a job will hold, estimated before resampling from the polyline
+// lengths in texels (the count pass yields the exact figure after admission).
+uint64_t estimate_bytes(const ServedJob& s) {
+  bt_options o = bt::resolve_options(&s.opts);
+  float bbox[4] = {3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f};
+  const uint32_t np = (uint32_t)(s.xy.size() / 2);
+  for (uint32_t i = 0; i < np; ++i) {
+    bbox[0] = std::min(bbox[0], s.xy[2 * i]);
+    bbox[1] = std::min(bbox[1], s.xy[2 * i + 1]);
+    bbox[2] = std::max(bbox[2], s.xy[2 * i]);
+    bbox[3] = std::max(bbox[3], s.xy[2 * i + 1]);
+  }
+  float norm[5];
+  if (!bt::normalization(bbox, norm)) return 0;
+  bt_iter_params p;
+  bt::iter_params(o, 0, 1, &p);
+  double samples = 0.0;
+  for (size_t e = 0; e + 1 < s.starts.size(); ++e) {
+    for (uint32_t k = s.starts[e] + 1; k < s.starts[e + 1]; ++k) {
+      const double dx = (double)(s.xy[2 * k] - s.xy[2 * k - 2]) * norm[2] * p.res_x;
+      const double dy = (double)(s.xy[2 * k + 1] - s.xy[2 * k - 1]) * norm[2] * p.res_y;
+      samples += std::sqrt(dx * dx + dy * dy);
+    }
+    samples += 2.0;
+  }
+  uint64_t cells = 1;
+  for (uint32_t i = 0; i < bt::schedule_len(o); ++i) {
+    bt_iter_params q;
+    bt::iter_params(o, i, 1, &q);
+    cells = std::max<uint64_t>(cells, (uint64_t)q.gx * q.gy);
+  }
+  const uint64_t n = (uint64_t)(samples * 1.02) + 64;
+  uint64_t bytes = n * (12 * 3 + 16 + 4 + 8 + 4) + (cells + 1) * 4 + s.xy.size() * 4 +
+                   s.starts.size() * 24 + 4096;
+  if (o.flags & BT_FLAG_DENSITY) bytes += (uint64_t)p.density_w * p.density_h * 32;
+  return bytes;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bt_server_create(const bt_server_options* opts, bt_server** out) {
+  if (!opts || !out) return BT_E_INVALID;
+  *out = nullptr;
+  auto s = std::make_unique<bt_server>();
+  s->o = *opts;
+  if (s->o.max_inflight == 0) s->o.max_inflight = 64;
+  if (s->o.memory_budget == 0) s->o.memory_budget = 64ull << 30;
+  std::string err;
+  int rc = guarded(&err, [&] {
+    s->engine = std::make_unique<bt::Engine>(bt::resolve_options(&s->o.defaults));
+  });
+  if (rc != BT_OK) return rc;
+  s->t0 = std::chrono::steady_clock::now();
+  *out = s.release();
+  return BT_OK;
+}
+
+void bt_server_destroy(bt_server* srv) {
+  if (!srv) return;
+  try {
+    srv->all.clear();
+    srv->engine.reset();
+  } catch (...) {
+  }
+  delete srv;
+}
+
+const char* bt_server_last_error(const bt_server* srv) { return srv ? srv->err.c_str() : ""; }
+
+double bt_server_clock_ms(const bt_server* srv) { return srv ? srv->now() : 0.0; }
+
+int bt_server_submit(bt_server* srv, const float* xy, uint32_t n_points, const uint32_t* starts,
+                     uint32_t n_starts, const bt_options* opts, uint32_t iterations,
+                     uint64_t* ticket) {
+  if (!srv || !xy || !starts || n_starts < 2 || iterations == 0 || !ticket) return BT_E_INVALID;
+  auto s = std::make_unique<ServedJob>();
+  s->ticket = srv->next_ticket++;
+  s->xy.assign(xy, xy + 2ull * n_points);
+  s->starts.assign(starts, starts + n_starts);
+  s->opts = bt::resolve_options(opts ? opts : &srv->o.defaults);
+  s->iterations = iterations;
+  s->submit_ms = srv->now();
+  s->estimate = estimate_bytes(*s);
+  if (s->estimate == 0 || s->estimate > srv->o.memory_budget) {
+    srv->err = "job footprint exceeds the server memory budget";
+    return BT_E_CAPACITY;
+  }
+  *ticket = s->ticket;
+  srv->queue.push_back(s.get());
+  srv->all.emplace(s->ticket, std::move(s));
+  return BT_OK;
+}
+
+int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
+  if (!srv) return BT_E_INVALID;
+  srv->err.clear();
+  return guarded(&srv->err, [&] {
+    bt::Engine& e = *srv->engine;
+    // Admission: FIFO while the footprint fits the budget and the batch cap.
+    std::vector<ServedJob*> admit;
+    uint64_t used = srv->used;
+    while (!srv->queue.empty() && srv->running.size() + admit.size() < srv->o.max_inflight) {
+      ServedJob* s = srv->queue.front();
+      if (used + s->estimate > srv->o.memory_budget) break;
+      used += s->estimate;
+      admit.push_back(s);
+      srv->queue.pop_front();
+    }
+    if (!admit.empty()) {
+      std::vector<bt::GraphIn> gs;
+      for (ServedJob* s : admit)
+        gs.push_back({s->xy.data(), (uint32_t)(s->xy.size() / 2), s->starts.data(),
+                      (uint32_t)s->starts.size(), s->opts});
+      std::vector<std::unique_ptr<bt::Job>> jobs;
+      try {
+        e.upload_jobs(gs, &jobs);
+      } catch (const bt::Error& err) {
+        // Isolate failures: retry one by one so one bad graph fails alone.
+        jobs.clear();
+        for (size_t i = 0; i < admit.size(); ++i) {
+          std::vector<std::unique_ptr<bt::Job>> one;
+          try {
+            e.upload_jobs({gs[i]}, &one);
+            jobs.push_back(std::move(one[0]));
+          } catch (const bt::Error& err1) {
+            admit[i]->state = 3;
+            admit[i]->err = err1.msg;
+            admit[i]->done_ms = srv->now();
+            jobs.push_back(nullptr);
+          }
+        }
+        (void)err;
+      }
+      const double t = srv->now();
+      for (size_t i = 0; i < admit.size(); ++i) {
+        if (!jobs[i]) continue;
+        ServedJob* s = admit[i];
+        s->job = std::move(jobs[i]);
+        s->state = 1;
+        s->admit_ms = t;
+        s->samples = s->job->n;
+        s->estimate = std::max<uint64_t>(s->estimate, s->job->device_bytes());
+        srv->used += s->estimate;
+        srv->running.push_back(s);
+      }
+    }
+    // One iteration of every running job in one batched launch per stage.
+    if (!srv->running.empty()) {
+      std::vector<bt::Job*> js;
+      for (ServedJob* s : srv->running) js.push_back(s->job.get());
+      e.iterate_jobs(js);
+      std::vector<ServedJob*> still;
+      for (ServedJob* s : srv->running) {
+        if (s->job->iteration < s->iterations) {
+          still.push_back(s);
+          continue;
+        }
+        s->points.resize((size_t)s->job->n * 12);
+        bt::Engine::read_points_of(e, *s->job, s->points.data(), s->job->n);
+        if (s->job->dens_cnt.p) {
+          s->dw = s->job->dw;
+          s->dh = s->job->dh;
+          s->density.resize((size_t)s->dw * s->dh);
+          bt::Engine::read_density_of(e, *s->job, 3, s->density.data(), s->density.size());
+        }
+        srv->used -= std::min(srv->used, s->estimate);
+        s->job.reset();
+        s->state = 2;
+        s->done_ms = srv->now();
+      }
+      srv->running.swap(still);
+    }
+    if (running) *running = (uint32_t)srv->running.size();
+    if (queued) *queued = (uint32_t)srv->queue.size();
+  });
+}
+
+int bt_server_poll(bt_server* srv, uint64_t ticket, bt_job_result* out) {
+  if (!srv || !out) return BT_E_INVALID;
+  auto it = srv->all.find(ticket);
+  if (it == srv->all.end()) return BT_E_INVALID;
+  const ServedJob& s = *it->second;
+  out->ticket = s.ticket;
+  out->state = s.state;
+  out->samples = s.samples;
+  out->iterations_done = s.job ? s.job->iteration : (s.state == 2 ? s.iterations : 0);
+  out->iterations_total = s.iterations;
+  out->submit_ms = s.submit_ms;
+  out->admit_ms = s.admit_ms;
+  out->done_ms = s.done_ms;
+  if (s.state == 3) srv->err = s.err;
+  return BT_OK;
+}
+
+int bt_server_read_points(bt_server* srv, uint64_t ticket, void* rec12, size_t n_samples) {
+  if (!srv || !rec12) return BT_E_INVALID;
+  auto it = srv->all.find(ticket);
+  if (it == srv->all.end()) return BT_E_INVALID;
+  const ServedJob& s = *it->second;
+  if (s.state != 2) return BT_E_BUSY;
+  if (n_samples * 12 != s.points.size()) return BT_E_INVALID;
+  std::memcpy(rec12, s.points.data(), s.points.size());
+  return BT_OK;
+}
+
+int bt_server_read_density(bt_server* srv, uint64_t ticket, uint32_t kind, void* out,
+                           size_t n_pixels) {
+  if (!srv || !out || kind != 3) return BT_E_INVALID;
+  auto it = srv->all.find(ticket);
+  if (it == srv->all.end()) return BT_E_INVALID;
+  const ServedJob& s = *it->second;
+  if (s.state != 2) return BT_E_BUSY;
+  if (n_pixels != s.density.size()) return BT_E_INVALID;
+  std::memcpy(out, s.density.data(), n_pixels * 4);
+  return BT_OK;
+}
+
+int bt_server_release(bt_server* srv, uint64_t ticket) {
+  if (!srv) return BT_E_INVALID;
+  auto it = srv->all.find(ticket);
+  if (it == srv->all.end()) return BT_E_INVALID;
+  if (it->second->state < 2) return BT_E_BUSY;
+  srv->all.erase(it);
+  return BT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// Host-side helpers shared by the runtime: errors, device buffers, the
+// per-iteration parameter model and the reference's host-side algorithms.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/bundling.h"
+
+namespace bt {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define HIPCHK(x) ::bt::hip_check((x), #x)
+
+// Owning device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t b);
+  // Grows (never shrinks); returns true if the address changed.
+  bool reserve(size_t b);
+  void release();
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Options with the reference defaults filled in (main.rs:25-90, gpu.rs:21).
+bt_options resolve_options(const bt_options* in);
+
+bool has_radius(const bt_options& o);
+
+// Parameters of iteration `it`; split = grid cells per radius (>= 1).
+void iter_params(const bt_options& o, uint32_t it, uint32_t split, bt_iter_params* p);
+
+// Number of distinct parameter sets of the schedule (reference mode: 1).
+uint32_t schedule_len(const bt_options& o);
+
+uint32_t blur_weights(float sigma, float* w, uint32_t max_taps);
+
+// graph.rs:79-91
+bool normalization(const float* bbox, float* out);
+
+// resampling.rs:72-89,321-397; returns BT_OK or an error code with message.
+int capacity_model(const bt_options& o, uint32_t n_points, uint32_t n_starts, uint64_t* capacity,
+                   uint64_t* reserved, std::string* err);
+
+// graph.rs:35-77
+int parse_trl(const char* text, size_t len, std::vector<float>& xy, std::vector<uint32_t>& starts,
+              float* bbox, std::string* err);
+
+// distortion.rs:118-216 (+ geometry.rs:28-73 layout rules)
+int distortion(const float* original, const float* current, const uint32_t* rec3,
+               uint32_t n_paths, uint32_t n_samples, uint32_t map_size, double* out,
+               std::string* err);
+
+}  // namespace bt

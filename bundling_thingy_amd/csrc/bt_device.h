@@ -1,0 +1,91 @@
+// Device-side data layout shared by the HIP kernels and the C++ host.
+//
+// One bundling job (one uploaded graph) owns its device state. A launch covers
+// any number of jobs through a job table; each job's work items (points, cells,
+// pixels, tiles) occupy a contiguous range of workgroups given by a per-kind
+// block-base array, so a workgroup never straddles two jobs. The plain drop-in
+// runs a table of one job; the batched and served forms run many.
+#pragma once
+#include <stdint.h>
+
+#define BT_FIXED_BIT 0x80000000u   // sample is a path endpoint (never moves)
+#define BT_MAX_TAPS 64             // blur half-width limit (sigma <= 21 px)
+#define BT_FIX_ONE 16777216.0f     // 2^24: fixed-point quantum of neighbor sums
+#define BT_TASK_POINTS 64          // query points per neighbor-mean task (one wave)
+
+// Block granularities (work items per workgroup) of the batched kernels.
+#define BT_BIN_BLOCK 256           // points per workgroup in bin/scatter
+#define BT_SCAN_ITEMS 4096         // cells per workgroup in the cell scan (256 x 16)
+#define BT_TASK_CELLS 1024         // cells per workgroup in task emission (256 x 4)
+#define BT_CLEAR_ITEMS 4096        // words per workgroup in the clear kernel
+#define BT_ROW_TILE_W 64           // row blur tile: 64 columns x 4 rows
+#define BT_ROW_TILE_H 4
+#define BT_COL_TILE_W 64           // column blur tile: 64 columns x 32 rows
+#define BT_COL_TILE_H 32
+#define BT_NORM_ITEMS 1024         // pixels per workgroup in normalize
+#define BT_EMIT_PATHS 4            // paths per workgroup in resample_emit (one wave each)
+
+struct __attribute__((aligned(16))) BtJob {
+  // ---- per-job device state (constant for the job's lifetime)
+  float* rec[2];          // ping-pong sample records {x, y, id} (12 B each), slot order
+  float4* sorted;         // samples sorted by grid cell: {x, y, fx bits, fy bits}
+  uint32_t* sid;          // id of each sorted sample (original index | BT_FIXED_BIT)
+  uint2* cellrank;        // per slot: fine grid cell and rank within the cell
+  uint32_t* cells;        // cell counts, then exclusive starts (max_cells + 1)
+  uint32_t* bsum;         // per-scan-block partial sums (cell scan)
+  uint32_t* edge_of;      // per original index: path (edge) id
+  uint32_t* dens_cnt;     // density counts (u32) W*H
+  float* dens_row;        // horizontally blurred density
+  float* dens_col;        // vertically blurred density
+  float* dens_norm;       // normalized density in [0, 1]
+  uint32_t* peak;         // [2] f32 bits of the blurred peak (double-buffered)
+  uint32_t* status;       // [4] error flags (resample count/emit mismatch, ...)
+  // ---- resampling inputs (upload only)
+  const float* ctrl;      // control points (x, y)
+  const uint32_t* starts; // path starts (n_paths + 1)
+  uint64_t* path_scan;    // per path: (count << 32) | valid, then exclusive scan
+  uint32_t* paths;        // per path: {point_offset, segment_offset, count}
+  // ---- sizes
+  uint32_t n;             // samples
+  uint32_t n_paths;
+  uint32_t dw, dh;        // density size
+  // ---- per-iteration parameters
+  uint32_t gx, gy;        // grid dims (cells)
+  uint32_t n_cells;       // gx * gy
+  uint32_t taps;          // blur half-width
+  float res_x, res_y;     // texel resolution per axis (reference: map size - 1)
+  float ts_x, ts_y;       // texel scale = res * 0.5
+  float cell;             // cell side in texels
+  float r2;               // radius^2 (texels^2)
+  float reach_x, reach_y; // window half-width: radius + 4 * pad
+  float norm_min_x, norm_min_y, norm_scale, norm_tr_x, norm_tr_y;  // Graph::normalization
+  uint32_t parity;        // which rec[] holds the current positions
+  uint32_t capacity;      // sample capacity (reference budget model)
+  const float* wt;        // blur weights [taps + 1]
+};
+
+// Per-launch work description: for each kernel kind, the first workgroup of
+// each job (njobs + 1 entries, last = total workgroups).
+enum BtKind {
+  BT_K_CLEAR = 0,
+  BT_K_BIN,
+  BT_K_SCAN,
+  BT_K_TASK,
+  BT_K_ROW,
+  BT_K_COL,
+  BT_K_NORM,
+  BT_K_PATH,     // resample count: 256 paths per workgroup
+  BT_K_EMIT,     // resample emit: BT_EMIT_PATHS paths per workgroup
+  BT_K_PSCAN,    // path scan: BT_SCAN_ITEMS paths per workgroup
+  BT_K_COUNT
+};
+
+struct BtControl {
+  uint32_t task_total;    // tasks emitted this iteration
+  uint32_t queue_head;    // work-queue cursor of the neighbor-mean kernel
+  uint32_t pad[2];
+};
+
+struct __attribute__((aligned(16))) BtTask {
+  uint32_t job, start, end, pad;
+};

@@ -1,0 +1,633 @@
+// Device runtime. Mirrors the reference's Gpu::upload / Gpu::iterate
+// (gpu.rs:492-553), Resampler::encode (resampling.rs:186-290),
+// NeighborMean::encode (neighbor_mean.rs:200-289) and Density::encode
+// (density.rs:155-166), restructured around batched launches over a job table.
+#include "bt_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "bt_launch.h"
+
+namespace bt {
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+static inline size_t align_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+static const uint32_t kWt = BT_MAX_TAPS + 1;  // weights per iteration row
+
+BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
+  BtJob d;
+  std::memset(&d, 0, sizeof d);
+  d.rec[0] = rec0.as<float>();
+  d.rec[1] = rec1.as<float>();
+  d.sorted = sorted.as<float4>();
+  d.sid = sid.as<uint32_t>();
+  d.cellrank = cellrank.as<uint2>();
+  d.cells = cells.as<uint32_t>();
+  d.bsum = bsum.as<uint32_t>();
+  d.edge_of = edge_of.as<uint32_t>();
+  d.dens_cnt = dens_cnt.as<uint32_t>();
+  d.dens_row = dens_row.as<float>();
+  d.dens_col = dens_col.as<float>();
+  d.dens_norm = dens_norm.as<float>();
+  d.peak = peak.as<uint32_t>();
+  d.status = status.as<uint32_t>();
+  d.ctrl = ctrl.as<float>();
+  d.starts = starts.as<uint32_t>();
+  d.path_scan = path_scan.as<uint64_t>();
+  d.paths = paths.as<uint32_t>();
+  d.n = n;
+  d.n_paths = n_paths;
+  const bool density = dens_cnt.p != nullptr;
+  d.dw = density ? dw : 0;
+  d.dh = density ? dh : 0;
+  const bt_iter_params& p = at(it);
+  d.gx = p.gx;
+  d.gy = p.gy;
+  d.n_cells = p.gx * p.gy;
+  d.taps = p.taps;
+  d.res_x = p.res_x;
+  d.res_y = p.res_y;
+  d.ts_x = p.res_x * 0.5f;
+  d.ts_y = p.res_y * 0.5f;
+  d.cell = p.cell;
+  d.r2 = p.radius * p.radius;
+  const float pad_x = (p.res_x + 1.0f) * 2.4e-7f, pad_y = (p.res_y + 1.0f) * 2.4e-7f;
+  d.reach_x = p.radius + pad_x * 4.0f;
+  d.reach_y = p.radius + pad_y * 4.0f;
+  d.norm_min_x = norm[0];
+  d.norm_min_y = norm[1];
+  d.norm_scale = norm[2];
+  d.norm_tr_x = norm[3];
+  d.norm_tr_y = norm[4];
+  d.parity = parity_at;
+  d.capacity = (uint32_t)std::min<uint64_t>(capacity, 0xfffffffeull);
+  d.wt = wt.as<float>() ? wt.as<float>() + (size_t)std::min(it, sched - 1) * kWt : nullptr;
+  return d;
+}
+
+size_t Job::device_bytes() const {
+  const DevBuf* all[] = {&ctrl, &starts, &path_scan, &paths, &bsum, &status, &rec0,
+                         &rec1, &rec_saved, &sorted, &sid, &cellrank, &cells, &edge_of,
+                         &dens_cnt, &dens_row, &dens_col, &dens_norm, &dens_saved, &peak, &wt};
+  size_t s = 0;
+  for (const DevBuf* b : all) s += b->bytes;
+  return s;
+}
+
+Engine::Engine(const bt_options& o) : opts_(o) {
+  device_ = o.device;
+  HIPCHK(hipSetDevice(device_));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device_));
+  cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  ctl_.alloc(sizeof(BtControl));
+  HIPCHK(hipMemset(ctl_.p, 0, sizeof(BtControl)));
+}
+
+Engine::~Engine() {
+  drop_graphs();
+  if (stream_) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+void Engine::sync() { HIPCHK(hipStreamSynchronize(stream_)); }
+
+void Engine::drop_graphs() {
+  if (!graphs_.empty() && stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& kv : graphs_) {
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+  }
+  graphs_.clear();
+  graph_bufs_.clear();
+}
+
+Job& Engine::job(uint32_t i) {
+  if (i >= jobs_.size()) fail(jobs_.empty() ? BT_E_NO_GRAPH : BT_E_INVALID,
+                              jobs_.empty() ? "no uploaded graph" : "job index out of range");
+  return *jobs_[i];
+}
+
+std::vector<Job*> Engine::all_jobs() {
+  std::vector<Job*> v;
+  for (auto& j : jobs_) v.push_back(j.get());
+  return v;
+}
+
+size_t Engine::device_bytes() const {
+  size_t s = ctl_.bytes + tasks_.bytes + plans_.bytes + staging_.bytes + staging2_.bytes;
+  for (auto& j : jobs_) s += j->device_bytes();
+  for (auto& g : graph_bufs_) s += g.second->bytes;
+  return s;
+}
+
+// Workgroups each job needs for each kernel kind.
+static uint32_t kind_blocks(int k, const Job& j, const BtJob& d) {
+  const uint64_t npix = (uint64_t)d.dw * d.dh;
+  switch (k) {
+    case BT_K_CLEAR: return cdiv(std::max<uint64_t>(d.n_cells + 1ull, npix), BT_CLEAR_ITEMS);
+    case BT_K_BIN: return cdiv(j.n, BT_BIN_BLOCK);
+    case BT_K_SCAN: return cdiv(d.n_cells + 1ull, BT_SCAN_ITEMS);
+    case BT_K_TASK: return cdiv(d.n_cells, BT_TASK_CELLS);
+    case BT_K_ROW: return cdiv(d.dw, BT_ROW_TILE_W) * cdiv(d.dh, BT_ROW_TILE_H);
+    case BT_K_COL: return cdiv(d.dw, BT_COL_TILE_W) * cdiv(d.dh, BT_COL_TILE_H);
+    case BT_K_NORM: return cdiv(npix, BT_NORM_ITEMS);
+    case BT_K_PATH: return cdiv(j.n_paths + 1ull, 256);
+    case BT_K_EMIT: return cdiv(j.n_paths, BT_EMIT_PATHS);
+    case BT_K_PSCAN: return cdiv(j.n_paths + 1ull, BT_SCAN_ITEMS);
+  }
+  return 0;
+}
+
+Plan Engine::make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>& iters) const {
+  Plan p;
+  const uint32_t nj = (uint32_t)js.size();
+  p.njobs = nj;
+  p.base_off = align_up(nj * sizeof(BtJob), 256);
+  p.host.assign(p.base_off + (size_t)BT_K_COUNT * (nj + 1) * 4, 0);
+  BtJob* J = reinterpret_cast<BtJob*>(p.host.data());
+  uint32_t* B = reinterpret_cast<uint32_t*>(p.host.data() + p.base_off);
+  for (uint32_t i = 0; i < nj; ++i) {
+    const uint32_t it = iters[i];
+    J[i] = js[i]->dev(it, (js[i]->parity + (it - js[i]->iteration)) & 1u);
+  }
+  for (int k = 0; k < BT_K_COUNT; ++k) {
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < nj; ++i) {
+      B[k * (nj + 1) + i] = (uint32_t)acc;
+      acc += kind_blocks(k, *js[i], J[i]);
+    }
+    if (acc > 0x7fffffffull) fail(BT_E_CAPACITY, "launch exceeds the workgroup limit");
+    B[k * (nj + 1) + nj] = (uint32_t)acc;
+    p.grid[k] = (uint32_t)acc;
+  }
+  return p;
+}
+
+void Engine::put_plan(const Plan& p, void* dst) {
+  HIPCHK(hipMemcpyAsync(dst, p.host.data(), p.host.size(), hipMemcpyHostToDevice, stream_));
+}
+
+void Engine::ensure_tasks(const std::vector<Job*>& js) {
+  uint64_t t = 0;
+  for (Job* j : js) t += j->n / BT_TASK_POINTS + 1 + std::min<uint64_t>(j->n, j->max_cells);
+  if ((size_t)t * sizeof(BtTask) > tasks_.bytes) {
+    drop_graphs();
+    sync();
+    tasks_.reserve((size_t)t * sizeof(BtTask) + (1 << 16));
+  }
+}
+
+void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool density,
+                              hipEvent_t* ev) {
+  const BtJob* J = reinterpret_cast<const BtJob*>(dev);
+  const uint32_t nj = p.njobs;
+  auto B = [&](int k) {
+    return reinterpret_cast<const uint32_t*>(dev + p.base_off) + (size_t)k * (nj + 1);
+  };
+  BtControl* ctl = ctl_.as<BtControl>();
+  auto mark = [&](int i) {
+    if (ev) HIPCHK(hipEventRecord(ev[i], stream_));
+  };
+  mark(0);
+  btk::clear(stream_, p.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)nj, ctl, 1);
+  mark(1);
+  btk::bin(stream_, p.grid[BT_K_BIN], J, B(BT_K_BIN), (int)nj);
+  mark(2);
+  btk::cell_scan(stream_, p.grid[BT_K_SCAN], J, B(BT_K_SCAN), (int)nj);
+  mark(3);
+  btk::tasks(stream_, p.grid[BT_K_TASK], J, B(BT_K_TASK), (int)nj, ctl, tasks_.as<BtTask>());
+  mark(4);
+  btk::scatter(stream_, p.grid[BT_K_BIN], J, B(BT_K_BIN), (int)nj);
+  mark(5);
+  btk::neighbor_mean(stream_, (uint32_t)cus_ * 8u, J, tasks_.as<BtTask>(), ctl);
+  mark(6);
+  if (density && p.grid[BT_K_ROW]) {
+    btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj);
+    mark(7);
+    btk::blur_cols(stream_, p.grid[BT_K_COL], J, B(BT_K_COL), (int)nj);
+    mark(8);
+    btk::normalize(stream_, p.grid[BT_K_NORM], J, B(BT_K_NORM), (int)nj);
+    mark(9);
+  } else if (ev) {
+    mark(7);
+    mark(8);
+    mark(9);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+void Engine::upload(const std::vector<GraphIn>& gs, std::vector<uint32_t>* samples) {
+  sync();
+  drop_graphs();
+  jobs_.clear();
+  ++generation_;
+  std::vector<std::unique_ptr<Job>> out;
+  upload_jobs(gs, &out);
+  jobs_ = std::move(out);
+  if (samples) {
+    samples->clear();
+    for (auto& j : jobs_) samples->push_back(j->n);
+  }
+}
+
+void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out) {
+  std::vector<std::unique_ptr<Job>> nj;
+  std::vector<Job*> js;
+  for (const GraphIn& g : gs) {
+    auto j = std::make_unique<Job>();
+    j->o = resolve_options(&g.opts);
+    if (!g.xy || !g.starts || g.n_starts < 2 || g.n_points < 2)
+      fail(BT_E_INVALID, "graph has no drawable edges");
+    for (uint32_t i = 0; i + 1 < g.n_starts; ++i)
+      if (g.starts[i] > g.starts[i + 1]) fail(BT_E_INVALID, "path starts are not sorted");
+    if (g.starts[g.n_starts - 1] > g.n_points) fail(BT_E_INVALID, "path starts exceed the points");
+    float bbox[4] = {3.40282347e38f, 3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+    for (uint32_t i = 0; i < g.n_points; ++i) {
+      const float x = g.xy[2 * i], y = g.xy[2 * i + 1];
+      if (!std::isfinite(x) || !std::isfinite(y)) fail(BT_E_INVALID, "non-finite TRL coordinate");
+      bbox[0] = std::min(bbox[0], x);
+      bbox[1] = std::min(bbox[1], y);
+      bbox[2] = std::max(bbox[2], x);
+      bbox[3] = std::max(bbox[3], y);
+    }
+    if (!normalization(bbox, j->norm)) fail(BT_E_INVALID, "graph has invalid extent");
+    std::string e;
+    int rc = capacity_model(j->o, g.n_points, g.n_starts, &j->capacity, &j->reserved, &e);
+    if (rc != BT_OK) fail(rc, e);
+    j->n_ctrl = g.n_points;
+    j->n_paths = g.n_starts - 1;
+    j->split = j->o.cell_split ? j->o.cell_split : 1;
+    j->sched = schedule_len(j->o);
+    j->params.resize(j->sched);
+    std::vector<float> wt((size_t)j->sched * kWt, 0.0f);
+    for (uint32_t i = 0; i < j->sched; ++i) {
+      iter_params(j->o, i, j->split, &j->params[i]);
+      j->max_cells = std::max(j->max_cells, j->params[i].gx * j->params[i].gy);
+      blur_weights(j->params[i].sigma, wt.data() + (size_t)i * kWt, BT_MAX_TAPS);
+    }
+    j->dw = j->params[0].density_w;
+    j->dh = j->params[0].density_h;
+    if (j->dw < 1 || j->dh < 1) fail(BT_E_INVALID, "density map dimensions must be nonzero");
+    if (j->max_cells + 1ull > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS)
+      fail(BT_E_CAPACITY, "neighbor grid exceeds GPU dispatch limits");
+    if (j->n_paths + 1ull > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS)
+      fail(BT_E_CAPACITY, "paths exceed GPU dispatch limits");
+    j->ctrl.alloc((size_t)g.n_points * 8);
+    j->starts.alloc((size_t)g.n_starts * 4);
+    j->path_scan.alloc((size_t)g.n_starts * 8);
+    j->paths.alloc((size_t)std::max(j->n_paths, 1u) * 12);
+    j->status.alloc(16);
+    const uint32_t nb = std::max(cdiv(j->n_paths + 1ull, BT_SCAN_ITEMS),
+                                 cdiv(j->max_cells + 1ull, BT_SCAN_ITEMS));
+    j->bsum.alloc((size_t)nb * 8);
+    j->wt.alloc(wt.size() * 4);
+    HIPCHK(hipMemcpyAsync(j->ctrl.p, g.xy, (size_t)g.n_points * 8, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(j->starts.p, g.starts, (size_t)g.n_starts * 4, hipMemcpyHostToDevice,
+                          stream_));
+    HIPCHK(hipMemcpyAsync(j->wt.p, wt.data(), wt.size() * 4, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemsetAsync(j->status.p, 0, 16, stream_));
+    js.push_back(j.get());
+    nj.push_back(std::move(j));
+  }
+  if (js.empty()) fail(BT_E_INVALID, "no graphs");
+  // Pass 1: per-path sample counts and their exclusive scan (count_paths,
+  // resampling.wgsl:138-159, with offsets from a scan instead of atomics).
+  std::vector<uint32_t> zeros(js.size(), 0);
+  Plan p1 = make_plan(js, zeros);
+  DevBuf table;
+  table.alloc(p1.host.size());
+  put_plan(p1, table.p);
+  {
+    const unsigned char* dev = table.as<unsigned char>();
+    const BtJob* J = reinterpret_cast<const BtJob*>(dev);
+    auto B = [&](int k) {
+      return reinterpret_cast<const uint32_t*>(dev + p1.base_off) + (size_t)k * (p1.njobs + 1);
+    };
+    btk::resample_count(stream_, p1.grid[BT_K_PATH], J, B(BT_K_PATH), (int)p1.njobs);
+    btk::path_scan(stream_, p1.grid[BT_K_PSCAN], J, B(BT_K_PSCAN), (int)p1.njobs);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<uint64_t> totals(js.size());
+  for (size_t i = 0; i < js.size(); ++i)
+    HIPCHK(hipMemcpyAsync(&totals[i], js[i]->path_scan.as<uint64_t>() + js[i]->n_paths, 8,
+                          hipMemcpyDeviceToHost, stream_));
+  sync();
+  for (size_t i = 0; i < js.size(); ++i) {
+    Job& j = *js[i];
+    const uint64_t n = totals[i] >> 32, valid = totals[i] & 0xffffffffull;
+    if (n > j.capacity)
+      fail(BT_E_CAPACITY, "GPU resampling exceeds the " + std::to_string(j.capacity) +
+                              "-sample buffer capacity; increase --max-graph-memory-mib or reduce "
+                              "the graph or render dimensions. Device buffer limits also apply.");
+    if (valid == 0 || n < 2) fail(BT_E_INVALID, "resampled graph has no drawable edges");
+    if (n > 0x7fffffffull) fail(BT_E_CAPACITY, "sample count exceeds 2^31");
+    j.n = (uint32_t)n;
+    j.valid_paths = (uint32_t)valid;
+    j.rec0.alloc((size_t)n * 12);
+    j.rec1.alloc((size_t)n * 12);
+    j.rec_saved.alloc((size_t)n * 12);
+    j.sorted.alloc((size_t)n * 16);
+    j.sid.alloc((size_t)n * 4);
+    j.cellrank.alloc((size_t)n * 8);
+    j.edge_of.alloc((size_t)n * 4);
+    j.cells.alloc(((size_t)j.max_cells + 1) * 4);
+    if (j.o.flags & BT_FLAG_DENSITY) {
+      const size_t pb = (size_t)j.dw * j.dh * 4;
+      j.dens_cnt.alloc(pb);
+      j.dens_row.alloc(pb);
+      j.dens_col.alloc(pb);
+      j.dens_norm.alloc(pb);
+      j.dens_saved.alloc(pb * 4);
+      j.peak.alloc(8);
+    }
+  }
+  // Pass 2: emit samples (emit_paths, resampling.wgsl:161-183) with the
+  // initial density splat, then the density chain (gpu.rs:512-514).
+  Plan p2 = make_plan(js, zeros);
+  put_plan(p2, table.p);
+  {
+    const unsigned char* dev = table.as<unsigned char>();
+    const BtJob* J = reinterpret_cast<const BtJob*>(dev);
+    const uint32_t n2 = p2.njobs;
+    auto B = [&](int k) {
+      return reinterpret_cast<const uint32_t*>(dev + p2.base_off) + (size_t)k * (n2 + 1);
+    };
+    btk::clear(stream_, p2.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)n2, ctl_.as<BtControl>(), 1);
+    btk::resample_emit(stream_, p2.grid[BT_K_EMIT], J, B(BT_K_EMIT), (int)n2);
+    if (p2.grid[BT_K_ROW]) {
+      btk::blur_rows(stream_, p2.grid[BT_K_ROW], J, B(BT_K_ROW), (int)n2);
+      btk::blur_cols(stream_, p2.grid[BT_K_COL], J, B(BT_K_COL), (int)n2);
+      btk::normalize(stream_, p2.grid[BT_K_NORM], J, B(BT_K_NORM), (int)n2);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<uint32_t> st(js.size() * 4);
+  for (size_t i = 0; i < js.size(); ++i)
+    HIPCHK(hipMemcpyAsync(&st[i * 4], js[i]->status.p, 16, hipMemcpyDeviceToHost, stream_));
+  for (Job* j : js) {
+    HIPCHK(hipMemcpyAsync(j->rec_saved.p, j->rec0.p, (size_t)j->n * 12, hipMemcpyDeviceToDevice,
+                          stream_));
+    if (j->dens_cnt.p) {
+      const size_t pb = (size_t)j->dw * j->dh * 4;
+      char* s = j->dens_saved.as<char>();
+      HIPCHK(hipMemcpyAsync(s, j->dens_cnt.p, pb, hipMemcpyDeviceToDevice, stream_));
+      HIPCHK(hipMemcpyAsync(s + pb, j->dens_row.p, pb, hipMemcpyDeviceToDevice, stream_));
+      HIPCHK(hipMemcpyAsync(s + 2 * pb, j->dens_col.p, pb, hipMemcpyDeviceToDevice, stream_));
+      HIPCHK(hipMemcpyAsync(s + 3 * pb, j->dens_norm.p, pb, hipMemcpyDeviceToDevice, stream_));
+    }
+  }
+  sync();
+  for (size_t i = 0; i < js.size(); ++i)
+    if (st[i * 4] & 2u) fail(BT_E_MISMATCH, "GPU resampling count and write passes disagree");
+  for (auto& j : nj) {
+    j->iteration = 0;
+    j->parity = 0;
+    out->push_back(std::move(j));
+  }
+}
+
+void Engine::rewind() {
+  if (jobs_.empty()) fail(BT_E_NO_GRAPH, "no uploaded graph");
+  for (auto& j : jobs_) {
+    HIPCHK(hipMemcpyAsync(j->rec0.p, j->rec_saved.p, (size_t)j->n * 12, hipMemcpyDeviceToDevice,
+                          stream_));
+    if (j->dens_cnt.p) {
+      const size_t pb = (size_t)j->dw * j->dh * 4;
+      const char* s = j->dens_saved.as<char>();
+      HIPCHK(hipMemcpyAsync(j->dens_cnt.p, s, pb, hipMemcpyDeviceToDevice, stream_));
+      HIPCHK(hipMemcpyAsync(j->dens_row.p, s + pb, pb, hipMemcpyDeviceToDevice, stream_));
+      HIPCHK(hipMemcpyAsync(j->dens_col.p, s + 2 * pb, pb, hipMemcpyDeviceToDevice, stream_));
+      HIPCHK(hipMemcpyAsync(j->dens_norm.p, s + 3 * pb, pb, hipMemcpyDeviceToDevice, stream_));
+    }
+    j->iteration = 0;
+    j->parity = 0;
+  }
+  sync();
+}
+
+static void check_iterable(const std::vector<Job*>& js) {
+  if (js.empty()) fail(BT_E_NO_GRAPH, "no uploaded graph");
+  for (Job* j : js)
+    if (!has_radius(j->o)) fail(BT_E_STATE, "neighbor radius is not configured");
+}
+
+void Engine::iterate_jobs(const std::vector<Job*>& js) {
+  check_iterable(js);
+  ensure_tasks(js);
+  std::vector<uint32_t> it;
+  bool density = false;
+  for (Job* j : js) {
+    it.push_back(j->iteration);
+    density |= j->dens_cnt.p != nullptr;
+  }
+  Plan p = make_plan(js, it);
+  const size_t slot = align_up(p.host.size(), 256);
+  if (2 * slot > plans_.bytes) {
+    sync();
+    plans_.reserve(2 * slot + 4096);
+  }
+  slot_bytes_ = slot;
+  eager_slot_ ^= 1u;
+  unsigned char* dev = plans_.as<unsigned char>() + eager_slot_ * slot;
+  put_plan(p, dev);
+  launch_iteration(p, dev, density);
+  for (Job* j : js) {
+    ++j->iteration;
+    j->parity ^= 1u;
+  }
+  sync();
+}
+
+void Engine::iterate(uint32_t n, bool use_graph) {
+  std::vector<Job*> js = all_jobs();
+  check_iterable(js);
+  if (n == 0) return;
+  ensure_tasks(js);
+  bool density = false;
+  for (Job* j : js) density |= j->dens_cnt.p != nullptr;
+  if (!use_graph) {
+    std::vector<Plan> keep;
+    for (uint32_t i = 0; i < n; ++i) {
+      std::vector<uint32_t> it;
+      for (Job* j : js) it.push_back(j->iteration);
+      keep.push_back(make_plan(js, it));
+      const Plan& p = keep.back();
+      const size_t slot = align_up(p.host.size(), 256);
+      if (2 * slot > plans_.bytes) {
+        sync();
+        plans_.reserve(2 * slot + 4096);
+      }
+      eager_slot_ ^= 1u;
+      unsigned char* dev = plans_.as<unsigned char>() + eager_slot_ * slot;
+      put_plan(p, dev);
+      launch_iteration(p, dev, density);
+      for (Job* j : js) {
+        ++j->iteration;
+        j->parity ^= 1u;
+      }
+      if (keep.size() > 4) {
+        sync();
+        keep.clear();
+      }
+    }
+    sync();
+    return;
+  }
+  std::vector<uint64_t> key = {generation_, n};
+  for (Job* j : js) key.push_back(((uint64_t)j->iteration << 1) | j->parity);
+  auto found = graphs_.find(key);
+  if (found == graphs_.end()) {
+    std::vector<Plan> plans;
+    size_t slot = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      std::vector<uint32_t> it;
+      for (Job* j : js) it.push_back(j->iteration + i);
+      plans.push_back(make_plan(js, it));
+      slot = std::max(slot, align_up(plans.back().host.size(), 256));
+    }
+    auto buf = std::make_unique<DevBuf>();
+    buf->alloc(slot * n);
+    for (uint32_t i = 0; i < n; ++i)
+      put_plan(plans[i], buf->as<unsigned char>() + i * slot);
+    sync();
+    GraphEntry g;
+    HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+    for (uint32_t i = 0; i < n; ++i)
+      launch_iteration(plans[i], buf->as<unsigned char>() + i * slot, density);
+    HIPCHK(hipStreamEndCapture(stream_, &g.graph));
+    HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    found = graphs_.emplace(key, g).first;
+    graph_bufs_.emplace(key, std::move(buf));
+  }
+  HIPCHK(hipGraphLaunch(found->second.exec, stream_));
+  for (Job* j : js) {
+    j->iteration += n;
+    j->parity ^= (n & 1u);
+  }
+  sync();
+}
+
+void Engine::profile(uint32_t n, double* stage_ms, uint32_t nst, double* total_ms) {
+  std::vector<Job*> js = all_jobs();
+  check_iterable(js);
+  ensure_tasks(js);
+  bool density = false;
+  for (Job* j : js) density |= j->dens_cnt.p != nullptr;
+  hipEvent_t ev[10];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  double acc[9] = {0};
+  double tot = 0.0;
+  try {
+    for (uint32_t i = 0; i < n; ++i) {
+      std::vector<uint32_t> it;
+      for (Job* j : js) it.push_back(j->iteration);
+      Plan p = make_plan(js, it);
+      const size_t slot = align_up(p.host.size(), 256);
+      if (2 * slot > plans_.bytes) {
+        sync();
+        plans_.reserve(2 * slot + 4096);
+      }
+      eager_slot_ ^= 1u;
+      unsigned char* dev = plans_.as<unsigned char>() + eager_slot_ * slot;
+      put_plan(p, dev);
+      launch_iteration(p, dev, density, ev);
+      sync();
+      for (int s = 0; s < 9; ++s) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+        acc[s] += ms;
+      }
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[9]));
+      tot += ms;
+      for (Job* j : js) {
+        ++j->iteration;
+        j->parity ^= 1u;
+      }
+    }
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (uint32_t s = 0; s < nst && s < 9; ++s) stage_ms[s] = acc[s];
+  if (total_ms) *total_ms = tot;
+}
+
+void Engine::read_paths(uint32_t job_i, uint32_t* rec3, size_t n) {
+  Job& j = job(job_i);
+  if (n != j.n_paths) fail(BT_E_INVALID, "path count mismatch");
+  HIPCHK(hipMemcpyAsync(rec3, j.paths.p, (size_t)n * 12, hipMemcpyDeviceToHost, stream_));
+  sync();
+}
+
+void Engine::read_points_of(Engine& e, Job& j, void* rec12, size_t n) {
+  if (n != j.n) fail(BT_E_INVALID, "sample count mismatch");
+  e.staging_.reserve((size_t)j.n * 12);
+  std::vector<Job*> js = {&j};
+  Plan p = e.make_plan(js, {j.iteration});
+  DevBuf table;
+  table.alloc(p.host.size());
+  e.put_plan(p, table.p);
+  const unsigned char* dev = table.as<unsigned char>();
+  btk::unpermute(e.stream_, p.grid[BT_K_BIN], reinterpret_cast<const BtJob*>(dev),
+                 reinterpret_cast<const uint32_t*>(dev + p.base_off) + BT_K_BIN * 2, 1,
+                 e.staging_.as<float>(), nullptr, 0u);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rec12, e.staging_.p, (size_t)j.n * 12, hipMemcpyDeviceToHost, e.stream_));
+  e.sync();
+}
+
+void Engine::read_points(uint32_t job_i, void* rec12, size_t n) {
+  read_points_of(*this, job(job_i), rec12, n);
+}
+
+void Engine::read_density_of(Engine& e, Job& j, uint32_t kind, void* out, size_t n) {
+  if (!j.dens_cnt.p) fail(BT_E_STATE, "density is disabled (color_by != density)");
+  if (n != (size_t)j.dw * j.dh) fail(BT_E_INVALID, "pixel count mismatch");
+  const DevBuf* src = kind == 0 ? &j.dens_cnt : kind == 1 ? &j.dens_row
+                    : kind == 2 ? &j.dens_col : kind == 3 ? &j.dens_norm : nullptr;
+  if (!src) fail(BT_E_INVALID, "unknown density kind");
+  HIPCHK(hipMemcpyAsync(out, src->p, n * 4, hipMemcpyDeviceToHost, e.stream_));
+  e.sync();
+}
+
+void Engine::read_density(uint32_t job_i, uint32_t kind, void* out, size_t n) {
+  read_density_of(*this, job(job_i), kind, out, n);
+}
+
+void Engine::read_grid(uint32_t job_i, uint32_t* cell_of, size_t n, uint32_t* counts,
+                       size_t ncells) {
+  Job& j = job(job_i);
+  if (j.iteration == 0) fail(BT_E_STATE, "no iteration has binned the samples yet");
+  if (n != j.n) fail(BT_E_INVALID, "sample count mismatch");
+  const bt_iter_params& p = j.at(j.iteration - 1);
+  const size_t nc = (size_t)p.gx * p.gy;
+  if (ncells != nc) fail(BT_E_INVALID, "cell count mismatch");
+  staging_.reserve((size_t)j.n * 12);
+  staging2_.reserve((size_t)j.n * 4);
+  std::vector<Job*> js = {&j};
+  Plan pl = make_plan(js, {j.iteration});
+  DevBuf table;
+  table.alloc(pl.host.size());
+  put_plan(pl, table.p);
+  const unsigned char* dev = table.as<unsigned char>();
+  btk::unpermute(stream_, pl.grid[BT_K_BIN], reinterpret_cast<const BtJob*>(dev),
+                 reinterpret_cast<const uint32_t*>(dev + pl.base_off) + BT_K_BIN * 2, 1,
+                 staging_.as<float>(), staging2_.as<uint32_t>(), 1u);
+  HIPCHK(hipGetLastError());
+  std::vector<uint32_t> starts(nc + 1);
+  HIPCHK(hipMemcpyAsync(cell_of, staging2_.p, (size_t)j.n * 4, hipMemcpyDeviceToHost, stream_));
+  HIPCHK(hipMemcpyAsync(starts.data(), j.cells.p, (nc + 1) * 4, hipMemcpyDeviceToHost, stream_));
+  sync();
+  for (size_t c = 0; c < nc; ++c) counts[c] = starts[c + 1] - starts[c];
+}
+
+}  // namespace bt

@@ -1,0 +1,108 @@
+// The device runtime: jobs (one uploaded graph each), per-iteration launch
+// plans, batched uploads and iterations, hipGraph capture, readbacks.
+#pragma once
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bt_common.h"
+#include "bt_device.h"
+
+namespace bt {
+
+// One bundling job: an uploaded graph and its device state.
+struct Job {
+  bt_options o{};
+  uint32_t n_ctrl = 0, n_paths = 0, n = 0, valid_paths = 0;
+  uint64_t capacity = 0, reserved = 0;
+  float norm[5] = {0, 0, 0, 0, 0};
+  uint32_t dw = 0, dh = 0;
+  uint32_t max_cells = 1;
+  uint32_t split = 1;           // grid cells per radius
+  uint32_t iteration = 0, parity = 0;
+  uint32_t sched = 1;           // distinct parameter sets
+  std::vector<bt_iter_params> params;
+  DevBuf ctrl, starts, path_scan, paths, bsum, status;
+  DevBuf rec0, rec1, rec_saved, sorted, sid, cellrank, cells, edge_of;
+  DevBuf dens_cnt, dens_row, dens_col, dens_norm, dens_saved, peak, wt;
+
+  const bt_iter_params& at(uint32_t it) const { return params[it < sched ? it : sched - 1]; }
+  BtJob dev(uint32_t it, uint32_t parity_at) const;
+  size_t device_bytes() const;
+};
+
+struct Plan {
+  std::vector<unsigned char> host;
+  uint32_t grid[BT_K_COUNT] = {};
+  uint32_t njobs = 0;
+  size_t base_off = 0;  // byte offset of the bases block
+};
+
+// Input of one graph of an upload.
+struct GraphIn {
+  const float* xy;
+  uint32_t n_points;
+  const uint32_t* starts;
+  uint32_t n_starts;
+  bt_options opts;
+};
+
+class Engine {
+ public:
+  explicit Engine(const bt_options& o);
+  ~Engine();
+
+  const bt_options& options() const { return opts_; }
+  // Replaces the current jobs with the given graphs; returns sample counts.
+  void upload(const std::vector<GraphIn>& gs, std::vector<uint32_t>* samples);
+  // Uploads graphs as new jobs appended to `out` (used by the server).
+  void upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out);
+  void iterate(uint32_t n, bool use_graph);
+  // Advances an explicit job set by one iteration (server tick).
+  void iterate_jobs(const std::vector<Job*>& js);
+  void rewind();
+  void profile(uint32_t n, double* stage_ms, uint32_t nst, double* total_ms);
+
+  void read_paths(uint32_t job, uint32_t* rec3, size_t n);
+  void read_points(uint32_t job, void* rec12, size_t n);
+  void read_density(uint32_t job, uint32_t kind, void* out, size_t n);
+  void read_grid(uint32_t job, uint32_t* cell_of, size_t n, uint32_t* counts, size_t ncells);
+  static void read_points_of(Engine& e, Job& j, void* rec12, size_t n);
+  static void read_density_of(Engine& e, Job& j, uint32_t kind, void* out, size_t n);
+
+  size_t njobs() const { return jobs_.size(); }
+  Job& job(uint32_t i);
+  size_t device_bytes() const;
+  hipStream_t stream() const { return stream_; }
+  void sync();
+
+  std::string err;
+
+ private:
+  Plan make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>& iters) const;
+  void put_plan(const Plan& p, void* dst);
+  void launch_iteration(const Plan& p, const unsigned char* dev, bool density,
+                        hipEvent_t* ev = nullptr);
+  void ensure_tasks(const std::vector<Job*>& js);
+  void drop_graphs();
+  std::vector<Job*> all_jobs();
+
+  bt_options opts_;
+  int device_ = 0;
+  int cus_ = 256;
+  hipStream_t stream_ = nullptr;
+  std::vector<std::unique_ptr<Job>> jobs_;
+  DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_;
+  size_t slot_bytes_ = 0;
+  uint32_t eager_slot_ = 0;
+  uint64_t generation_ = 0;
+  struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+  };
+  std::map<std::vector<uint64_t>, GraphEntry> graphs_;
+  std::map<std::vector<uint64_t>, std::unique_ptr<DevBuf>> graph_bufs_;
+};
+
+}  // namespace bt

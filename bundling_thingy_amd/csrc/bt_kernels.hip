@@ -1,0 +1,835 @@
+// Hand-written gfx950 kernels for the edge-bundling hot path.
+//
+// Reference (lpstankus/bundling_thingy) kernels replaced here, per stage:
+//   resample_count / resample_emit  <- resampling.wgsl:138-183 (count_paths, emit_paths)
+//   bin / cell scan / tasks / scatter <- spatial_grid.wgsl:72-138 (count_cells,
+//                                      reserve_cells, scatter_points, prepare_dispatch)
+//   neighbor_mean                    <- neighbor_mean.wgsl:75-203 (tiled + sparse)
+//   blur_rows / blur_cols_peak       <- blur.wgsl:13-83
+//   normalize                        <- normalization.wgsl:5-17
+//
+// Arithmetic contract (identical to the CPU oracle): FP32 without contraction
+// (built with -ffp-contract=off; fmaf only where written), correctly rounded
+// '/' and sqrtf, rintf for WGSL round(). Neighbor sums are exact 2^-24
+// fixed-point int64 sums, so results do not depend on visiting order.
+#include <hip/hip_runtime.h>
+
+#include "bt_device.h"
+
+#define LANE() (threadIdx.x & 63u)
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const f4v cf4v;
+typedef __attribute__((address_space(4))) const uint32_t cu32;
+typedef __attribute__((address_space(4))) const float cf32;
+
+// Loads through the constant address space are eligible for scalar (SMEM)
+// loads when the address is wave-uniform; the buffers read this way are never
+// written by the kernel that reads them.
+__device__ __forceinline__ float4 ld_uniform(const float4* p, uint32_t i) {
+  const f4v v = ((cf4v*)(uintptr_t)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t ld_uniform(const uint32_t* p, uint32_t i) {
+  return ((cu32*)(uintptr_t)p)[i];
+}
+__device__ __forceinline__ float ld_uniform(const float* p, uint32_t i) {
+  return ((cf32*)(uintptr_t)p)[i];
+}
+
+__device__ __forceinline__ int find_job(const uint32_t* __restrict__ base, int njobs, uint32_t b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ld_uniform(base, (uint32_t)mid) <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// spatial_grid.wgsl:59-61
+__device__ __forceinline__ float texel(float p, float res) { return (p + 1.0f) * 0.5f * res; }
+
+// spatial_grid.wgsl:63-65, one axis; clamps before converting.
+__device__ __forceinline__ uint32_t cell_axis(float t, float cell, uint32_t g) {
+  float v = fmaxf(t / cell, 0.0f);
+  return v >= (float)(g - 1u) ? g - 1u : (uint32_t)v;
+}
+
+// neighbor_mean.wgsl:66-73 / resampling.wgsl:69-75 pixel of a sample (y flipped).
+__device__ __forceinline__ uint32_t splat_index(float x, float y, uint32_t w, uint32_t h) {
+  float ux = x * 0.5f + 0.5f, uy = 0.5f - y * 0.5f;
+  float fx = rintf(ux * (float)(w - 1u)), fy = rintf(uy * (float)(h - 1u));
+  fx = fminf(fmaxf(fx, 0.0f), (float)(w - 1u));
+  fy = fminf(fmaxf(fy, 0.0f), (float)(h - 1u));
+  return (uint32_t)fy * w + (uint32_t)fx;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const uint32_t lane = LANE();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T o = __shfl_up(v, d, 64);
+    if (lane >= (uint32_t)d) v += o;
+  }
+  return v;
+}
+
+// Exclusive scan across a 256-thread block; returns the exclusive prefix and
+// the block total. lds must hold 4 values.
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* lds, T* total) {
+  const uint32_t lane = LANE(), wave = threadIdx.x >> 6;
+  T inc = wave_incl_scan(v);
+  if (lane == 63) lds[wave] = inc;
+  __syncthreads();
+  T w0 = lds[0], w1 = lds[1], w2 = lds[2], w3 = lds[3];
+  T before = (wave > 0 ? w0 : T(0)) + (wave > 1 ? w1 : T(0)) + (wave > 2 ? w2 : T(0));
+  *total = w0 + w1 + w2 + w3;
+  __syncthreads();
+  return before + inc - v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+
+#define JOB_PROLOGUE(KIND)                                                   \
+  const int j = find_job(base, njobs, blockIdx.x);                          \
+  const uint32_t lb = blockIdx.x - ld_uniform(base, (uint32_t)j);           \
+  const BtJob& J = jobs[j];
+
+// ------------------------------------------------------------------ clear
+// Zeroes the cell counters (n_cells + 1 words), the density counts and the
+// peak of each job, and the launch's control block.
+extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restrict__ jobs,
+                                                           const uint32_t* __restrict__ base,
+                                                           int njobs, BtControl* ctl,
+                                                           int clear_density) {
+  JOB_PROLOGUE(BT_K_CLEAR);
+  const uint32_t ncell = J.n_cells + 1u, npix = J.dw * J.dh;
+  uint32_t* cells = J.cells;
+  uint32_t* dens = J.dens_cnt;
+  const uint32_t i0 = lb * BT_CLEAR_ITEMS + threadIdx.x;
+#pragma unroll 4
+  for (uint32_t k = 0; k < BT_CLEAR_ITEMS / 256; ++k) {
+    uint32_t i = i0 + k * 256u;
+    if (i < ncell) cells[i] = 0u;
+    if (clear_density && i < npix) dens[i] = 0u;
+  }
+  if (lb == 0 && threadIdx.x == 0) {
+    J.peak[0] = 0u;
+    if (j == 0) {
+      ctl->task_total = 0u;
+      ctl->queue_head = 0u;
+    }
+  }
+}
+
+// -------------------------------------------------------------------- bin
+// spatial_grid.wgsl:72-81 (count_cells) fused with the rank half of
+// scatter_points (:127-138): each sample gets its cell and its rank among the
+// cell's samples from a wave-aggregated atomic add. Ranks depend on scheduling,
+// which is harmless: every consumer is order-independent.
+extern "C" __global__ __launch_bounds__(256) void bt_bin(const BtJob* __restrict__ jobs,
+                                                         const uint32_t* __restrict__ base,
+                                                         int njobs) {
+  JOB_PROLOGUE(BT_K_BIN);
+  const uint32_t i = lb * BT_BIN_BLOCK + threadIdx.x;
+  const bool valid = i < J.n;
+  const float* rec = J.rec[J.parity];
+  uint32_t c = 0;
+  if (valid) {
+    float x = rec[3u * i], y = rec[3u * i + 1u];
+    uint32_t cx = cell_axis(texel(x, J.res_x), J.cell, J.gx);
+    uint32_t cy = cell_axis(texel(y, J.res_y), J.cell, J.gy);
+    c = cy * J.gx + cx;
+  }
+  uint32_t* cells = J.cells;
+  uint64_t active = ballot(valid);
+  uint32_t rank = 0;
+  const uint32_t lane = LANE();
+  while (active) {
+    const int leader = __builtin_ctzll(active);
+    const uint32_t key = __builtin_amdgcn_readlane(c, leader);
+    const bool mine = valid && c == key;
+    const uint64_t same = ballot(mine);
+    uint32_t b = 0;
+    if (lane == (uint32_t)leader) b = atomicAdd(&cells[key], (uint32_t)__popcll(same));
+    b = __builtin_amdgcn_readlane(b, leader);
+    if (mine) rank = b + mbcnt64(same);
+    active &= ~same;
+  }
+  if (valid) J.cellrank[i] = make_uint2(c, rank);
+}
+
+// -------------------------------------------------------------- cell scan
+// Three-phase exclusive scan of each job's n_cells + 1 counters (the last one
+// is zero, so it becomes the total). Replaces reserve_cells
+// (spatial_grid.wgsl:86-125), whose atomically ordered block bases are not
+// deterministic.
+template <typename T>
+__device__ __forceinline__ void load_block(const T* src, uint32_t n, uint32_t first, T* lds) {
+  for (uint32_t k = threadIdx.x; k < BT_SCAN_ITEMS; k += 256u) {
+    uint32_t i = first + k;
+    lds[k] = i < n ? src[i] : T(0);
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void scan_reduce_body(const T* src, uint32_t n, uint32_t lb, T* bsum) {
+  __shared__ T lds[BT_SCAN_ITEMS];
+  __shared__ T part[4];
+  load_block(src, n, lb * BT_SCAN_ITEMS, lds);
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s += lds[threadIdx.x * 16 + k];
+  T total;
+  (void)block_excl_scan(s, part, &total);
+  if (threadIdx.x == 0) bsum[lb] = total;
+}
+
+template <typename T>
+__device__ __forceinline__ void scan_bsum_body(T* bsum, uint32_t nb) {
+  // nb <= BT_SCAN_ITEMS
+  __shared__ T lds[BT_SCAN_ITEMS];
+  __shared__ T part[4];
+  load_block(bsum, nb, 0u, lds);
+  T v[16];
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    v[k] = s;
+    s += lds[threadIdx.x * 16 + k];
+  }
+  T total;
+  T before = block_excl_scan(s, part, &total);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) lds[threadIdx.x * 16 + k] = before + v[k];
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < nb; k += 256u) bsum[k] = lds[k];
+}
+
+template <typename T>
+__device__ __forceinline__ void scan_apply_body(T* data, uint32_t n, uint32_t lb, const T* bsum) {
+  __shared__ T lds[BT_SCAN_ITEMS];
+  __shared__ T part[4];
+  const uint32_t first = lb * BT_SCAN_ITEMS;
+  load_block(data, n, first, lds);
+  T v[16];
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    v[k] = s;
+    s += lds[threadIdx.x * 16 + k];
+  }
+  T total;
+  T before = block_excl_scan(s, part, &total) + bsum[lb];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) lds[threadIdx.x * 16 + k] = before + v[k];
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < BT_SCAN_ITEMS; k += 256u) {
+    uint32_t i = first + k;
+    if (i < n) data[i] = lds[k];
+  }
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_cell_scan_reduce(
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
+  JOB_PROLOGUE(BT_K_SCAN);
+  scan_reduce_body<uint32_t>(J.cells, J.n_cells + 1u, lb, J.bsum);
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_cell_scan_bsum(
+    const BtJob* __restrict__ jobs, int njobs) {
+  const BtJob& J = jobs[blockIdx.x];
+  const uint32_t nb = (J.n_cells + 1u + BT_SCAN_ITEMS - 1u) / BT_SCAN_ITEMS;
+  scan_bsum_body<uint32_t>(J.bsum, nb);
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_cell_scan_apply(
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
+  JOB_PROLOGUE(BT_K_SCAN);
+  scan_apply_body<uint32_t>(J.cells, J.n_cells + 1u, lb, J.bsum);
+}
+
+// ------------------------------------------------------------------ tasks
+// One task per <= 64 samples of a cell (the reference's Task, one per <= 128,
+// spatial_grid.wgsl:117-123). Task slots are reserved with one atomic per
+// workgroup; task order is irrelevant to the results.
+extern "C" __global__ __launch_bounds__(256) void bt_tasks(const BtJob* __restrict__ jobs,
+                                                           const uint32_t* __restrict__ base,
+                                                           int njobs, BtControl* ctl,
+                                                           BtTask* __restrict__ tasks) {
+  JOB_PROLOGUE(BT_K_TASK);
+  __shared__ uint32_t part[4];
+  __shared__ uint32_t block_base;
+  const uint32_t c0 = lb * BT_TASK_CELLS + threadIdx.x * 4u;
+  const uint32_t* cells = J.cells;
+  uint32_t st[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    uint32_t c = c0 + (uint32_t)k;
+    st[k] = c <= J.n_cells ? cells[c] : 0u;
+  }
+  uint32_t nt[4];
+  uint32_t mine = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    uint32_t c = c0 + (uint32_t)k;
+    uint32_t cnt = c < J.n_cells ? st[k + 1] - st[k] : 0u;
+    nt[k] = (cnt + BT_TASK_POINTS - 1u) / BT_TASK_POINTS;
+    mine += nt[k];
+  }
+  uint32_t total;
+  uint32_t before = block_excl_scan(mine, part, &total);
+  if (threadIdx.x == 0) block_base = total ? atomicAdd(&ctl->task_total, total) : 0u;
+  __syncthreads();
+  uint32_t pos = block_base + before;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    for (uint32_t t = 0; t < nt[k]; ++t) {
+      uint32_t s = st[k] + t * BT_TASK_POINTS;
+      uint32_t e = min(s + BT_TASK_POINTS, st[k + 1]);
+      BtTask task;
+      task.job = (uint32_t)j;
+      task.start = s;
+      task.end = e;
+      task.pad = 0u;
+      tasks[pos++] = task;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- scatter
+// Counting-sort scatter (spatial_grid.wgsl:127-138) with the rank from bt_bin.
+// Also quantizes each position to the 2^-24 fixed-point grid once.
+extern "C" __global__ __launch_bounds__(256) void bt_scatter(const BtJob* __restrict__ jobs,
+                                                             const uint32_t* __restrict__ base,
+                                                             int njobs) {
+  JOB_PROLOGUE(BT_K_BIN);
+  const uint32_t i = lb * BT_BIN_BLOCK + threadIdx.x;
+  if (i >= J.n) return;
+  const float* rec = J.rec[J.parity];
+  float x = rec[3u * i], y = rec[3u * i + 1u];
+  uint32_t id = __float_as_uint(rec[3u * i + 2u]);
+  uint2 cr = J.cellrank[i];
+  uint32_t slot = J.cells[cr.x] + cr.y;
+  int fx = (int)rintf(x * BT_FIX_ONE), fy = (int)rintf(y * BT_FIX_ONE);
+  J.sorted[slot] = make_float4(x, y, __int_as_float(fx), __int_as_float(fy));
+  J.sid[slot] = id;
+}
+
+// ---------------------------------------------------------- neighbor mean
+// neighbor_mean.wgsl:75-203. One wave per task (<= 64 query samples of one
+// cell). The wave's cell window is the union of the per-sample windows
+// [texel(p) - reach, texel(p) + reach]; within each window row the cells are
+// contiguous in sorted order, so the candidates form one range, streamed
+// through scalar loads and tested by every lane against its own sample:
+//   t = (q - p) * texel_scale; accept when fmaf(tx, tx, ty*ty) <= r^2.
+// Accepted neighbors add their fixed-point coordinates to exact int64 sums.
+// Output is written in sorted order (the next iteration's input order), and
+// every sample, fixed or not, is splatted into the density counts.
+extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
+    const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl) {
+  const uint32_t lane = LANE();
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&ctl->queue_head, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    const uint32_t total = __builtin_amdgcn_readfirstlane(
+        __atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
+    if (t >= total) break;
+    const uint32_t tj = __builtin_amdgcn_readfirstlane(tasks[t].job);
+    const uint32_t ts = __builtin_amdgcn_readfirstlane(tasks[t].start);
+    const uint32_t te = __builtin_amdgcn_readfirstlane(tasks[t].end);
+    const BtJob& J = jobs[tj];
+    const uint32_t slot = ts + lane;
+    const bool valid = slot < te;
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t id = 0;
+    if (valid) {
+      q = J.sorted[slot];
+      id = J.sid[slot];
+    }
+    const bool movable = valid && !(id & BT_FIXED_BIT);
+    const float px = q.x, py = q.y;
+    float mx = px, my = py;
+    if (ballot(movable)) {
+      const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2;
+      const float lo_x = wave_min(movable ? px : 3.0e38f);
+      const float lo_y = wave_min(movable ? py : 3.0e38f);
+      const float hi_x = wave_max(movable ? px : -3.0e38f);
+      const float hi_y = wave_max(movable ? py : -3.0e38f);
+      // The reductions leave the same value in every lane; readfirstlane makes
+      // the window (and so every candidate address) wave-uniform.
+      const uint32_t x0 = __builtin_amdgcn_readfirstlane(
+          cell_axis(texel(lo_x, J.res_x) - J.reach_x, J.cell, J.gx));
+      const uint32_t x1 = __builtin_amdgcn_readfirstlane(
+          cell_axis(texel(hi_x, J.res_x) + J.reach_x, J.cell, J.gx));
+      const uint32_t y0 = __builtin_amdgcn_readfirstlane(
+          cell_axis(texel(lo_y, J.res_y) - J.reach_y, J.cell, J.gy));
+      const uint32_t y1 = __builtin_amdgcn_readfirstlane(
+          cell_axis(texel(hi_y, J.res_y) + J.reach_y, J.cell, J.gy));
+      const float4* S = J.sorted;
+      const uint32_t* cells = J.cells;
+      long long ax = 0, ay = 0;
+      int n = 0;
+      for (uint32_t cy = y0; cy <= y1; ++cy) {
+        const uint32_t b = ld_uniform(cells, cy * J.gx + x0);
+        const uint32_t e = ld_uniform(cells, cy * J.gx + x1 + 1u);
+        for (uint32_t c0 = b; c0 < e; c0 += 64u) {
+          const uint32_t c1 = min(c0 + 64u, e);
+          int sx = 0, sy = 0, sc = 0;
+          uint32_t k = c0;
+          for (; k + 4u <= c1; k += 4u) {
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+              const float4 s = ld_uniform(S, k + u);
+              const float dx = (s.x - px) * tsx;
+              const float dy = (s.y - py) * tsy;
+              const bool ok = fmaf(dx, dx, dy * dy) <= r2;
+              sx += ok ? __float_as_int(s.z) : 0;
+              sy += ok ? __float_as_int(s.w) : 0;
+              sc += ok ? 1 : 0;
+            }
+          }
+          for (; k < c1; ++k) {
+            const float4 s = ld_uniform(S, k);
+            const float dx = (s.x - px) * tsx;
+            const float dy = (s.y - py) * tsy;
+            const bool ok = fmaf(dx, dx, dy * dy) <= r2;
+            sx += ok ? __float_as_int(s.z) : 0;
+            sy += ok ? __float_as_int(s.w) : 0;
+            sc += ok ? 1 : 0;
+          }
+          ax += sx;
+          ay += sy;
+          n += sc;
+        }
+      }
+      if (movable) {
+        const long long pfx = (long long)__float_as_int(q.z), pfy = (long long)__float_as_int(q.w);
+        const long long dxs = ax - (long long)n * pfx, dys = ay - (long long)n * pfy;
+        const double den = (double)n * 16777216.0;
+        mx = (float)((double)px + (double)dxs / den);
+        my = (float)((double)py + (double)dys / den);
+      }
+    }
+    if (valid) {
+      float* out = J.rec[J.parity ^ 1u];
+      out[3u * slot] = mx;
+      out[3u * slot + 1u] = my;
+      out[3u * slot + 2u] = __uint_as_float(id);
+      if (J.dens_cnt) atomicAdd(&J.dens_cnt[splat_index(mx, my, J.dw, J.dh)], 1u);
+    }
+  }
+}
+
+// ------------------------------------------------------------------- blur
+// blur.wgsl:13-65 generalized to any sigma: row pass from the u32 counts.
+// A 64 x 4 tile plus a taps-wide halo on each side sits in LDS; zero padding
+// outside the image; v = t[c]*w0, then v += (t[c-k] + t[c+k]) * w[k].
+extern "C" __global__ __launch_bounds__(256) void bt_blur_rows(const BtJob* __restrict__ jobs,
+                                                               const uint32_t* __restrict__ base,
+                                                               int njobs) {
+  JOB_PROLOGUE(BT_K_ROW);
+  __shared__ float tile[BT_ROW_TILE_H][BT_ROW_TILE_W + 2 * BT_MAX_TAPS];
+  const uint32_t W = J.dw, H = J.dh, K = J.taps;
+  const uint32_t tiles_x = (W + BT_ROW_TILE_W - 1u) / BT_ROW_TILE_W;
+  const uint32_t x0 = (lb % tiles_x) * BT_ROW_TILE_W, y0 = (lb / tiles_x) * BT_ROW_TILE_H;
+  const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
+  const uint32_t y = y0 + ty;
+  const uint32_t span = BT_ROW_TILE_W + 2u * K;
+  const uint32_t* src = J.dens_cnt + (size_t)y * W;
+  for (uint32_t e = tx; e < span; e += 64u) {
+    const int gx = (int)x0 - (int)K + (int)e;
+    float v = 0.0f;
+    if (y < H && gx >= 0 && gx < (int)W) v = (float)src[gx];
+    tile[ty][e] = v;
+  }
+  __syncthreads();
+  const uint32_t x = x0 + tx;
+  if (x < W && y < H) {
+    const float* wt = J.wt;
+    const uint32_t c = K + tx;
+    float v = tile[ty][c] * ld_uniform(wt, 0u);
+    for (uint32_t k = 1; k <= K; ++k) v = v + (tile[ty][c - k] + tile[ty][c + k]) * ld_uniform(wt, k);
+    J.dens_row[(size_t)y * W + x] = v;
+  }
+}
+
+// blur.wgsl:13-83 column pass plus the peak (atomicMax on the f32 bits of a
+// non-negative value, as the reference does).
+extern "C" __global__ __launch_bounds__(256) void bt_blur_cols(const BtJob* __restrict__ jobs,
+                                                               const uint32_t* __restrict__ base,
+                                                               int njobs) {
+  JOB_PROLOGUE(BT_K_COL);
+  __shared__ float tile[BT_COL_TILE_H + 2 * BT_MAX_TAPS][BT_COL_TILE_W];
+  __shared__ float wmax[4];
+  const uint32_t W = J.dw, H = J.dh, K = J.taps;
+  const uint32_t tiles_x = (W + BT_COL_TILE_W - 1u) / BT_COL_TILE_W;
+  const uint32_t x0 = (lb % tiles_x) * BT_COL_TILE_W, y0 = (lb / tiles_x) * BT_COL_TILE_H;
+  const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
+  const uint32_t x = x0 + tx;
+  const uint32_t span = BT_COL_TILE_H + 2u * K;
+  const float* src = J.dens_row;
+  for (uint32_t e = ty; e < span; e += 4u) {
+    const int gy = (int)y0 - (int)K + (int)e;
+    float v = 0.0f;
+    if (x < W && gy >= 0 && gy < (int)H) v = src[(size_t)gy * W + x];
+    tile[e][tx] = v;
+  }
+  __syncthreads();
+  const float* wt = J.wt;
+  float peak = 0.0f;
+  for (uint32_t r = ty; r < BT_COL_TILE_H; r += 4u) {
+    const uint32_t y = y0 + r;
+    if (x < W && y < H) {
+      const uint32_t c = K + r;
+      float v = tile[c][tx] * ld_uniform(wt, 0u);
+      for (uint32_t k = 1; k <= K; ++k) v = v + (tile[c - k][tx] + tile[c + k][tx]) * ld_uniform(wt, k);
+      J.dens_col[(size_t)y * W + x] = v;
+      peak = fmaxf(peak, v);
+    }
+  }
+  peak = wave_max(peak);
+  if ((threadIdx.x & 63u) == 0) wmax[ty] = peak;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    atomicMax(J.peak, __float_as_uint(m));
+  }
+}
+
+// normalization.wgsl:5-17, f32 R channel.
+extern "C" __global__ __launch_bounds__(256) void bt_normalize(const BtJob* __restrict__ jobs,
+                                                               const uint32_t* __restrict__ base,
+                                                               int njobs) {
+  JOB_PROLOGUE(BT_K_NORM);
+  const uint32_t npix = J.dw * J.dh;
+  const float peak = __uint_as_float(*(volatile uint32_t*)J.peak);
+  const uint32_t i0 = lb * BT_NORM_ITEMS + threadIdx.x;
+#pragma unroll
+  for (uint32_t k = 0; k < BT_NORM_ITEMS / 256u; ++k) {
+    const uint32_t i = i0 + k * 256u;
+    if (i < npix) {
+      const float v = J.dens_col[i];
+      J.dens_norm[i] = peak > 0.0f ? fminf(fmaxf(v / peak, 0.0f), 1.0f) : 0.0f;
+    }
+  }
+}
+
+// ------------------------------------------------------------- resampling
+// resampling.wgsl:45-48
+__device__ __forceinline__ void normalized(const BtJob& J, uint32_t i, float& x, float& y) {
+  const float* c = J.ctrl;
+  x = ((c[2u * i] - J.norm_min_x) * J.norm_scale + J.norm_tr_x - 0.5f) * 2.0f;
+  y = ((c[2u * i + 1u] - J.norm_min_y) * J.norm_scale + J.norm_tr_y - 0.5f) * 2.0f;
+}
+
+__device__ __forceinline__ uint32_t steps_of(float distance, float remaining) {
+  float f = fmaxf(floorf(distance - remaining) + 1.0f, 0.0f);
+  return f >= 4294967296.0f ? 0xffffffffu : (uint32_t)f;
+}
+
+// resampling.wgsl:82-159 count mode: one thread per path, closed-form step
+// count per segment. Writes (count << 32) | 1 for paths with >= 2 samples,
+// else 0; entry n_paths is zeroed so the scan yields totals.
+extern "C" __global__ __launch_bounds__(256) void bt_resample_count(
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
+  JOB_PROLOGUE(BT_K_PATH);
+  const uint32_t i = lb * 256u + threadIdx.x;
+  if (i > J.n_paths) return;
+  if (i == J.n_paths) {
+    J.path_scan[i] = 0ull;
+    return;
+  }
+  const uint32_t cap = J.capacity;
+  const uint32_t s = J.starts[i], e = J.starts[i + 1u];
+  float fx, fy;
+  normalized(J, s, fx, fy);
+  const float tsx = J.ts_x, tsy = J.ts_y;
+  uint32_t count = 0;
+  float lx = fx, ly = fy;
+  // push(first)
+  if (count >= cap) count = cap + 1u; else { lx = fx; ly = fy; count = 1u; }
+  float px = fx, py = fy, remaining = 1.0f;
+  bool over = count > cap;
+  for (uint32_t k = s + 1u; k < e && !over; ++k) {
+    float cx, cy;
+    normalized(J, k, cx, cy);
+    const float dx = cx - px, dy = cy - py;
+    const float ex = dx * tsx, ey = dy * tsy;
+    const float distance = sqrtf(fmaf(ex, ex, ey * ey));
+    if (distance > 0.0f) {
+      const uint32_t steps = steps_of(distance, remaining);
+      if (steps > 0u) {
+        const float t0 = remaining / distance;
+        const float t1 = (remaining + (float)(steps - 1u)) / distance;
+        const float sx = px + dx * t0, sy = py + dy * t0;
+        const float ux = px + dx * t1, uy = py + dy * t1;
+        count += steps - ((sx == lx && sy == ly) ? 1u : 0u);
+        lx = ux;
+        ly = uy;
+        if (count > cap) { count = cap + 1u; over = true; }
+      }
+      remaining = remaining + (float)steps - distance;
+    }
+    px = cx;
+    py = cy;
+  }
+  if (!over) {
+    // push(previous)
+    if (!(count > 0u && px == lx && py == ly)) {
+      if (count >= cap) count = cap + 1u; else { lx = px; ly = py; ++count; }
+    }
+    if (count == 1u) {
+      for (uint32_t k = s + 1u; k < e; ++k) {
+        float nx, ny;
+        normalized(J, k, nx, ny);
+        if (nx != fx || ny != fy) {
+          if (!(nx == lx && ny == ly)) {
+            if (count >= cap) count = cap + 1u; else { lx = nx; ly = ny; ++count; }
+          }
+          if (!(count > 0u && px == lx && py == ly)) {
+            if (count >= cap) count = cap + 1u; else { lx = px; ly = py; ++count; }
+          }
+          break;
+        }
+      }
+    }
+  }
+  J.path_scan[i] = count < 2u ? 0ull : (((unsigned long long)count << 32) | 1ull);
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_path_scan_reduce(
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
+  JOB_PROLOGUE(BT_K_PSCAN);
+  scan_reduce_body<unsigned long long>((const unsigned long long*)J.path_scan, J.n_paths + 1u, lb,
+                                       (unsigned long long*)J.bsum);
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_path_scan_bsum(const BtJob* __restrict__ jobs,
+                                                                    int njobs) {
+  const BtJob& J = jobs[blockIdx.x];
+  const uint32_t nb = (J.n_paths + 1u + BT_SCAN_ITEMS - 1u) / BT_SCAN_ITEMS;
+  scan_bsum_body<unsigned long long>((unsigned long long*)J.bsum, nb);
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_path_scan_apply(
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
+  JOB_PROLOGUE(BT_K_PSCAN);
+  scan_apply_body<unsigned long long>((unsigned long long*)J.path_scan, J.n_paths + 1u, lb,
+                                      (const unsigned long long*)J.bsum);
+}
+
+// resampling.wgsl:50-136 write mode + emit_paths (:161-183): one wave per
+// path. Within a segment the samples are independent (sample k sits at
+// previous + delta * ((remaining + k) / distance)); lanes produce 64 at a
+// time and the duplicate rule (a sample equal to the previous one is
+// dropped, :51-53) is applied against the neighbouring lane's sample.
+// Writes records {x, y, slot | fixed}, the path id of each sample, the
+// Reservation triple of the path, and splats every written sample.
+extern "C" __global__ __launch_bounds__(256) void bt_resample_emit(
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
+  JOB_PROLOGUE(BT_K_EMIT);
+  const uint32_t lane = LANE();
+  const uint32_t p = lb * BT_EMIT_PATHS + (threadIdx.x >> 6);
+  if (p >= J.n_paths) return;
+  const unsigned long long sc = J.path_scan[p];
+  const unsigned long long nx_scan = J.path_scan[p + 1u];
+  const uint32_t count = (uint32_t)((nx_scan >> 32) - (sc >> 32));
+  const uint32_t off = (uint32_t)(sc >> 32);
+  const uint32_t valid_before = (uint32_t)(sc & 0xffffffffu);
+  if (lane == 0) {
+    J.paths[3u * p] = count ? off : 0u;
+    J.paths[3u * p + 1u] = count ? off - valid_before : 0u;
+    J.paths[3u * p + 2u] = count;
+  }
+  if (count < 2u) return;
+  float* rec = J.rec[0];
+  uint32_t* edge_of = J.edge_of;
+  uint32_t* dens = J.dens_cnt;
+  const uint32_t W = J.dw, H = J.dh;
+  bool bad = false;
+  uint32_t emitted = 0;
+  float lx, ly;
+  // Writes one uniform sample (lane 0), applying the duplicate rule.
+  auto push1 = [&](float x, float y) {
+    if (emitted > 0u && x == lx && y == ly) return;
+    if (emitted >= count) { bad = true; return; }
+    if (lane == 0) {
+      const uint32_t slot = off + emitted;
+      const uint32_t fixed = (emitted == 0u || emitted == count - 1u) ? BT_FIXED_BIT : 0u;
+      rec[3u * slot] = x;
+      rec[3u * slot + 1u] = y;
+      rec[3u * slot + 2u] = __uint_as_float(slot | fixed);
+      edge_of[slot] = p;
+      if (dens) atomicAdd(&dens[splat_index(x, y, W, H)], 1u);
+    }
+    lx = x;
+    ly = y;
+    ++emitted;
+  };
+  const uint32_t s = J.starts[p], e = J.starts[p + 1u];
+  float fx, fy;
+  normalized(J, s, fx, fy);
+  lx = fx;
+  ly = fy;
+  push1(fx, fy);
+  const float tsx = J.ts_x, tsy = J.ts_y;
+  float px = fx, py = fy, remaining = 1.0f;
+  for (uint32_t k = s + 1u; k < e && !bad; ++k) {
+    float cx, cy;
+    normalized(J, k, cx, cy);
+    const float dx = cx - px, dy = cy - py;
+    const float ex = dx * tsx, ey = dy * tsy;
+    const float distance = sqrtf(fmaf(ex, ex, ey * ey));
+    if (distance > 0.0f) {
+      const uint32_t steps = steps_of(distance, remaining);
+      for (uint32_t b0 = 0; b0 < steps && !bad; b0 += 64u) {
+        const uint32_t st = b0 + lane;
+        const bool active = st < steps;
+        const float t = (remaining + (float)st) / distance;
+        const float sx = px + dx * t, sy = py + dy * t;
+        float qx = __shfl_up(sx, 1, 64), qy = __shfl_up(sy, 1, 64);
+        if (lane == 0) { qx = lx; qy = ly; }
+        const bool keep = active && !(sx == qx && sy == qy);
+        const uint64_t km = ballot(keep);
+        const uint32_t pos = emitted + mbcnt64(km);
+        if (keep) {
+          if (pos >= count) {
+            bad = true;
+          } else {
+            const uint32_t slot = off + pos;
+            const uint32_t fixed = (pos == 0u || pos == count - 1u) ? BT_FIXED_BIT : 0u;
+            rec[3u * slot] = sx;
+            rec[3u * slot + 1u] = sy;
+            rec[3u * slot + 2u] = __uint_as_float(slot | fixed);
+            edge_of[slot] = p;
+            if (dens) atomicAdd(&dens[splat_index(sx, sy, W, H)], 1u);
+          }
+        }
+        bad = ballot(bad) != 0ull;
+        emitted += (uint32_t)__popcll(km);
+        const uint64_t am = ballot(active);
+        const int last = 63 - __builtin_clzll(am);
+        lx = __shfl(sx, last, 64);
+        ly = __shfl(sy, last, 64);
+      }
+      remaining = remaining + (float)steps - distance;
+    }
+    px = cx;
+    py = cy;
+  }
+  if (!bad) push1(px, py);
+  if (!bad && emitted == 1u) {
+    for (uint32_t k = s + 1u; k < e; ++k) {
+      float nx, ny;
+      normalized(J, k, nx, ny);
+      if (nx != fx || ny != fy) {
+        push1(nx, ny);
+        if (!bad) push1(px, py);
+        break;
+      }
+    }
+  }
+  if ((bad || emitted != count) && lane == 0) atomicOr(&J.status[0], 2u);
+}
+
+// ------------------------------------------------------------ readbacks
+// Undo the sorted order: record at slot -> original index, with the path id
+// (the reference's Point {x, y, edge_id}, graph.wgsl:2-6).
+extern "C" __global__ __launch_bounds__(256) void bt_unpermute(const BtJob* __restrict__ jobs,
+                                                               const uint32_t* __restrict__ base,
+                                                               int njobs, float* __restrict__ out,
+                                                               uint32_t* __restrict__ cell_out,
+                                                               uint32_t want_cells) {
+  JOB_PROLOGUE(BT_K_BIN);
+  const uint32_t i = lb * BT_BIN_BLOCK + threadIdx.x;
+  if (i >= J.n) return;
+  const float* rec = J.rec[J.parity];
+  const uint32_t id = __float_as_uint(rec[3u * i + 2u]);
+  const uint32_t orig = id & ~BT_FIXED_BIT;
+  out[3u * orig] = rec[3u * i];
+  out[3u * orig + 1u] = rec[3u * i + 1u];
+  out[3u * orig + 2u] = __uint_as_float(J.edge_of[orig]);
+  if (want_cells) {
+    // cell of the sample as binned in the last iteration: sid/sorted are in
+    // the order this iteration's input was binned from (rec of the other parity)
+    const float* prev = J.rec[J.parity ^ 1u];
+    const uint32_t pid = __float_as_uint(prev[3u * i + 2u]) & ~BT_FIXED_BIT;
+    cell_out[pid] = J.cellrank[i].x;
+  }
+}
+
+// ------------------------------------------------------- launch wrappers
+// Host-callable entry points used by the C++ runtime (bt_host.cpp).
+#include "bt_launch.h"
+
+namespace btk {
+#define L256(grid) dim3(grid), dim3(256), 0, s
+void clear(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+           BtControl* ctl, int clear_density) {
+  hipLaunchKernelGGL(bt_clear, L256(grid), jobs, base, nj, ctl, clear_density);
+}
+void bin(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  hipLaunchKernelGGL(bt_bin, L256(grid), jobs, base, nj);
+}
+void cell_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  hipLaunchKernelGGL(bt_cell_scan_reduce, L256(grid), jobs, base, nj);
+  hipLaunchKernelGGL(bt_cell_scan_bsum, L256((uint32_t)nj), jobs, nj);
+  hipLaunchKernelGGL(bt_cell_scan_apply, L256(grid), jobs, base, nj);
+}
+void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+           BtControl* ctl, BtTask* tasks_out) {
+  hipLaunchKernelGGL(bt_tasks, L256(grid), jobs, base, nj, ctl, tasks_out);
+}
+void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  hipLaunchKernelGGL(bt_scatter, L256(grid), jobs, base, nj);
+}
+void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
+                   BtControl* ctl) {
+  hipLaunchKernelGGL(bt_neighbor_mean, L256(grid), jobs, tasks_in, ctl);
+}
+void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  hipLaunchKernelGGL(bt_blur_rows, L256(grid), jobs, base, nj);
+}
+void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  hipLaunchKernelGGL(bt_blur_cols, L256(grid), jobs, base, nj);
+}
+void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  hipLaunchKernelGGL(bt_normalize, L256(grid), jobs, base, nj);
+}
+void resample_count(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  hipLaunchKernelGGL(bt_resample_count, L256(grid), jobs, base, nj);
+}
+void path_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  hipLaunchKernelGGL(bt_path_scan_reduce, L256(grid), jobs, base, nj);
+  hipLaunchKernelGGL(bt_path_scan_bsum, L256((uint32_t)nj), jobs, nj);
+  hipLaunchKernelGGL(bt_path_scan_apply, L256(grid), jobs, base, nj);
+}
+void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  hipLaunchKernelGGL(bt_resample_emit, L256(grid), jobs, base, nj);
+}
+void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+               float* out, uint32_t* cell_out, uint32_t want_cells) {
+  hipLaunchKernelGGL(bt_unpermute, L256(grid), jobs, base, nj, out, cell_out, want_cells);
+}
+#undef L256
+}  // namespace btk

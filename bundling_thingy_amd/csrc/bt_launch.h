@@ -1,0 +1,25 @@
+// Launch wrappers of the kernels in bt_kernels.hip (one call per stage).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bt_device.h"
+
+namespace btk {
+void clear(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+           BtControl* ctl, int clear_density);
+void bin(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void cell_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+           BtControl* ctl, BtTask* tasks_out);
+void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
+                   BtControl* ctl);
+void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void resample_count(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void path_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+               float* out, uint32_t* cell_out, uint32_t want_cells);
+}  // namespace btk

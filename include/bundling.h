@@ -1,0 +1,235 @@
+/*
+ * bundling.h - C ABI of the MI355X edge-bundling engine.
+ *
+ * Drop-in for the compute half of the reference's `Gpu` facade
+ * (lpstankus/bundling_thingy, src/gpu.rs) and the host helpers around it.
+ * All arrays are plain host pointers and sizes; the caller owns them and they
+ * are copied during the call. A context owns its device memory and one HIP
+ * stream; it is not thread-safe (the reference's `&mut self`); many contexts
+ * may coexist. Every call blocks until its device work is complete, like the
+ * reference's submit + poll(Wait) (gpu.rs:484-490,515-516,551-552).
+ *
+ * Return values: BT_OK or a BT_E_* code; bt_last_error() gives the message.
+ */
+#ifndef BUNDLING_H
+#define BUNDLING_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BT_ABI_VERSION 1
+
+#define BT_OK 0
+#define BT_E_INVALID 1   /* bad argument or malformed input */
+#define BT_E_NO_GRAPH 2  /* "no uploaded graph" (gpu.rs:533) */
+#define BT_E_CAPACITY 3  /* graph memory budget / sample capacity (resampling.rs:86-89,300-304,346-372) */
+#define BT_E_DEVICE 4    /* HIP runtime failure */
+#define BT_E_STATE 5     /* "neighbor radius is not configured" (gpu.rs:529-532) */
+#define BT_E_MISMATCH 6  /* "GPU resampling count and write passes disagree" (resampling.rs:296-299) */
+#define BT_E_BUSY 7      /* server: not admitted yet / queue full */
+
+/* bt_options.flags */
+#define BT_FLAG_DENSITY 1u    /* ColorBy::Density (palette.rs:3-10, the reference default) */
+#define BT_FLAG_DISTORTION 2u /* Options::evaluate_distortion: staging bytes count in the budget */
+
+/* Options of one context / job. Replaces the reference's `Options`
+ * (main.rs:25-90) plus the map size (gpu.rs:21,99-102).
+ * spacing == 0 selects reference mode: samples 1 texel apart on a map_w x
+ * map_h map, integer radius_texels, a map-sized density and sigma = 6.
+ * spacing > 0 selects generalized mode (canvas units = the unit square after
+ * the reference's 95 % normalization): radius and sigma follow geometric
+ * schedules over schedule_len iterations and the density is density_w x
+ * density_h. The generalized formulas reduce exactly to reference mode when
+ * spacing = 1/(S-1), radius = r/(S-1), density = S x S and sigma = 6. */
+typedef struct bt_options {
+  uint32_t map_w, map_h;           /* reference map size (SIZE = 1600) */
+  uint32_t radius_texels;          /* Options::neighbor_radius; 0 = not configured */
+  uint32_t flags;                  /* BT_FLAG_* */
+  uint64_t max_graph_memory_bytes; /* Options::max_graph_memory_mib * MiB */
+  int32_t device;                  /* HIP device ordinal */
+  float spacing;                   /* generalized: sample spacing, canvas units */
+  float radius0, radius1;          /* generalized: radius schedule, canvas units */
+  float sigma0, sigma1;            /* generalized: blur sigma schedule, pixels */
+  uint32_t schedule_len;           /* generalized: iterations the schedules span */
+  uint32_t density_w, density_h;   /* generalized: density size */
+  uint32_t cell_split;             /* grid cells per radius; 0 = autotuned */
+  uint32_t reserved[3];
+} bt_options;
+
+/* Exact parameters of one iteration (what the kernels use). */
+typedef struct bt_iter_params {
+  float res_x, res_y;   /* texel resolution: texel_scale = res * 0.5 */
+  float radius;         /* texels */
+  float cell;           /* grid cell side, texels */
+  uint32_t gx, gy;      /* grid dims */
+  float sigma;          /* pixels */
+  uint32_t taps;        /* blur half-width = ceil(3 sigma) */
+  uint32_t density_w, density_h;
+} bt_iter_params;
+
+typedef struct bt_info {
+  uint32_t jobs;
+  uint32_t samples;     /* job 0 */
+  uint32_t paths;       /* job 0: path count (starts - 1) */
+  uint32_t iteration;   /* job 0: iterations since upload */
+  uint64_t capacity;    /* job 0: sample capacity of the budget model */
+  uint64_t reserved_bytes;
+  uint64_t device_bytes;/* device memory held by the context */
+} bt_info;
+
+typedef struct bt_ctx bt_ctx;
+
+/* Reference defaults: map 1600 x 1600, no radius, 1024 MiB, density on. */
+void bt_options_default(bt_options* opts);
+
+/* Gpu::new (gpu.rs:59-168). */
+int bt_create(const bt_options* opts, bt_ctx** out);
+void bt_destroy(bt_ctx* ctx);
+const char* bt_last_error(const bt_ctx* ctx);
+
+/* Gpu::upload (gpu.rs:492-525): resamples the graph (CSR polylines: xy holds
+ * n_points (x, y) pairs, starts holds n_starts = paths + 1 offsets, as in
+ * Graph::points()/starts(), graph.rs:93-99), computes the initial density and
+ * drops any previous graph. *samples receives the sample count. */
+int bt_upload(bt_ctx* ctx, const float* xy, uint32_t n_points, const uint32_t* starts,
+              uint32_t n_starts, uint32_t* samples);
+
+/* Gpu::iterate (gpu.rs:527-553): one neighbor-mean iteration, density splat,
+ * blur and normalize. */
+int bt_iterate(bt_ctx* ctx);
+
+/* n iterations back to back, captured once as a hipGraph and replayed. */
+int bt_iterate_n(bt_ctx* ctx, uint32_t n);
+
+/* Restore the state right after the last upload (positions, density,
+ * iteration 0) without resampling. */
+int bt_rewind(bt_ctx* ctx);
+
+/* Gpu::distortion_layout (gpu.rs:438-467): one Reservation triple
+ * {point_offset, segment_offset, count} per path (count 0 = skipped path). */
+int bt_read_paths(bt_ctx* ctx, uint32_t* rec3, size_t n_paths);
+
+/* Gpu::distortion_snapshot (gpu.rs:469-482): n_samples little-endian 12-byte
+ * records {f32 x, f32 y, u32 edge_id} in Layout order (geometry.rs:131-173). */
+int bt_read_points(bt_ctx* ctx, void* rec12, size_t n_samples);
+
+/* Density readback (density.rs:82-166). kind: 0 = u32 counts, 1 = row-blurred
+ * f32, 2 = blurred f32, 3 = normalized f32 (the R channel of the reference's
+ * rgba16f texture, before f16 rounding). */
+int bt_read_density(bt_ctx* ctx, uint32_t kind, void* out, size_t n_pixels);
+
+/* Grid of the last iteration: the cell of every sample (in Layout order) and
+ * the per-cell counts (row-major cy * gx + cx, spatial_grid.wgsl:63-70). */
+int bt_read_grid(bt_ctx* ctx, uint32_t* cell_of, size_t n_samples, uint32_t* counts,
+                 size_t n_cells);
+
+int bt_get_info(bt_ctx* ctx, bt_info* info);
+
+/* Parameters iteration `iteration` of a job with these options runs with
+ * (neighbor_mean.rs:19-46 grid sizes, blur.wgsl:41-60 taps). */
+int bt_iter_params_for(const bt_options* opts, uint32_t iteration, bt_iter_params* out);
+
+/* Per-stage device time of the last bt_profile call, in ms, by stage:
+ * 0 clear, 1 bin, 2 scan, 3 tasks, 4 scatter, 5 neighbor_mean, 6 blur_rows,
+ * 7 blur_cols, 8 normalize (the reference's GpuTimings stage keys,
+ * neighbor_mean.rs:5-13, density.rs:5-9). Runs n iterations eagerly with
+ * hipEvents around each stage; returns total ms. */
+int bt_profile(bt_ctx* ctx, uint32_t n, double* stage_ms, uint32_t n_stages, double* total_ms);
+
+/* ---------------------------------------------------------------- batched */
+
+/* One graph of a batch. */
+typedef struct bt_graph_desc {
+  const float* xy;
+  uint32_t n_points;
+  const uint32_t* starts;
+  uint32_t n_starts;
+  const bt_options* opts; /* per-graph options; NULL = the context's */
+} bt_graph_desc;
+
+/* Replaces the context's graphs with n_graphs independent jobs resampled in
+ * one set of launches; samples[i] receives job i's sample count. Every later
+ * bt_iterate / bt_iterate_n advances all jobs with one launch per stage. */
+int bt_upload_batch(bt_ctx* ctx, const bt_graph_desc* graphs, uint32_t n_graphs,
+                    uint32_t* samples);
+
+int bt_read_paths_job(bt_ctx* ctx, uint32_t job, uint32_t* rec3, size_t n_paths);
+int bt_read_points_job(bt_ctx* ctx, uint32_t job, void* rec12, size_t n_samples);
+int bt_read_density_job(bt_ctx* ctx, uint32_t job, uint32_t kind, void* out, size_t n_pixels);
+int bt_job_samples(bt_ctx* ctx, uint32_t job, uint32_t* samples, uint32_t* iteration);
+
+/* ------------------------------------------------------------ host helpers */
+
+/* Graph::parse (graph.rs:35-77). Arrays are malloc'ed; free with bt_free.
+ * bbox = {min_x, min_y, max_x, max_y}. err receives up to errlen bytes. */
+int bt_trl_parse(const char* text, size_t len, float** xy, uint32_t* n_points,
+                 uint32_t** starts, uint32_t* n_starts, float* bbox, char* err,
+                 size_t errlen);
+void bt_free(void* p);
+
+/* Graph::normalization (graph.rs:79-91): out = {min_x, min_y, scale, tr_x, tr_y}. */
+int bt_normalization(const float* bbox, float* out);
+
+/* Allocation::new (resampling.rs:321-397) with Resampler::new's budget check
+ * (resampling.rs:72-89): sample capacity and reserved bytes of a graph. */
+int bt_capacity(const bt_options* opts, uint32_t n_points, uint32_t n_starts,
+                uint64_t* capacity, uint64_t* reserved_bytes);
+
+/* Blur weights w[0..taps] for sigma (blur.wgsl:42-60 at sigma = 6); returns taps. */
+uint32_t bt_blur_weights(float sigma, float* w, uint32_t max_taps);
+
+/* Aggregate distortion (distortion.rs:118-216) of a snapshot against the
+ * baseline snapshot under Layout paths (rec3 triples). out = {distortion,
+ * rms displacement (clip units), rms displacement in pixels at map size}. */
+int bt_distortion(const float* original_xy, const float* current_xy, const uint32_t* rec3,
+                  uint32_t n_paths, uint32_t n_samples, uint32_t map_size, double* out);
+
+/* ----------------------------------------------------------------- served */
+
+typedef struct bt_server bt_server;
+
+typedef struct bt_server_options {
+  bt_options defaults;       /* device, flags */
+  uint64_t memory_budget;    /* device bytes the in-flight jobs may hold */
+  uint32_t max_inflight;     /* jobs per tick (continuous batch size cap) */
+  uint32_t use_graph;        /* reserved */
+} bt_server_options;
+
+typedef struct bt_job_result {
+  uint64_t ticket;
+  uint32_t state;            /* 0 queued, 1 running, 2 done, 3 failed */
+  uint32_t samples;
+  uint32_t iterations_done;
+  uint32_t iterations_total;
+  double submit_ms, admit_ms, done_ms;  /* server clock */
+} bt_job_result;
+
+int bt_server_create(const bt_server_options* opts, bt_server** out);
+void bt_server_destroy(bt_server* srv);
+const char* bt_server_last_error(const bt_server* srv);
+
+/* Queues a job (graph copied): `iterations` iterations under `opts`. */
+int bt_server_submit(bt_server* srv, const float* xy, uint32_t n_points, const uint32_t* starts,
+                     uint32_t n_starts, const bt_options* opts, uint32_t iterations,
+                     uint64_t* ticket);
+
+/* One scheduler tick: admits queued jobs that fit the memory budget (and
+ * resamples them in one batched launch), advances every running job by one
+ * iteration in one batched launch per stage, retires finished jobs. */
+int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued);
+
+int bt_server_poll(bt_server* srv, uint64_t ticket, bt_job_result* out);
+int bt_server_read_points(bt_server* srv, uint64_t ticket, void* rec12, size_t n_samples);
+int bt_server_read_density(bt_server* srv, uint64_t ticket, uint32_t kind, void* out,
+                           size_t n_pixels);
+/* Frees a finished job's device state. */
+int bt_server_release(bt_server* srv, uint64_t ticket);
+double bt_server_clock_ms(const bt_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,216 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs. Integer outputs (sample counts, path reservations, edge
+ids, grid cells, per-cell counts, density counts) must be bit-exact; so must
+the FP32 positions and density fields, because the kernels and the oracle
+share one arithmetic contract. Tolerances from the reference's parity bar are
+asserted too (positions within 1e-5 of the bbox diagonal, density relative
+L-inf <= 1e-5) so the bar is visible in the test."""
+import numpy as np
+import pytest
+
+from bundling_thingy_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+POS_TOL = 1e-5   # of the bounding-box diagonal, after one iteration
+DENS_TOL = 1e-5  # relative L-inf
+
+
+def _oracle_pipeline(orc, graph, options):
+    return orc.Pipeline(graph.points, graph.starts, graph.bbox, options.iter_params(0))
+
+
+def _assert_points(gpu_pts, ref_pts, label):
+    diag = float(np.linalg.norm(ref_pts.max(0) - ref_pts.min(0)))
+    err = float(np.abs(gpu_pts.astype(np.float64) - ref_pts).max()) if len(ref_pts) else 0.0
+    assert err <= POS_TOL * diag, f"{label}: max error {err} > {POS_TOL} * {diag}"
+    np.testing.assert_array_equal(gpu_pts.view(np.uint32), ref_pts.view(np.uint32), err_msg=label)
+
+
+def _assert_density(gpu, pl, label):
+    np.testing.assert_array_equal(gpu.density(0), pl.density_counts, err_msg=label + " counts")
+    for kind, ref in ((2, pl.blurred), (3, pl.normalized)):
+        d = gpu.density(kind)
+        scale = max(float(np.abs(ref).max()), 1e-30)
+        assert float(np.abs(d - ref).max()) / scale <= DENS_TOL, label
+        np.testing.assert_array_equal(d.view(np.uint32), ref.view(np.uint32), err_msg=label)
+
+
+@pytest.fixture(scope="module")
+def cpu_case():
+    return workloads.plain_cpu_graph()
+
+
+def test_reference_mode_upload_and_iterations(bt, orc, cpu_case):
+    """Reference mode S=201, r=10, sigma 6 (the plain CPU case)."""
+    opts = workloads.reference_cpu_options()
+    pl = _oracle_pipeline(orc, cpu_case, opts)
+    with bt.Gpu(opts) as gpu:
+        n = gpu.upload(cpu_case)
+        assert n == len(pl.points)
+        paths = gpu.paths()
+        np.testing.assert_array_equal(paths[:, 2], pl.counts)
+        np.testing.assert_array_equal(paths[:, 0], pl.offsets.astype(np.uint32))
+        pts, eid = gpu.points()
+        np.testing.assert_array_equal(eid, pl.edge_id)
+        _assert_points(pts, pl.points, "upload")
+        _assert_density(gpu, pl, "upload")
+        for it in range(3):
+            gpu.iterate()
+            p = opts.iter_params(it)
+            ref = pl.iterate(p)
+            pts, _ = gpu.points()
+            _assert_points(pts, ref, f"iteration {it + 1}")
+            _assert_density(gpu, pl, f"iteration {it + 1}")
+
+
+def test_grid_cells_and_counts_bitexact(bt, orc, cpu_case):
+    opts = workloads.reference_cpu_options()
+    pl = _oracle_pipeline(orc, cpu_case, opts)
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(cpu_case)
+        before, _ = gpu.points()
+        gpu.iterate()
+        cells, counts = gpu.grid()
+        ocell, ocounts = orc.grid_bin(before, orc.grid(opts.iter_params(0)))
+        np.testing.assert_array_equal(cells, ocell)
+        np.testing.assert_array_equal(counts.reshape(-1), ocounts)
+        assert counts.sum() == len(before)
+
+
+def test_generalized_cpu_case(bt, orc, cpu_case):
+    opts = workloads.plain_cpu_options()
+    pl = _oracle_pipeline(orc, cpu_case, opts)
+    with bt.Gpu(opts) as gpu:
+        assert gpu.upload(cpu_case) == len(pl.points)
+        for it in range(2):
+            gpu.iterate()
+            ref = pl.iterate(opts.iter_params(it))
+            _assert_points(gpu.points()[0], ref, f"iteration {it + 1}")
+            _assert_density(gpu, pl, f"iteration {it + 1}")
+
+
+def test_gpu_case_resample_and_last_radius(bt, orc):
+    """The plain GPU workload (9,800 edges, ~1.3M samples): resampling and
+    density bit-exact; one iteration at the schedule's smallest radius."""
+    g = workloads.plain_gpu_graph()
+    opts = bt.Options(spacing=0.002, radius=0.01, sigma=2.0, density=1024)
+    pl = _oracle_pipeline(orc, g, opts)
+    with bt.Gpu(opts) as gpu:
+        assert gpu.upload(g) == len(pl.points)
+        _assert_points(gpu.points()[0], pl.points, "upload")
+        _assert_density(gpu, pl, "upload")
+        gpu.iterate()
+        ref = pl.iterate(opts.iter_params(0))
+        _assert_points(gpu.points()[0], ref, "iteration 1")
+        _assert_density(gpu, pl, "iteration 1")
+
+
+def test_deterministic_and_graph_replay(bt, cpu_case):
+    """Two eager runs agree bit for bit, and a hipGraph of the schedule
+    replayed three times (with rewinds and reads between) equals them."""
+    opts = workloads.plain_cpu_options()
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(cpu_case)
+        gpu.iterate(5)
+        a = gpu.points()[0]
+        da = gpu.density(3)
+        gpu.rewind()
+        gpu.iterate(5)
+        np.testing.assert_array_equal(gpu.points()[0].view(np.uint32), a.view(np.uint32))
+        for _ in range(3):
+            gpu.rewind()
+            _ = gpu.points()
+            gpu.iterate(5, graph=True)
+            np.testing.assert_array_equal(gpu.points()[0].view(np.uint32), a.view(np.uint32))
+            np.testing.assert_array_equal(gpu.density(3).view(np.uint32), da.view(np.uint32))
+
+
+def test_batched_equals_alone(bt, cpu_case):
+    """Jobs advanced together in one launch per stage give the same bits as
+    each job alone."""
+    graphs = workloads.batched_graphs(count=5, lo=50, hi=600)
+    opts = [bt.Options(spacing=0.004, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=256,
+                       schedule_len=4),
+            bt.Options(map_size=301, neighbor_radius=8),
+            workloads.plain_cpu_options(),
+            bt.Options(spacing=0.003, radius=0.03, sigma=3.0, density=(320, 200)),
+            bt.Options(spacing=0.006, radius=(0.08, 0.04), sigma=5.0, density=128,
+                       schedule_len=3)]
+    alone = []
+    for g, o in zip(graphs, opts):
+        with bt.Gpu(o) as gpu:
+            gpu.upload(g)
+            gpu.iterate(4)
+            alone.append((gpu.points()[0], gpu.density(3)))
+    with bt.Gpu(opts[0]) as gpu:
+        gpu.upload_batch(graphs, opts)
+        gpu.iterate(4, graph=True)
+        for j in range(len(graphs)):
+            p = gpu.points(j)[0]
+            np.testing.assert_array_equal(p.view(np.uint32), alone[j][0].view(np.uint32))
+        np.testing.assert_array_equal(gpu.density(3, job=0).view(np.uint32),
+                                      alone[0][1].view(np.uint32))
+
+
+def test_full_size_properties(bt):
+    """At the full plain GPU workload: fixed endpoints never move, samples
+    stay inside the initial bounding box, the grid partitions every sample."""
+    g = workloads.plain_gpu_graph()
+    opts = workloads.plain_gpu_options()
+    with bt.Gpu(opts) as gpu:
+        n = gpu.upload(g)
+        layout = gpu.distortion_layout()
+        base = gpu.distortion_snapshot(layout)
+        gpu.iterate(3, graph=True)
+        snap = gpu.distortion_snapshot(layout)
+        ends = np.concatenate([layout.paths[layout.edges, 0],
+                               layout.paths[layout.edges, 0] + layout.paths[layout.edges, 2] - 1])
+        np.testing.assert_array_equal(snap.positions[ends], base.positions[ends])
+        lo, hi = base.positions.min(0), base.positions.max(0)
+        assert np.all(snap.positions >= lo) and np.all(snap.positions <= hi)
+        cells, counts = gpu.grid()
+        assert counts.sum() == n
+        np.testing.assert_array_equal(np.bincount(cells, minlength=counts.size), counts.reshape(-1))
+        d = bt.distortion(base, snap, 1024)
+        assert 0 < d[0] < 0.1
+
+
+def test_error_paths(bt, cpu_case):
+    with bt.Gpu(bt.Options(map_size=201)) as gpu:
+        with pytest.raises(bt.BtError) as e:
+            gpu.iterate()
+        assert e.value.code == 2  # no uploaded graph
+        gpu.upload(cpu_case)
+        with pytest.raises(bt.BtError) as e:
+            gpu.iterate()
+        assert e.value.code == 5  # neighbor radius is not configured
+    with bt.Gpu(bt.Options(map_size=1600, neighbor_radius=4, max_graph_memory_mib=8)) as gpu:
+        with pytest.raises(bt.BtError) as e:
+            gpu.upload(workloads.plain_gpu_graph())
+        assert e.value.code == 3 and "sample buffer capacity" in e.value.msg
+
+
+def test_server_results_match_alone(bt):
+    graphs = workloads.batched_graphs(count=4, lo=60, hi=400)
+    opts = bt.Options(spacing=0.004, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=256,
+                      schedule_len=6)
+    iters = [6, 3, 5, 4]
+    alone = []
+    for g, it in zip(graphs, iters):
+        with bt.Gpu(opts) as gpu:
+            gpu.upload(g)
+            gpu.iterate(it)
+            alone.append(gpu.points()[0])
+    srv = bt.Server(opts, memory_budget=1 << 30, max_inflight=3)
+    tickets = [srv.submit(g, opts, it) for g, it in zip(graphs, iters)]
+    for _ in range(64):
+        r, q = srv.step()
+        if r == 0 and q == 0:
+            break
+    for t, ref in zip(tickets, alone):
+        res = srv.poll(t)
+        assert res.state == 2
+        p, _ = srv.points(t, res.samples)
+        np.testing.assert_array_equal(p.view(np.uint32), ref.view(np.uint32))
+    srv.close()

@@ -1,0 +1,140 @@
+"""Host-side logic of the product library (no GPU): the C ABI loads and
+exports every declared symbol; TRL parsing, normalization, the capacity
+model, blur weights, iteration parameters and distortion agree with the
+oracle and the reference's rules."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.test_oracle import TRL_CASES, TRL_ERRORS
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(bt):
+    import ctypes
+    hdr = open(os.path.join(ROOT, "include", "bundling.h")).read()
+    names = set(re.findall(r"^\s*(?:[a-z_0-9]+\s*\*?\s+)+\**(bt_[a-z_0-9]+)\(", hdr, re.M))
+    assert len(names) >= 30
+    L = ctypes.CDLL(os.path.join(ROOT, "bundling_thingy_amd", "libbundling.so"))
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    from bundling_thingy_amd.api import SIGNATURES
+    assert names == set(SIGNATURES), names ^ set(SIGNATURES)
+
+
+@pytest.mark.parametrize("text,pts,starts", TRL_CASES)
+def test_parse_matches_oracle(bt, orc, text, pts, starts):
+    g = bt.Graph.parse(text)
+    p, s, bbox = orc.parse_trl(text)
+    np.testing.assert_array_equal(g.points, p)
+    np.testing.assert_array_equal(g.starts, s)
+    np.testing.assert_array_equal(g.bbox, bbox)
+
+
+@pytest.mark.parametrize("text", TRL_ERRORS)
+def test_parse_errors(bt, text):
+    with pytest.raises(bt.BtError):
+        bt.Graph.parse(text)
+
+
+def test_parse_generated_trl_roundtrip(bt):
+    from bundling_thingy_amd import workloads
+    g = workloads.plain_cpu_graph()
+    h = bt.Graph.parse(workloads.to_trl(g))
+    np.testing.assert_array_equal(g.points, h.points)
+    np.testing.assert_array_equal(g.starts, h.starts)
+
+
+def test_normalization_matches_oracle(bt, orc):
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        bb = np.sort(rng.uniform(-5, 5, (2, 2)), axis=0).T.reshape(-1)[[0, 2, 1, 3]]
+        bb = bb.astype(np.float32)
+        n = bt.Graph(np.zeros((2, 2)), [0, 2], bb).normalization()
+        o = orc.normalization(bb)
+        assert (n.minimum, n.scale, n.translation) == ((o.min_x, o.min_y), o.scale, (o.tr_x, o.tr_y))
+
+
+def test_blur_weights_match_oracle(bt, orc):
+    for s in [2.0, 3.3, 4.0, 6.0, 8.0, 21.0]:
+        np.testing.assert_array_equal(bt.blur_weights(s), orc.blur_weights(s))
+
+
+def test_reference_mode_grid_dims(bt):
+    # neighbor_mean.rs:27: grid dims ceil(S / r) per axis
+    for S, r in [(1600, 10), (201, 10), (1600, 7), (512, 512), (100, 1)]:
+        p = bt.Options(map_size=S, neighbor_radius=r).iter_params(0)
+        assert (p.gx, p.gy) == (math.ceil(S / r),) * 2
+        assert p.res_x == S - 1 and p.radius == r and p.cell == r
+        assert p.sigma == 6.0 and p.taps == 18 and p.density_w == S
+
+
+@pytest.mark.parametrize("S,r", [(201, 10), (1600, 10), (1025, 30)])
+def test_generalized_reduces_to_reference_mode(bt, S, r):
+    ref = bt.Options(map_size=S, neighbor_radius=r).iter_params(0)
+    gen = bt.Options(spacing=1.0 / (S - 1), radius=r / (S - 1), sigma=6.0, density=S,
+                     schedule_len=1).iter_params(0)
+    for f in ["res_x", "res_y", "radius", "cell", "gx", "gy", "sigma", "taps", "density_w",
+              "density_h"]:
+        assert getattr(ref, f) == getattr(gen, f), f
+
+
+def test_schedule_is_geometric(bt):
+    o = bt.Options(spacing=0.002, radius=(0.06, 0.01), sigma=(8.0, 2.0), density=1024,
+                   schedule_len=30)
+    rad = [o.iter_params(i).radius for i in range(32)]
+    np.testing.assert_allclose(rad[0], 30.0, rtol=1e-6)
+    np.testing.assert_allclose(rad[29], 5.0, rtol=1e-6)
+    assert rad[30] == rad[29] == rad[31]
+    ratios = np.array(rad[1:30]) / np.array(rad[:29])
+    np.testing.assert_allclose(ratios, (1 / 6) ** (1 / 29), rtol=1e-6)
+    assert o.iter_params(0).taps == 24 and o.iter_params(29).taps == 6
+
+
+def test_capacity_model(bt):
+    o = bt.Options(neighbor_radius=10, max_graph_memory_mib=64)
+    cap, res = bt.capacity(o, 1000, 501)
+    assert res <= 64 << 20 and (64 << 20) - res < 48  # sample_bytes = 16 + 32
+    cap2, _ = bt.capacity(bt.Options(max_graph_memory_mib=64), 1000, 501)
+    assert cap2 > cap  # 16-byte samples without a radius
+    with pytest.raises(bt.BtError) as e:
+        bt.capacity(bt.Options(neighbor_radius=1, max_graph_memory_mib=1), 10, 3)
+    assert e.value.code == 3
+    with pytest.raises(bt.BtError):
+        bt.capacity(bt.Options(neighbor_radius=100, max_graph_memory_mib=1), 200000, 3)
+
+
+def test_distortion_matches_oracle(bt, orc):
+    rng = np.random.default_rng(4)
+    n = 400
+    a = rng.uniform(-0.9, 0.9, (n, 2)).astype(np.float32)
+    b = (a + rng.normal(0, 0.01, (n, 2))).astype(np.float32)
+    counts = np.array([0, 50, 150, 2, 198], dtype=np.uint32)
+    offs = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.uint64)
+    segs = np.concatenate([[0], np.cumsum(np.maximum(counts.astype(np.int64) - 1, 0))[:-1]])
+    paths = np.stack([offs, segs, counts], 1).astype(np.uint32)
+    paths[0] = 0
+    L = bt.Layout(paths, n)
+    d = bt.distortion(bt.Snapshot(L, a), bt.Snapshot(L, b), map_size=1600)
+    od, orms = orc.distortion(a, b, offs, counts)
+    assert d[0] == od and d[1] == orms and d[2] == orms * 800
+
+
+def test_layout_rejects_bad_reservations(bt):
+    ok = np.array([[0, 0, 3], [3, 2, 2], [0, 0, 0]], dtype=np.uint32)
+    L = bt.Layout(ok, 5)
+    assert L.edge_count == 2 and L.skipped == 1
+    with pytest.raises(bt.BtError):
+        bt.Layout(np.array([[0, 0, 3], [4, 2, 2]], dtype=np.uint32), 6)   # gap
+    with pytest.raises(bt.BtError):
+        bt.Layout(np.array([[0, 0, 1]], dtype=np.uint32), 1)              # one sample
+    rec = np.zeros(5, dtype=[("x", "<f4"), ("y", "<f4"), ("id", "<u4")])
+    rec["id"] = [0, 0, 0, 1, 1]
+    L.decode(rec.tobytes())
+    rec["id"][4] = 0
+    with pytest.raises(bt.BtError):
+        L.decode(rec.tobytes())
