@@ -103,6 +103,15 @@ def main():
     upload_ms = (time.perf_counter() - t) * 1e3
     log(f"upload: {n} samples in {upload_ms:.1f} ms")
 
+    # autotune the grid cell split per iteration (results are bit-identical
+    # under any split; only speed changes)
+    tune = None
+    if not args.cell_split:
+        t = time.perf_counter()
+        chosen, best = gpu.autotune(ITERS, (1, 2, 3, 4, 6, 8, 12))
+        tune = {"splits": chosen.tolist(), "seconds": time.perf_counter() - t}
+        log(f"autotune: {chosen.tolist()} in {tune['seconds']:.2f} s")
+
     # warmup (also builds the hipGraph of the schedule)
     t = time.perf_counter()
     for _ in range(max(args.warmup, 1)):
@@ -131,7 +140,13 @@ def main():
         for k, v in st.items():
             stage_ms[k] += v
         _, counts = gpu.grid()
-        flops += candidate_flops(counts)
+        p = gpu.job_params(i)
+        k = max(1, int(round(p.radius / p.cell)))
+        gy, gx = counts.shape
+        pad = np.zeros(((gy + k - 1) // k * k, (gx + k - 1) // k * k), dtype=np.int64)
+        pad[:gy, :gx] = counts
+        coarse = pad.reshape(pad.shape[0] // k, k, pad.shape[1] // k, k).sum(axis=(1, 3))
+        flops += candidate_flops(coarse)
     dom = max(stage_ms, key=stage_ms.get)
     mean_ms = stage_ms["neighbor_mean"] / ITERS
     P = 1024 * 1024
@@ -175,6 +190,7 @@ def main():
         "latency": {"p50_ms": float(np.percentile(np.array(times) * 1e3, 50)),
                     "p99_ms": float(np.percentile(np.array(times) * 1e3, 99)),
                     "upload_ms": upload_ms},
+        "autotune": tune,
         "stage_ms_per_schedule": stage_ms,
         "dominant_stage": dom,
         "roofline": roof,

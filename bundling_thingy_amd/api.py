@@ -92,6 +92,8 @@ SIGNATURES = {
     "bt_get_info": (C.c_int, [_P, C.POINTER(bt_info)]),
     "bt_iter_params_for": (C.c_int, [C.POINTER(bt_options), _U32, C.POINTER(bt_iter_params)]),
     "bt_profile": (C.c_int, [_P, _U32, _PD, _U32, _PD]),
+    "bt_autotune": (C.c_int, [_P, _U32, _PU32, _U32, _PU32, _PD]),
+    "bt_job_iter_params": (C.c_int, [_P, _U32, _U32, C.POINTER(bt_iter_params)]),
     "bt_upload_batch": (C.c_int, [_P, C.POINTER(bt_graph_desc), _U32, _PU32]),
     "bt_read_paths_job": (C.c_int, [_P, _U32, _PU32, _SZ]),
     "bt_read_points_job": (C.c_int, [_P, _U32, _P, _SZ]),
@@ -434,10 +436,25 @@ class Gpu:
         self._check(lib().bt_read_density_job(self._h, job, kind, out.ctypes.data_as(_P), w * h))
         return out.reshape(h, w)
 
+    def job_params(self, iteration, job=0):
+        """The exact parameters job `job` runs iteration `iteration` with."""
+        p = bt_iter_params()
+        self._check(lib().bt_job_iter_params(self._h, job, iteration, C.byref(p)))
+        return p
+
+    def autotune(self, iterations, candidates=(1, 2, 3, 4, 6, 8)):
+        """Times every candidate grid cell split per iteration; keeps the best."""
+        cand = np.ascontiguousarray(candidates, dtype=np.uint32)
+        chosen = np.zeros(iterations, dtype=np.uint32)
+        best = np.zeros(iterations, dtype=np.float64)
+        self._check(lib().bt_autotune(self._h, iterations, _u32p(cand), len(cand),
+                                      _u32p(chosen), best.ctypes.data_as(_PD)))
+        return chosen, best
+
     def grid(self):
         """(cell of each sample in Layout order, per-cell counts) of the last iteration."""
         n, it = self.job_samples(0)
-        p = self.options.iter_params(max(it - 1, 0))
+        p = self.job_params(max(it - 1, 0))
         cells = np.zeros(n, dtype=np.uint32)
         counts = np.zeros(p.gx * p.gy, dtype=np.uint32)
         self._check(lib().bt_read_grid(self._h, _u32p(cells), n, _u32p(counts), len(counts)))

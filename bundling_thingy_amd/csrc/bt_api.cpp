@@ -190,7 +190,7 @@ int bt_get_info(bt_ctx* ctx, bt_info* info) {
 int bt_iter_params_for(const bt_options* opts, uint32_t iteration, bt_iter_params* out) {
   if (!out) return BT_E_INVALID;
   bt_options o = bt::resolve_options(opts);
-  bt::iter_params(o, iteration, o.cell_split ? o.cell_split : 1, out);
+  bt::iter_params(o, iteration, o.cell_split, out);
   return BT_OK;
 }
 
@@ -198,6 +198,20 @@ int bt_profile(bt_ctx* ctx, uint32_t n, double* stage_ms, uint32_t n_stages, dou
   CTX_CHECK();
   if (!stage_ms && n_stages) return BT_E_INVALID;
   return guarded(&ctx->err, [&] { ctx->engine->profile(n, stage_ms, n_stages, total_ms); });
+}
+
+int bt_autotune(bt_ctx* ctx, uint32_t n, const uint32_t* candidates, uint32_t n_candidates,
+                uint32_t* chosen, double* best_ms) {
+  CTX_CHECK();
+  return guarded(&ctx->err, [&] {
+    ctx->engine->autotune(n, candidates, n_candidates, chosen, best_ms);
+  });
+}
+
+int bt_job_iter_params(bt_ctx* ctx, uint32_t job, uint32_t iteration, bt_iter_params* out) {
+  CTX_CHECK();
+  if (!out) return BT_E_INVALID;
+  return guarded(&ctx->err, [&] { *out = ctx->engine->job(job).at(iteration); });
 }
 
 int bt_trl_parse(const char* text, size_t len, float** xy, uint32_t* n_points, uint32_t** starts,

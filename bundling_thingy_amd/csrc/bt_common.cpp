@@ -79,8 +79,12 @@ static uint32_t grid_dim(float res, float cell) {
   return (uint32_t)g;
 }
 
+uint32_t default_split(float radius) {
+  const float k = std::floor(radius / 4.0f);
+  return (uint32_t)std::min(8.0f, std::max(1.0f, k));
+}
+
 void iter_params(const bt_options& o, uint32_t it, uint32_t split, bt_iter_params* p) {
-  if (split == 0) split = 1;
   float sigma;
   if (o.spacing > 0.0f) {
     const double res = resolution(o.spacing);
@@ -105,6 +109,7 @@ void iter_params(const bt_options& o, uint32_t it, uint32_t split, bt_iter_param
   float w[BT_MAX_TAPS + 1];
   p->taps = blur_weights(sigma, w, BT_MAX_TAPS);
   if (p->radius > 0.0f) {
+    if (split == 0) split = default_split(p->radius);
     p->cell = split == 1 ? p->radius : p->radius / (float)split;
     p->gx = grid_dim(p->res_x, p->cell);
     p->gy = grid_dim(p->res_y, p->cell);

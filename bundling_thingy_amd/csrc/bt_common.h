@@ -41,7 +41,11 @@ bt_options resolve_options(const bt_options* in);
 
 bool has_radius(const bt_options& o);
 
-// Parameters of iteration `it`; split = grid cells per radius (>= 1).
+// Grid cells per radius when none is given: cells about 4 texels wide, at
+// most 8 per radius (the autotuner refines it per iteration).
+uint32_t default_split(float radius);
+
+// Parameters of iteration `it`; split = grid cells per radius (0 = default).
 void iter_params(const bt_options& o, uint32_t it, uint32_t split, bt_iter_params* p);
 
 // Number of distinct parameter sets of the schedule (reference mode: 1).

@@ -33,6 +33,8 @@ struct __attribute__((aligned(16))) BtJob {
   uint2* cellrank;        // per slot: fine grid cell and rank within the cell
   uint32_t* cells;        // cell counts, then exclusive starts (max_cells + 1)
   uint32_t* bsum;         // per-scan-block partial sums (cell scan)
+  uint64_t* pre_x;        // exclusive prefix sums of the sorted fixed-point x (n + 1)
+  uint64_t* pre_y;        // same for y
   uint32_t* edge_of;      // per original index: path (edge) id
   uint32_t* dens_cnt;     // density counts (u32) W*H
   float* dens_row;        // horizontally blurred density
@@ -56,6 +58,9 @@ struct __attribute__((aligned(16))) BtJob {
   float res_x, res_y;     // texel resolution per axis (reference: map size - 1)
   float ts_x, ts_y;       // texel scale = res * 0.5
   float cell;             // cell side in texels
+  float inv_cell;         // 1 / cell (window bounds only; binning divides)
+  float radius;           // texels
+  float eps;              // geometric safety margin of the run classification, texels
   float r2;               // radius^2 (texels^2)
   float reach_x, reach_y; // window half-width: radius + 4 * pad
   float norm_min_x, norm_min_y, norm_scale, norm_tr_x, norm_tr_y;  // Graph::normalization
@@ -77,6 +82,7 @@ enum BtKind {
   BT_K_PATH,     // resample count: 256 paths per workgroup
   BT_K_EMIT,     // resample emit: BT_EMIT_PATHS paths per workgroup
   BT_K_PSCAN,    // path scan: BT_SCAN_ITEMS paths per workgroup
+  BT_K_PTS,      // sample prefix scan: BT_SCAN_ITEMS samples per workgroup
   BT_K_COUNT
 };
 

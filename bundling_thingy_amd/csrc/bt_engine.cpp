@@ -27,6 +27,8 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   d.cellrank = cellrank.as<uint2>();
   d.cells = cells.as<uint32_t>();
   d.bsum = bsum.as<uint32_t>();
+  d.pre_x = pre_x.as<uint64_t>();
+  d.pre_y = pre_y.as<uint64_t>();
   d.edge_of = edge_of.as<uint32_t>();
   d.dens_cnt = dens_cnt.as<uint32_t>();
   d.dens_row = dens_row.as<float>();
@@ -53,6 +55,9 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   d.ts_x = p.res_x * 0.5f;
   d.ts_y = p.res_y * 0.5f;
   d.cell = p.cell;
+  d.inv_cell = p.cell > 0.0f ? 1.0f / p.cell : 0.0f;
+  d.radius = p.radius;
+  d.eps = std::max(1e-3f * p.cell, (std::max(p.res_x, p.res_y) + 1.0f) * 4e-6f);
   d.r2 = p.radius * p.radius;
   const float pad_x = (p.res_x + 1.0f) * 2.4e-7f, pad_y = (p.res_y + 1.0f) * 2.4e-7f;
   d.reach_x = p.radius + pad_x * 4.0f;
@@ -68,9 +73,27 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   return d;
 }
 
+bool Job::set_split(uint32_t e, uint32_t k) {
+  splits[e] = k;
+  iter_params(o, e, k, &params[e]);
+  const uint32_t nc = params[e].gx * params[e].gy;
+  if (nc <= max_cells && cells.p) return false;
+  max_cells = std::max(max_cells, nc);
+  if (max_cells + 1ull > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS)
+    fail(BT_E_CAPACITY, "neighbor grid exceeds GPU dispatch limits");
+  if (cells.p) {
+    cells.alloc(((size_t)max_cells + 1) * 4);
+    const size_t nb = cdiv(max_cells + 1ull, BT_SCAN_ITEMS);
+    if (nb * 8 > bsum.bytes) bsum.alloc(nb * 8);
+    return true;
+  }
+  return false;
+}
+
 size_t Job::device_bytes() const {
   const DevBuf* all[] = {&ctrl, &starts, &path_scan, &paths, &bsum, &status, &rec0,
                          &rec1, &rec_saved, &sorted, &sid, &cellrank, &cells, &edge_of,
+                         &pre_x, &pre_y,
                          &dens_cnt, &dens_row, &dens_col, &dens_norm, &dens_saved, &peak, &wt};
   size_t s = 0;
   for (const DevBuf* b : all) s += b->bytes;
@@ -141,6 +164,7 @@ static uint32_t kind_blocks(int k, const Job& j, const BtJob& d) {
     case BT_K_PATH: return cdiv(j.n_paths + 1ull, 256);
     case BT_K_EMIT: return cdiv(j.n_paths, BT_EMIT_PATHS);
     case BT_K_PSCAN: return cdiv(j.n_paths + 1ull, BT_SCAN_ITEMS);
+    case BT_K_PTS: return cdiv(j.n + 1ull, BT_SCAN_ITEMS);
   }
   return 0;
 }
@@ -205,6 +229,7 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   btk::tasks(stream_, p.grid[BT_K_TASK], J, B(BT_K_TASK), (int)nj, ctl, tasks_.as<BtTask>());
   mark(4);
   btk::scatter(stream_, p.grid[BT_K_BIN], J, B(BT_K_BIN), (int)nj);
+  btk::pre_scan(stream_, p.grid[BT_K_PTS], J, B(BT_K_PTS), (int)nj);
   mark(5);
   btk::neighbor_mean(stream_, (uint32_t)cus_ * 8u, J, tasks_.as<BtTask>(), ctl);
   mark(6);
@@ -263,12 +288,15 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
     if (rc != BT_OK) fail(rc, e);
     j->n_ctrl = g.n_points;
     j->n_paths = g.n_starts - 1;
-    j->split = j->o.cell_split ? j->o.cell_split : 1;
     j->sched = schedule_len(j->o);
     j->params.resize(j->sched);
+    j->splits.resize(j->sched);
     std::vector<float> wt((size_t)j->sched * kWt, 0.0f);
     for (uint32_t i = 0; i < j->sched; ++i) {
-      iter_params(j->o, i, j->split, &j->params[i]);
+      iter_params(j->o, i, j->o.cell_split, &j->params[i]);
+      j->splits[i] = j->params[i].radius > 0.0f
+                         ? (uint32_t)std::lround(j->params[i].radius / j->params[i].cell)
+                         : 1u;
       j->max_cells = std::max(j->max_cells, j->params[i].gx * j->params[i].gy);
       blur_weights(j->params[i].sigma, wt.data() + (size_t)i * kWt, BT_MAX_TAPS);
     }
@@ -338,6 +366,11 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
     j.cellrank.alloc((size_t)n * 8);
     j.edge_of.alloc((size_t)n * 4);
     j.cells.alloc(((size_t)j.max_cells + 1) * 4);
+    j.pre_x.alloc(((size_t)n + 1) * 8);
+    j.pre_y.alloc(((size_t)n + 1) * 8);
+    if (n + 1 > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS) fail(BT_E_CAPACITY, "too many samples");
+    const size_t nb = std::max<size_t>(j.bsum.bytes / 8, cdiv(n + 1, BT_SCAN_ITEMS));
+    if (nb * 8 > j.bsum.bytes) j.bsum.alloc(nb * 8);
     if (j.o.flags & BT_FLAG_DENSITY) {
       const size_t pb = (size_t)j.dw * j.dh * 4;
       j.dens_cnt.alloc(pb);
@@ -559,6 +592,91 @@ void Engine::profile(uint32_t n, double* stage_ms, uint32_t nst, double* total_m
   for (auto& e : ev) (void)hipEventDestroy(e);
   for (uint32_t s = 0; s < nst && s < 9; ++s) stage_ms[s] = acc[s];
   if (total_ms) *total_ms = tot;
+}
+
+void Engine::autotune(uint32_t n, const uint32_t* cand, uint32_t ncand, uint32_t* chosen,
+                      double* best_ms) {
+  std::vector<Job*> js = all_jobs();
+  check_iterable(js);
+  if (!cand || ncand == 0) fail(BT_E_INVALID, "no autotune candidates");
+  uint32_t sched = 1;
+  for (Job* j : js) sched = std::max(sched, j->sched);
+  n = std::min(n, sched);
+  rewind();
+  drop_graphs();
+  std::vector<std::unique_ptr<DevBuf>> snap;
+  for (Job* j : js) {
+    snap.push_back(std::make_unique<DevBuf>());
+    snap.back()->alloc((size_t)j->n * 12);
+  }
+  auto cur = [](Job* j) { return j->parity ? j->rec1.p : j->rec0.p; };
+  auto restore = [&]() {
+    for (size_t k = 0; k < js.size(); ++k)
+      HIPCHK(hipMemcpyAsync(cur(js[k]), snap[k]->p, (size_t)js[k]->n * 12,
+                            hipMemcpyDeviceToDevice, stream_));
+  };
+  bool density = false;
+  for (Job* j : js) density |= j->dens_cnt.p != nullptr;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto run_one = [&](bool timed) -> double {
+    ensure_tasks(js);
+    std::vector<uint32_t> it;
+    for (Job* j : js) it.push_back(j->iteration);
+    Plan p = make_plan(js, it);
+    const size_t slot = align_up(p.host.size(), 256);
+    if (2 * slot > plans_.bytes) {
+      sync();
+      plans_.reserve(2 * slot + 4096);
+    }
+    eager_slot_ ^= 1u;
+    unsigned char* dev = plans_.as<unsigned char>() + eager_slot_ * slot;
+    put_plan(p, dev);
+    HIPCHK(hipEventRecord(e0, stream_));
+    launch_iteration(p, dev, density);
+    HIPCHK(hipEventRecord(e1, stream_));
+    sync();
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)timed;
+    return ms;
+  };
+  try {
+    for (uint32_t i = 0; i < n; ++i) {
+      for (size_t k = 0; k < js.size(); ++k)
+        HIPCHK(hipMemcpyAsync(snap[k]->p, cur(js[k]), (size_t)js[k]->n * 12,
+                              hipMemcpyDeviceToDevice, stream_));
+      double best = 1e300;
+      uint32_t bk = cand[0];
+      for (uint32_t c = 0; c < ncand; ++c) {
+        if (cand[c] == 0) continue;
+        restore();
+        for (Job* j : js) j->set_split(j->entry(i), cand[c]);
+        const double t = run_one(true);
+        if (t < best) {
+          best = t;
+          bk = cand[c];
+        }
+      }
+      restore();
+      for (Job* j : js) j->set_split(j->entry(i), bk);
+      (void)run_one(false);
+      for (Job* j : js) {
+        ++j->iteration;
+        j->parity ^= 1u;
+      }
+      if (chosen) chosen[i] = bk;
+      if (best_ms) best_ms[i] = best;
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  rewind();
 }
 
 void Engine::read_paths(uint32_t job_i, uint32_t* rec3, size_t n) {

@@ -19,15 +19,18 @@ struct Job {
   float norm[5] = {0, 0, 0, 0, 0};
   uint32_t dw = 0, dh = 0;
   uint32_t max_cells = 1;
-  uint32_t split = 1;           // grid cells per radius
   uint32_t iteration = 0, parity = 0;
   uint32_t sched = 1;           // distinct parameter sets
   std::vector<bt_iter_params> params;
+  std::vector<uint32_t> splits; // grid cells per radius, per schedule entry
   DevBuf ctrl, starts, path_scan, paths, bsum, status;
-  DevBuf rec0, rec1, rec_saved, sorted, sid, cellrank, cells, edge_of;
+  DevBuf rec0, rec1, rec_saved, sorted, sid, cellrank, cells, edge_of, pre_x, pre_y;
   DevBuf dens_cnt, dens_row, dens_col, dens_norm, dens_saved, peak, wt;
 
   const bt_iter_params& at(uint32_t it) const { return params[it < sched ? it : sched - 1]; }
+  uint32_t entry(uint32_t it) const { return it < sched ? it : sched - 1; }
+  // Sets schedule entry e's cell split; returns true if the cell buffers grew.
+  bool set_split(uint32_t e, uint32_t k);
   BtJob dev(uint32_t it, uint32_t parity_at) const;
   size_t device_bytes() const;
 };
@@ -63,6 +66,10 @@ class Engine {
   void iterate_jobs(const std::vector<Job*>& js);
   void rewind();
   void profile(uint32_t n, double* stage_ms, uint32_t nst, double* total_ms);
+  // Picks the grid cell split of each of the first n iterations by timing
+  // every candidate from the same state; leaves the jobs rewound.
+  void autotune(uint32_t n, const uint32_t* cand, uint32_t ncand, uint32_t* chosen,
+                double* best_ms);
 
   void read_paths(uint32_t job, uint32_t* rec3, size_t n);
   void read_points(uint32_t job, void* rec12, size_t n);

@@ -132,6 +132,10 @@ extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restri
   }
   if (lb == 0 && threadIdx.x == 0) {
     J.peak[0] = 0u;
+    if (J.pre_x) {
+      J.pre_x[J.n] = 0ull;
+      J.pre_y[J.n] = 0ull;
+    }
     if (j == 0) {
       ctl->task_total = 0u;
       ctl->queue_head = 0u;
@@ -332,31 +336,104 @@ extern "C" __global__ __launch_bounds__(256) void bt_scatter(const BtJob* __rest
   int fx = (int)rintf(x * BT_FIX_ONE), fy = (int)rintf(y * BT_FIX_ONE);
   J.sorted[slot] = make_float4(x, y, __int_as_float(fx), __int_as_float(fy));
   J.sid[slot] = id;
+  J.pre_x[slot] = (uint64_t)(int64_t)fx;
+  J.pre_y[slot] = (uint64_t)(int64_t)fy;
+}
+
+// Exclusive prefix sums of the sorted fixed-point coordinates (two's
+// complement int64 in u64 lanes), so any run of sorted samples sums in O(1).
+extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_reduce(
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs, int axis) {
+  JOB_PROLOGUE(BT_K_PTS);
+  scan_reduce_body<unsigned long long>((const unsigned long long*)(axis ? J.pre_y : J.pre_x),
+                                       J.n + 1u, lb, (unsigned long long*)J.bsum);
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_bsum(const BtJob* __restrict__ jobs,
+                                                                   int njobs) {
+  const BtJob& J = jobs[blockIdx.x];
+  const uint32_t nb = (J.n + 1u + BT_SCAN_ITEMS - 1u) / BT_SCAN_ITEMS;
+  scan_bsum_body<unsigned long long>((unsigned long long*)J.bsum, nb);
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_apply(
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs, int axis) {
+  JOB_PROLOGUE(BT_K_PTS);
+  scan_apply_body<unsigned long long>((unsigned long long*)(axis ? J.pre_y : J.pre_x), J.n + 1u,
+                                      lb, (const unsigned long long*)J.bsum);
 }
 
 // ---------------------------------------------------------- neighbor mean
 // neighbor_mean.wgsl:75-203. One wave per task (<= 64 query samples of one
-// cell). The wave's cell window is the union of the per-sample windows
-// [texel(p) - reach, texel(p) + reach]; within each window row the cells are
-// contiguous in sorted order, so the candidates form one range, streamed
-// through scalar loads and tested by every lane against its own sample:
-//   t = (q - p) * texel_scale; accept when fmaf(tx, tx, ty*ty) <= r^2.
-// Accepted neighbors add their fixed-point coordinates to exact int64 sums.
+// cell). For every grid row the task's window touches, the wave (uniformly)
+// splits the row's cells into
+//   - cells fully inside the disk of every query sample (conservative, with
+//     an eps margin): their samples all pass the neighbor test, so their
+//     fixed-point sums come in O(1) from the prefix sums of the sorted
+//     coordinates (the role of the reference's CellSummary, :56-59);
+//   - boundary cells, whose samples are streamed through scalar loads and
+//     tested by every lane against its own sample:
+//       t = (q - p) * texel_scale; accept when fmaf(tx, tx, ty*ty) <= r^2;
+//   - cells beyond every disk, skipped.
+// Within a row the cells are contiguous in sorted order, so each part is one
+// range. All sums are exact int64, so the result equals the all-pairs test.
 // Output is written in sorted order (the next iteration's input order), and
 // every sample, fixed or not, is splatted into the density counts.
+__device__ __forceinline__ void scan_range(const float4* S, uint32_t b, uint32_t e, float px,
+                                           float py, float tsx, float tsy, float r2,
+                                           long long& ax, long long& ay, int& n) {
+  for (uint32_t c0 = b; c0 < e; c0 += 64u) {
+    const uint32_t c1 = min(c0 + 64u, e);
+    int sx = 0, sy = 0, sc = 0;
+    uint32_t k = c0;
+    for (; k + 4u <= c1; k += 4u) {
+#pragma unroll
+      for (uint32_t u = 0; u < 4u; ++u) {
+        const float4 s = ld_uniform(S, k + u);
+        const float dx = (s.x - px) * tsx;
+        const float dy = (s.y - py) * tsy;
+        const bool ok = fmaf(dx, dx, dy * dy) <= r2;
+        sx += ok ? __float_as_int(s.z) : 0;
+        sy += ok ? __float_as_int(s.w) : 0;
+        sc += ok ? 1 : 0;
+      }
+    }
+    for (; k < c1; ++k) {
+      const float4 s = ld_uniform(S, k);
+      const float dx = (s.x - px) * tsx;
+      const float dy = (s.y - py) * tsy;
+      const bool ok = fmaf(dx, dx, dy * dy) <= r2;
+      sx += ok ? __float_as_int(s.z) : 0;
+      sy += ok ? __float_as_int(s.w) : 0;
+      sc += ok ? 1 : 0;
+    }
+    ax += sx;
+    ay += sy;
+    n += sc;
+  }
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Cell index of a texel coordinate for window bounds (multiplies by 1/cell;
+// the eps margins absorb the difference from the binning's division).
+__device__ __forceinline__ int cell_of_fast(float t, float inv_cell, uint32_t g) {
+  float v = fminf(fmaxf(t * inv_cell, 0.0f), (float)(g - 1u));
+  return (int)v;
+}
+
 extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl) {
   const uint32_t lane = LANE();
   for (;;) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(&ctl->queue_head, 1u);
-    t = __builtin_amdgcn_readfirstlane(t);
-    const uint32_t total = __builtin_amdgcn_readfirstlane(
-        __atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
+    t = uni(t);
+    const uint32_t total = uni(__atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
     if (t >= total) break;
-    const uint32_t tj = __builtin_amdgcn_readfirstlane(tasks[t].job);
-    const uint32_t ts = __builtin_amdgcn_readfirstlane(tasks[t].start);
-    const uint32_t te = __builtin_amdgcn_readfirstlane(tasks[t].end);
+    const uint32_t tj = uni(tasks[t].job);
+    const uint32_t ts = uni(tasks[t].start);
+    const uint32_t te = uni(tasks[t].end);
     const BtJob& J = jobs[tj];
     const uint32_t slot = ts + lane;
     const bool valid = slot < te;
@@ -371,61 +448,69 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
     float mx = px, my = py;
     if (ballot(movable)) {
       const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2;
-      const float lo_x = wave_min(movable ? px : 3.0e38f);
-      const float lo_y = wave_min(movable ? py : 3.0e38f);
-      const float hi_x = wave_max(movable ? px : -3.0e38f);
-      const float hi_y = wave_max(movable ? py : -3.0e38f);
-      // The reductions leave the same value in every lane; readfirstlane makes
-      // the window (and so every candidate address) wave-uniform.
-      const uint32_t x0 = __builtin_amdgcn_readfirstlane(
-          cell_axis(texel(lo_x, J.res_x) - J.reach_x, J.cell, J.gx));
-      const uint32_t x1 = __builtin_amdgcn_readfirstlane(
-          cell_axis(texel(hi_x, J.res_x) + J.reach_x, J.cell, J.gx));
-      const uint32_t y0 = __builtin_amdgcn_readfirstlane(
-          cell_axis(texel(lo_y, J.res_y) - J.reach_y, J.cell, J.gy));
-      const uint32_t y1 = __builtin_amdgcn_readfirstlane(
-          cell_axis(texel(hi_y, J.res_y) + J.reach_y, J.cell, J.gy));
+      const float c = J.cell, inv_c = J.inv_cell, eps = J.eps, r = J.radius;
+      const uint32_t gx = J.gx, gy = J.gy;
+      // query bbox in texels (the same texel() as the binning; monotone)
+      const float tx0 = __uint_as_float(uni(__float_as_uint(
+          texel(wave_min(movable ? px : 3.0e38f), J.res_x))));
+      const float tx1 = __uint_as_float(uni(__float_as_uint(
+          texel(wave_max(movable ? px : -3.0e38f), J.res_x))));
+      const float ty0 = __uint_as_float(uni(__float_as_uint(
+          texel(wave_min(movable ? py : 3.0e38f), J.res_y))));
+      const float ty1 = __uint_as_float(uni(__float_as_uint(
+          texel(wave_max(movable ? py : -3.0e38f), J.res_y))));
+      const float rr = r + eps, ri = r - eps;
+      const uint32_t cy0 = uni((uint32_t)cell_of_fast(ty0 - rr, inv_c, gy));
+      const uint32_t cy1 = uni((uint32_t)cell_of_fast(ty1 + rr, inv_c, gy));
       const float4* S = J.sorted;
       const uint32_t* cells = J.cells;
-      long long ax = 0, ay = 0;
+      const uint64_t* PX = J.pre_x;
+      const uint64_t* PY = J.pre_y;
+      long long ax = 0, ay = 0;  // per-lane sums of scanned neighbors
       int n = 0;
-      for (uint32_t cy = y0; cy <= y1; ++cy) {
-        const uint32_t b = ld_uniform(cells, cy * J.gx + x0);
-        const uint32_t e = ld_uniform(cells, cy * J.gx + x1 + 1u);
-        for (uint32_t c0 = b; c0 < e; c0 += 64u) {
-          const uint32_t c1 = min(c0 + 64u, e);
-          int sx = 0, sy = 0, sc = 0;
-          uint32_t k = c0;
-          for (; k + 4u <= c1; k += 4u) {
-#pragma unroll
-            for (uint32_t u = 0; u < 4u; ++u) {
-              const float4 s = ld_uniform(S, k + u);
-              const float dx = (s.x - px) * tsx;
-              const float dy = (s.y - py) * tsy;
-              const bool ok = fmaf(dx, dx, dy * dy) <= r2;
-              sx += ok ? __float_as_int(s.z) : 0;
-              sy += ok ? __float_as_int(s.w) : 0;
-              sc += ok ? 1 : 0;
-            }
-          }
-          for (; k < c1; ++k) {
-            const float4 s = ld_uniform(S, k);
-            const float dx = (s.x - px) * tsx;
-            const float dy = (s.y - py) * tsy;
-            const bool ok = fmaf(dx, dx, dy * dy) <= r2;
-            sx += ok ? __float_as_int(s.z) : 0;
-            sy += ok ? __float_as_int(s.w) : 0;
-            sc += ok ? 1 : 0;
-          }
-          ax += sx;
-          ay += sy;
-          n += sc;
+      unsigned long long ux = 0, uy = 0;  // wave-uniform sums of inside runs
+      uint32_t un = 0;
+      for (uint32_t cy = cy0; cy <= cy1; ++cy) {
+        const float Y0 = (float)cy * c, Y1 = Y0 + c;
+        const float dyn = fmaxf(fmaxf(Y0 - eps - ty1, ty0 - (Y1 + eps)), 0.0f);
+        if (dyn >= rr) continue;
+        const float wout = __builtin_sqrtf(rr * rr - dyn * dyn);
+        const uint32_t xlo = uni((uint32_t)cell_of_fast(tx0 - wout, inv_c, gx));
+        const uint32_t xhi = uni((uint32_t)cell_of_fast(tx1 + wout, inv_c, gx));
+        int xa = 1, xb = 0;
+        const float dyf = fmaxf(Y1 + eps - ty0, ty1 - (Y0 - eps));
+        if (cy > 0u && cy + 1u < gy && dyf < ri) {
+          const float win = __builtin_sqrtf(ri * ri - dyf * dyf);
+          const float fa = ceilf((tx1 - win + eps) * inv_c);
+          const float fb = floorf((tx0 + win - eps) * inv_c) - 1.0f;
+          xa = (int)fminf(fmaxf(fa, fmaxf((float)xlo, 1.0f)), (float)gx);
+          xb = (int)fmaxf(fminf(fb, fminf((float)xhi, (float)gx - 2.0f)), -1.0f);
+        }
+        xa = (int)uni((uint32_t)xa);
+        xb = (int)uni((uint32_t)xb);
+        const uint32_t row = cy * gx;
+        const uint32_t s_lo = ld_uniform(cells, row + xlo);
+        const uint32_t s_hi = ld_uniform(cells, row + xhi + 1u);
+        if (xa <= xb) {
+          const uint32_t s_a = ld_uniform(cells, row + (uint32_t)xa);
+          const uint32_t s_b = ld_uniform(cells, row + (uint32_t)xb + 1u);
+          ux += ((const __attribute__((address_space(4))) unsigned long long*)(uintptr_t)PX)[s_b] -
+                ((const __attribute__((address_space(4))) unsigned long long*)(uintptr_t)PX)[s_a];
+          uy += ((const __attribute__((address_space(4))) unsigned long long*)(uintptr_t)PY)[s_b] -
+                ((const __attribute__((address_space(4))) unsigned long long*)(uintptr_t)PY)[s_a];
+          un += s_b - s_a;
+          scan_range(S, s_lo, s_a, px, py, tsx, tsy, r2, ax, ay, n);
+          scan_range(S, s_b, s_hi, px, py, tsx, tsy, r2, ax, ay, n);
+        } else {
+          scan_range(S, s_lo, s_hi, px, py, tsx, tsy, r2, ax, ay, n);
         }
       }
       if (movable) {
+        const long long tn = (long long)n + (long long)un;
+        const long long sx = ax + (long long)ux, sy = ay + (long long)uy;
         const long long pfx = (long long)__float_as_int(q.z), pfy = (long long)__float_as_int(q.w);
-        const long long dxs = ax - (long long)n * pfx, dys = ay - (long long)n * pfy;
-        const double den = (double)n * 16777216.0;
+        const long long dxs = sx - tn * pfx, dys = sy - tn * pfy;
+        const double den = (double)tn * 16777216.0;
         mx = (float)((double)px + (double)dxs / den);
         my = (float)((double)py + (double)dys / den);
       }
@@ -795,6 +880,13 @@ void cell_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
   hipLaunchKernelGGL(bt_cell_scan_reduce, L256(grid), jobs, base, nj);
   hipLaunchKernelGGL(bt_cell_scan_bsum, L256((uint32_t)nj), jobs, nj);
   hipLaunchKernelGGL(bt_cell_scan_apply, L256(grid), jobs, base, nj);
+}
+void pre_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  for (int axis = 0; axis < 2; ++axis) {
+    hipLaunchKernelGGL(bt_pre_scan_reduce, L256(grid), jobs, base, nj, axis);
+    hipLaunchKernelGGL(bt_pre_scan_bsum, L256((uint32_t)nj), jobs, nj);
+    hipLaunchKernelGGL(bt_pre_scan_apply, L256(grid), jobs, base, nj, axis);
+  }
 }
 void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
            BtControl* ctl, BtTask* tasks_out) {

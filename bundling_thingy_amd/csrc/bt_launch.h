@@ -12,6 +12,7 @@ void cell_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
 void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
            BtControl* ctl, BtTask* tasks_out);
 void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void pre_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
                    BtControl* ctl);
 void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);

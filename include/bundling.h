@@ -139,6 +139,17 @@ int bt_iter_params_for(const bt_options* opts, uint32_t iteration, bt_iter_param
  * hipEvents around each stage; returns total ms. */
 int bt_profile(bt_ctx* ctx, uint32_t n, double* stage_ms, uint32_t n_stages, double* total_ms);
 
+/* Autotuner (no counterpart in the reference): for each of the first n
+ * iterations, times one iteration under every candidate grid cell split
+ * (cells per radius) from the same state, keeps the fastest in chosen[i]
+ * (best_ms[i] its time) and uses it from then on, including in hipGraphs.
+ * Results are bit-identical under any split. Leaves the jobs rewound. */
+int bt_autotune(bt_ctx* ctx, uint32_t n, const uint32_t* candidates, uint32_t n_candidates,
+                uint32_t* chosen, double* best_ms);
+
+/* Parameters job `job` runs iteration `iteration` with (after autotuning). */
+int bt_job_iter_params(bt_ctx* ctx, uint32_t job, uint32_t iteration, bt_iter_params* out);
+
 /* ---------------------------------------------------------------- batched */
 
 /* One graph of a batch. */

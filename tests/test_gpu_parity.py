@@ -106,6 +106,41 @@ def test_gpu_case_resample_and_last_radius(bt, orc):
         _assert_density(gpu, pl, "iteration 1")
 
 
+@pytest.mark.parametrize("split", [1, 2, 3, 5, 8, 16])
+def test_cell_split_does_not_change_results(bt, orc, cpu_case, split):
+    """Any grid cell size gives the same bits (inside runs + scanned borders
+    equal the all-pairs test), including the reference grid (split 1)."""
+    base = workloads.plain_cpu_options()
+    opts = bt.Options(spacing=0.005, radius=0.05, sigma=4.0, density=512, schedule_len=10,
+                      cell_split=split)
+    pl = _oracle_pipeline(orc, cpu_case, base)
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(cpu_case)
+        for it in range(2):
+            gpu.iterate()
+            ref = pl.iterate(base.iter_params(it))
+            _assert_points(gpu.points()[0], ref, f"split {split} iteration {it + 1}")
+        if split == 1:
+            cells, counts = gpu.grid()
+            assert counts.shape == (21, 21)  # ceil(201 / 10): grid of cell = radius
+
+
+def test_autotune_keeps_results(bt, cpu_case):
+    opts = workloads.plain_cpu_options()
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(cpu_case)
+        gpu.iterate(4)
+        ref = gpu.points()[0]
+        chosen, best = gpu.autotune(4, (1, 2, 4, 8))
+        assert len(chosen) == 4 and np.all(np.isin(chosen, [1, 2, 4, 8])) and np.all(best > 0)
+        assert gpu.job_samples(0)[1] == 0  # rewound
+        gpu.iterate(4, graph=True)
+        np.testing.assert_array_equal(gpu.points()[0].view(np.uint32), ref.view(np.uint32))
+        for it in range(4):
+            p = gpu.job_params(it)
+            assert abs(p.radius / p.cell - chosen[it]) < 1e-3
+
+
 def test_deterministic_and_graph_replay(bt, cpu_case):
     """Two eager runs agree bit for bit, and a hipGraph of the schedule
     replayed three times (with rewinds and reads between) equals them."""

@@ -67,7 +67,7 @@ def test_blur_weights_match_oracle(bt, orc):
 def test_reference_mode_grid_dims(bt):
     # neighbor_mean.rs:27: grid dims ceil(S / r) per axis
     for S, r in [(1600, 10), (201, 10), (1600, 7), (512, 512), (100, 1)]:
-        p = bt.Options(map_size=S, neighbor_radius=r).iter_params(0)
+        p = bt.Options(map_size=S, neighbor_radius=r, cell_split=1).iter_params(0)
         assert (p.gx, p.gy) == (math.ceil(S / r),) * 2
         assert p.res_x == S - 1 and p.radius == r and p.cell == r
         assert p.sigma == 6.0 and p.taps == 18 and p.density_w == S
