@@ -25,6 +25,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 ITERS = 30
+TUNE_CANDIDATES = (1, 2, 3, 4, 6, 8, 12, 16)
+COPY_PROBE_BYTES = 256 << 20
 
 
 def log(*a):
@@ -78,6 +80,129 @@ def cpu_baseline(graph, opts, target_s=15.0):
                       f"est. {total:.1f} s per schedule"}
 
 
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def batched_section(bt, workloads, n_graphs, iters=30):
+    """256 independent graphs: the plain per-graph drop-in (upload, 30 eager
+    iterates, readback, one graph at a time) against one batched job table
+    (one launch per stage per iteration, the schedule replayed as a hipGraph)."""
+    t = time.perf_counter()
+    graphs = workloads.batched_graphs(n_graphs)
+    opts = [workloads.batched_options(i, iters) for i in range(n_graphs)]
+    log(f"batched: generated {n_graphs} graphs in {time.perf_counter() - t:.1f} s")
+    gpu = bt.Gpu(opts[0])
+    # warm the code paths once
+    gpu.upload_batch(graphs[:2], opts[:2])
+    gpu.iterate(2)
+    # plain per-graph form
+    lat, samples = [], []
+    t0 = time.perf_counter()
+    for g, o in zip(graphs, opts):
+        ts = time.perf_counter()
+        s = gpu.upload_batch([g], [o])
+        gpu.iterate(iters)
+        gpu.points(0)
+        lat.append((time.perf_counter() - ts) * 1e3)
+        samples.append(int(s[0]))
+    plain_s = time.perf_counter() - t0
+    updates = iters * sum(samples)
+    log(f"batched: plain per-graph {plain_s:.2f} s")
+    # batched form
+    t0 = time.perf_counter()
+    s = gpu.upload_batch(graphs, opts)
+    up_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    gpu.iterate(iters, graph=True)  # first call captures + instantiates the hipGraph
+    first_s = time.perf_counter() - t1
+    gpu.rewind()
+    t1 = time.perf_counter()
+    gpu.iterate(iters, graph=True)
+    run_s = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    for j in range(n_graphs):
+        gpu.points(j)
+    read_s = time.perf_counter() - t1
+    batch_s = up_s + run_s + read_s
+    gpu.close()
+    log(f"batched: upload {up_s:.3f} s, schedule {run_s:.3f} s (first {first_s:.3f} s), "
+        f"readback {read_s:.3f} s")
+    assert int(s.sum()) == sum(samples)
+    return {"graphs": n_graphs, "samples": int(sum(samples)), "iterations": iters,
+            "plain": {"updates_per_s": updates / plain_s, "seconds": plain_s,
+                      "job_p50_ms": pct(lat, 50), "job_p99_ms": pct(lat, 99)},
+            "batched": {"updates_per_s": updates / batch_s, "seconds": batch_s,
+                        "upload_s": up_s, "schedule_s": run_s, "readback_s": read_s,
+                        "graph_build_s": first_s - run_s,
+                        "job_p50_ms": batch_s * 1e3, "job_p99_ms": batch_s * 1e3},
+            "speedup": plain_s / batch_s}
+
+
+def served_section(bt, workloads, n_jobs, rate_hz=200.0):
+    """Poisson arrivals at rate_hz: the plain per-job form (FIFO, one job at a
+    time, as the drop-in does) against the continuous-batching server."""
+    t = time.perf_counter()
+    jobs = workloads.served_jobs(n_jobs, rate_hz)
+    log(f"served: generated {n_jobs} jobs in {time.perf_counter() - t:.1f} s")
+    total_updates = 0
+    # plain per-job FIFO
+    gpu = bt.Gpu()
+    gpu.upload_batch([jobs[0][1]], [jobs[0][2]])
+    gpu.iterate(2)
+    lat = []
+    t0 = time.perf_counter()
+    for ta, g, o, it in jobs:
+        now = time.perf_counter() - t0
+        if now < ta:
+            time.sleep(ta - now)
+        s = gpu.upload_batch([g], [o])
+        gpu.iterate(it)
+        gpu.points(0)
+        lat.append((time.perf_counter() - t0 - ta) * 1e3)
+        total_updates += it * int(s[0])
+    plain_s = time.perf_counter() - t0
+    gpu.close()
+    log(f"served: plain FIFO makespan {plain_s:.2f} s")
+    # continuous-batching server
+    srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128)
+    clk0 = srv.clock_ms()
+    t0 = time.perf_counter()
+    pending, slat, i, done = {}, [], 0, 0
+    ticks = 0
+    while done < n_jobs:
+        now = time.perf_counter() - t0
+        while i < n_jobs and jobs[i][0] <= now:
+            ta, g, o, it = jobs[i]
+            pending[srv.submit(g, o, it)] = (ta, i)
+            i += 1
+        running, queued = srv.step()
+        ticks += 1
+        for tk in list(pending):
+            r = srv.poll(tk)
+            if r.state >= 2:
+                ta, k = pending.pop(tk)
+                slat.append((k, r.done_ms - (clk0 + ta * 1e3)))
+                srv.release(tk)
+                done += 1
+        if running == 0 and queued == 0 and i < n_jobs:
+            time.sleep(max(0.0, jobs[i][0] - (time.perf_counter() - t0)))
+    srv_s = time.perf_counter() - t0
+    srv.close()
+    log(f"served: server makespan {srv_s:.2f} s in {ticks} ticks")
+    small = [k for k, (_, g, _, _) in enumerate(jobs) if g.edge_count <= 20000]
+    return {"jobs": n_jobs, "rate_hz": rate_hz, "updates": total_updates,
+            "plain_fifo": {"updates_per_s": total_updates / plain_s, "makespan_s": plain_s,
+                           "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                           "p99_ms_le20k_edges": pct([lat[k] for k in small], 99)},
+            "server": {"updates_per_s": total_updates / srv_s, "makespan_s": srv_s,
+                       "p50_ms": pct([v for _, v in slat], 50),
+                       "p99_ms": pct([v for _, v in slat], 99),
+                       "p99_ms_le20k_edges": pct([v for k, v in slat if k in set(small)], 99),
+                       "ticks": ticks},
+            "speedup": plain_s / srv_s}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +210,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell-split", type=int, default=0)
+    ap.add_argument("--batched", type=int, default=256, help="graphs in the batched case (0: skip)")
+    ap.add_argument("--served", type=int, default=100, help="jobs in the served case (0: skip)")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="for rocprofv3 counter runs: a 256 MiB device copy (calibration), "
+                         "then upload + autotune + one eager 30-iteration schedule")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -98,17 +228,27 @@ def main():
     graph = workloads.plain_gpu_graph()
     opts = workloads.plain_gpu_options(ITERS, cell_split=args.cell_split)
     gpu = bt.Gpu(opts)
+    if args.profile_only:
+        ms = gpu.copy_probe(COPY_PROBE_BYTES)
+        log(f"copy probe: {COPY_PROBE_BYTES} bytes in {ms:.3f} ms")
     t = time.perf_counter()
     n = gpu.upload(graph)
     upload_ms = (time.perf_counter() - t) * 1e3
     log(f"upload: {n} samples in {upload_ms:.1f} ms")
+    if args.profile_only:
+        if not args.cell_split:
+            gpu.autotune(ITERS, TUNE_CANDIDATES)
+        gpu.iterate(ITERS)
+        log("profile-only schedule done")
+        gpu.close()
+        return
 
     # autotune the grid cell split per iteration (results are bit-identical
     # under any split; only speed changes)
     tune = None
     if not args.cell_split:
         t = time.perf_counter()
-        chosen, best = gpu.autotune(ITERS, (1, 2, 3, 4, 6, 8, 12))
+        chosen, best = gpu.autotune(ITERS, TUNE_CANDIDATES)
         tune = {"splits": chosen.tolist(), "seconds": time.perf_counter() - t}
         log(f"autotune: {chosen.tolist()} in {tune['seconds']:.2f} s")
 
@@ -135,12 +275,15 @@ def main():
     gpu.rewind()
     stage_ms = {k: 0.0 for k in bt.api.STAGES}
     flops = 0
+    per_iter = []
     for i in range(ITERS):
         st, _ = gpu.profile(1)
         for k, v in st.items():
             stage_ms[k] += v
         _, counts = gpu.grid()
         p = gpu.job_params(i)
+        per_iter.append({"radius": round(p.radius, 3), "cell": round(p.cell, 3),
+                         "mean_ms": round(st["neighbor_mean"], 3)})
         k = max(1, int(round(p.radius / p.cell)))
         gy, gx = counts.shape
         pad = np.zeros(((gy + k - 1) // k * k, (gx + k - 1) // k * k), dtype=np.int64)
@@ -163,6 +306,10 @@ def main():
     iter_bytes = 24 * n + 28 * P
     path = {"bytes_per_iteration": iter_bytes,
             "achieved_GBs": iter_bytes * ITERS / (total / args.steps) / 1e9}
+
+    gpu.close()
+    batched = batched_section(bt, workloads, args.batched) if args.batched else None
+    served = served_section(bt, workloads, args.served) if args.served else None
 
     cpu = None
     if not args.no_cpu:
@@ -191,17 +338,19 @@ def main():
                     "p99_ms": float(np.percentile(np.array(times) * 1e3, 99)),
                     "upload_ms": upload_ms},
         "autotune": tune,
+        "per_iteration": per_iter,
         "stage_ms_per_schedule": stage_ms,
         "dominant_stage": dom,
         "roofline": roof,
         "valu": valu,
         "path": path,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
     print(json.dumps(out))
-    gpu.close()
 
 
 if __name__ == "__main__":

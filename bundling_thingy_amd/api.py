@@ -94,6 +94,7 @@ SIGNATURES = {
     "bt_profile": (C.c_int, [_P, _U32, _PD, _U32, _PD]),
     "bt_autotune": (C.c_int, [_P, _U32, _PU32, _U32, _PU32, _PD]),
     "bt_job_iter_params": (C.c_int, [_P, _U32, _U32, C.POINTER(bt_iter_params)]),
+    "bt_copy_probe": (C.c_int, [_P, _U64, _PD]),
     "bt_upload_batch": (C.c_int, [_P, C.POINTER(bt_graph_desc), _U32, _PU32]),
     "bt_read_paths_job": (C.c_int, [_P, _U32, _PU32, _SZ]),
     "bt_read_points_job": (C.c_int, [_P, _U32, _P, _SZ]),
@@ -435,6 +436,12 @@ class Gpu:
         out = np.zeros(w * h, dtype=np.uint32 if kind == DENSITY_COUNTS else np.float32)
         self._check(lib().bt_read_density_job(self._h, job, kind, out.ctypes.data_as(_P), w * h))
         return out.reshape(h, w)
+
+    def copy_probe(self, nbytes):
+        """Device-to-device copy of nbytes (profiler calibration); returns ms."""
+        ms = C.c_double()
+        self._check(lib().bt_copy_probe(self._h, nbytes, C.byref(ms)))
+        return ms.value
 
     def job_params(self, iteration, job=0):
         """The exact parameters job `job` runs iteration `iteration` with."""

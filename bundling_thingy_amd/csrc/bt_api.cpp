@@ -208,6 +208,28 @@ int bt_autotune(bt_ctx* ctx, uint32_t n, const uint32_t* candidates, uint32_t n_
   });
 }
 
+int bt_copy_probe(bt_ctx* ctx, uint64_t bytes, double* ms) {
+  CTX_CHECK();
+  return guarded(&ctx->err, [&] {
+    bt::DevBuf a, b;
+    a.alloc(bytes);
+    b.alloc(bytes);
+    HIPCHK(hipMemsetAsync(a.p, 1, bytes, ctx->engine->stream()));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, ctx->engine->stream()));
+    HIPCHK(hipMemcpyAsync(b.p, a.p, bytes, hipMemcpyDeviceToDevice, ctx->engine->stream()));
+    HIPCHK(hipEventRecord(e1, ctx->engine->stream()));
+    ctx->engine->sync();
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t;
+  });
+}
+
 int bt_job_iter_params(bt_ctx* ctx, uint32_t job, uint32_t iteration, bt_iter_params* out) {
   CTX_CHECK();
   if (!out) return BT_E_INVALID;

@@ -29,26 +29,23 @@ def gmm_nodes(rng, n, k):
 
 
 def sample_pairs(rng, nodes, n_edges, scale=0.3):
+    """Distinct unordered node pairs, each drawn with probability proportional
+    to exp(-dist / scale) (rejection sampling), in draw order."""
     n = len(nodes)
-    seen = set()
-    pairs = []
-    while len(pairs) < n_edges:
-        m = 4 * (n_edges - len(pairs)) + 64
+    keys = np.empty(0, dtype=np.int64)
+    while True:
+        m = 4 * max(n_edges - len(keys), 16) + 64
         i = rng.integers(n, size=m)
         j = rng.integers(n, size=m)
         d = np.linalg.norm(nodes[i] - nodes[j], axis=1)
-        acc = rng.random(m) < np.exp(-d / scale)
-        for a, b in zip(i[acc], j[acc]):
-            if a == b:
-                continue
-            key = (min(a, b), max(a, b))
-            if key in seen:
-                continue
-            seen.add(key)
-            pairs.append(key)
-            if len(pairs) == n_edges:
-                break
-    return np.array(pairs, dtype=np.int64)
+        acc = (rng.random(m) < np.exp(-d / scale)) & (i != j)
+        a, b = np.minimum(i[acc], j[acc]), np.maximum(i[acc], j[acc])
+        keys = np.concatenate([keys, a * n + b])
+        _, first = np.unique(keys, return_index=True)
+        keys = keys[np.sort(first)]
+        if len(keys) >= n_edges:
+            keys = keys[:n_edges]
+            return np.stack([keys // n, keys % n], 1)
 
 
 def straight_graph(nodes, pairs):
@@ -95,26 +92,22 @@ def large_graph():
     cl = (centers[:, None, :] + rng.normal(size=(50, 100, 2)) * 0.01).reshape(-1, 2)
     nodes = np.clip(np.concatenate([uni, cl]), 0.0, 1.0)
     tree = cKDTree(nodes)
-    pairs = []
-    seen = set()
-    while len(pairs) < 100000:
+    n = len(nodes)
+    keys = np.empty(0, dtype=np.int64)
+    while len(keys) < 100000:
         m = 120000
-        a = rng.integers(len(nodes), size=m)
+        a = rng.integers(n, size=m)
         length = rng.lognormal(-2.0, 0.7, m)
         th = rng.uniform(0, 2 * np.pi, m)
         tgt = nodes[a] + length[:, None] * np.stack([np.cos(th), np.sin(th)], 1)
         _, b = tree.query(tgt)
-        for x, y in zip(a, b):
-            if x == y:
-                continue
-            k = (min(x, y), max(x, y))
-            if k in seen:
-                continue
-            seen.add(k)
-            pairs.append(k)
-            if len(pairs) == 100000:
-                break
-    return straight_graph(nodes, np.array(pairs, dtype=np.int64))
+        ok = a != b
+        k = np.minimum(a[ok], b[ok]) * n + np.maximum(a[ok], b[ok])
+        keys = np.concatenate([keys, k])
+        _, first = np.unique(keys, return_index=True)
+        keys = keys[np.sort(first)]
+    keys = keys[:100000]
+    return straight_graph(nodes, np.stack([keys // n, keys % n], 1))
 
 
 def batched_graphs(count=256, lo=50, hi=20000):
@@ -149,6 +142,38 @@ def plain_gpu_options(iterations=30, cell_split=0):
     iterations, 1024^2 density, sigma 8 -> 2."""
     return Options(spacing=0.002, radius=(0.06, 0.01), sigma=(8.0, 2.0), density=1024,
                    schedule_len=iterations, cell_split=cell_split)
+
+
+def batched_options(graph_index, iterations=30):
+    """Per-graph parameters of the batched case (seeded by the graph index):
+    spacing 0.01, radius 0.05 -> 0.02, sigma 4 -> 2, density 256^2 to 512^2."""
+    rng = np.random.default_rng(1000 + graph_index)
+    d = int(rng.choice([256, 384, 512]))
+    return Options(spacing=0.01, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=d,
+                   schedule_len=iterations)
+
+
+def served_jobs(count, rate_hz=200.0, lo=100, hi=50000, seed=5):
+    """Poisson arrival stream: (arrival seconds, graph, options, iterations).
+    Edge counts log-uniform in [lo, hi], 20-50 iterations, density 512^2 to
+    2048^2; spacing 0.01, radius 0.05 -> 0.02, sigma 4 -> 2."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate_hz)
+        e = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        n = max(8, e // 6)
+        k = int(rng.integers(4, 21))
+        nodes = gmm_nodes(rng, n, k)
+        e = min(e, n * (n - 1) // 2)
+        g = straight_graph(nodes, sample_pairs(rng, nodes, e))
+        iters = int(rng.integers(20, 51))
+        d = int(rng.choice([512, 1024, 2048]))
+        o = Options(spacing=0.01, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=d,
+                    schedule_len=iters)
+        out.append((t, g, o, iters))
+    return out
 
 
 def large_options(iterations=50):

@@ -147,6 +147,12 @@ int bt_profile(bt_ctx* ctx, uint32_t n, double* stage_ms, uint32_t n_stages, dou
 int bt_autotune(bt_ctx* ctx, uint32_t n, const uint32_t* candidates, uint32_t n_candidates,
                 uint32_t* chosen, double* best_ms);
 
+/* Measurement helper (no counterpart in the reference): copies `bytes` bytes
+ * device-to-device on the context's stream (HIP's copy kernel), so profiler
+ * byte counters can be scaled against a transfer of known size. *ms receives
+ * the copy time from HIP events. */
+int bt_copy_probe(bt_ctx* ctx, uint64_t bytes, double* ms);
+
 /* Parameters job `job` runs iteration `iteration` with (after autotuning). */
 int bt_job_iter_params(bt_ctx* ctx, uint32_t job, uint32_t iteration, bt_iter_params* out);
 
