@@ -1,21 +1,24 @@
-"""Per-kernel duration summary of a rocprofv3 kernel trace.
+"""Summaries of rocprofv3 output for the profiles/ notes.
 
-Usage: python -m bundling_thingy_amd.profstats <run_results.db | kernel_trace.csv>
-Reads the rocpd SQLite database (rocprofv3's default output) or a
-kernel_trace CSV and prints calls, average, total, min and max per kernel.
+  python -m bundling_thingy_amd.profstats stats <kernel_trace.csv | results.db>
+      per-kernel calls / average / total / min / max durations
+  python -m bundling_thingy_amd.profstats last <kernel_trace.csv> <kernel> <n>
+      average duration of the kernel's last n dispatches
+  python -m bundling_thingy_amd.profstats traffic <fetch.csv> <write.csv> <kernel> <n> <copy_bytes>
+      HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
+      each scaled by what the counter reads for HIP's device copy of
+      copy_bytes bytes (read once, written once) in the same run; prints JSON
 """
 import csv
+import json
 import sqlite3
 import sys
 
 
-def rows_from_db(path):
-    db = sqlite3.connect(path)
-    q = "select name, duration from kernels"
-    return [(n, float(d)) for n, d in db.execute(q)]
-
-
-def rows_from_csv(path):
+def _trace_rows(path):
+    if path.endswith(".db"):
+        db = sqlite3.connect(path)
+        return [(n, float(d)) for n, d in db.execute("select name, duration from kernels")]
     out = []
     with open(path) as f:
         for r in csv.DictReader(f):
@@ -23,9 +26,9 @@ def rows_from_csv(path):
     return out
 
 
-def summarize(rows):
+def stats(path):
     agg = {}
-    for name, d in rows:
+    for name, d in _trace_rows(path):
         a = agg.setdefault(name, [0, 0.0, float("inf"), 0.0])
         a[0] += 1
         a[1] += d
@@ -38,6 +41,41 @@ def summarize(rows):
     return "\n".join(lines)
 
 
+def last(path, kernel, n):
+    d = [t for name, t in _trace_rows(path) if name == kernel][-n:]
+    return sum(d) / len(d) / 1e6
+
+
+def _counter(path, name):
+    with open(path) as f:
+        return [r for r in csv.DictReader(f) if r["Counter_Name"] == name]
+
+
+def traffic(fetch_csv, write_csv, kernel, n, copy_bytes):
+    out = {"kernel": kernel, "dispatches": n, "copy_bytes": copy_bytes}
+    total = 0.0
+    for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
+        rows = _counter(path, counter)
+        copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        calib_kib = max(copies)  # the large calibration copy
+        factor = (copy_bytes / 1024.0) / calib_kib
+        vals = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"] == kernel][-n:]
+        kib = sum(vals) / len(vals)
+        out[key] = {"kib_per_dispatch": kib, "calibration_kib": calib_kib, "factor": factor,
+                    "bytes_per_dispatch": kib * 1024.0 * factor}
+        total += kib * 1024.0 * factor
+    out["bytes_per_dispatch"] = total
+    return out
+
+
 if __name__ == "__main__":
-    p = sys.argv[1]
-    print(summarize(rows_from_db(p) if p.endswith(".db") else rows_from_csv(p)))
+    cmd = sys.argv[1]
+    if cmd == "stats":
+        print(stats(sys.argv[2]))
+    elif cmd == "last":
+        print(f"{last(sys.argv[2], sys.argv[3], int(sys.argv[4])):.4f} ms")
+    elif cmd == "traffic":
+        print(json.dumps(traffic(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]),
+                                 int(sys.argv[6])), indent=1))
+    else:
+        raise SystemExit(__doc__)
