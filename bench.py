@@ -294,9 +294,15 @@ def main():
     mean_ms = stage_ms["neighbor_mean"] / ITERS
     P = 1024 * 1024
     alg_bytes = 24 * n + 4 * n  # read + write 12 B records, splat 4 B per sample
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "neighbor_mean_plain_gpu_case_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["bytes_per_dispatch"]
     roof = {"kernel": "bt_neighbor_mean", "bound": "hbm",
             "achieved": alg_bytes / (mean_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "traffic": None}
+            "unit": "GB/s", "traffic": traffic, "algorithmic_bytes": alg_bytes,
+            "avg_launch_ms": mean_ms}
     roof["frac"] = roof["achieved"] / roof["peak"]
     valu = {"kernel": "bt_neighbor_mean", "bound": "valu",
             "achieved": flops / (stage_ms["neighbor_mean"] * 1e-3) / 1e12,

@@ -86,10 +86,13 @@ enum BtKind {
   BT_K_COUNT
 };
 
+#define BT_XCDS 8  // MI355X: workgroups are dealt round-robin to 8 XCDs (one L2 each)
+
 struct BtControl {
   uint32_t task_total;    // tasks emitted this iteration
-  uint32_t queue_head;    // work-queue cursor of the neighbor-mean kernel
+  uint32_t queue_head;    // unused (kept for layout)
   uint32_t pad[2];
+  uint32_t xq[BT_XCDS];   // per-XCD work-queue cursors of the neighbor-mean kernel
 };
 
 struct __attribute__((aligned(16))) BtTask {

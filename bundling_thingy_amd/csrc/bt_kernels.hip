@@ -139,6 +139,7 @@ extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restri
     if (j == 0) {
       ctl->task_total = 0u;
       ctl->queue_head = 0u;
+      for (int x = 0; x < BT_XCDS; ++x) ctl->xq[x] = 0u;
     }
   }
 }
@@ -422,14 +423,34 @@ __device__ __forceinline__ int cell_of_fast(float t, float inv_cell, uint32_t g)
   return (int)v;
 }
 
+// Work distribution: the task list is cut into BT_XCDS contiguous parts (tasks
+// are emitted in roughly spatial order, so a part is a spatially coherent
+// region); workgroup b runs on XCD b % BT_XCDS and first drains that XCD's
+// part, keeping its candidate ranges in one L2, then takes tasks from the
+// other parts. Every cursor only grows, so every wave ends after at most one
+// failed claim per part.
+__device__ __forceinline__ uint32_t claim_task(BtControl* ctl, uint32_t total, uint32_t& part,
+                                               uint32_t& tried) {
+  while (tried < BT_XCDS) {
+    const uint32_t lo = (uint32_t)(((uint64_t)total * part) / BT_XCDS);
+    const uint32_t hi = (uint32_t)(((uint64_t)total * (part + 1u)) / BT_XCDS);
+    uint32_t k = 0;
+    if (LANE() == 0) k = atomicAdd(&ctl->xq[part], 1u);
+    k = __builtin_amdgcn_readfirstlane(k);
+    if (lo + k < hi) return lo + k;
+    part = (part + 1u) % BT_XCDS;
+    ++tried;
+  }
+  return 0xffffffffu;
+}
+
 extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl) {
   const uint32_t lane = LANE();
+  const uint32_t total = uni(__atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
+  uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
   for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(&ctl->queue_head, 1u);
-    t = uni(t);
-    const uint32_t total = uni(__atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
+    const uint32_t t = uni(claim_task(ctl, total, part, tried));
     if (t >= total) break;
     const uint32_t tj = uni(tasks[t].job);
     const uint32_t ts = uni(tasks[t].start);
