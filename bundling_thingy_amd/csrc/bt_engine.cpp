@@ -231,7 +231,9 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   btk::scatter(stream_, p.grid[BT_K_BIN], J, B(BT_K_BIN), (int)nj);
   btk::pre_scan(stream_, p.grid[BT_K_PTS], J, B(BT_K_PTS), (int)nj);
   mark(5);
-  btk::neighbor_mean(stream_, (uint32_t)cus_ * 8u, J, tasks_.as<BtTask>(), ctl);
+  // Persistent grid: 4 workgroups (16 waves) per CU, the kernel's occupancy
+  // at its register count; a multiple of the XCD count.
+  btk::neighbor_mean(stream_, (uint32_t)cus_ * 4u, J, tasks_.as<BtTask>(), ctl);
   mark(6);
   if (density && p.grid[BT_K_ROW]) {
     btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj);

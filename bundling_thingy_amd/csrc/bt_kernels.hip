@@ -366,53 +366,23 @@ extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_apply(
 
 // ---------------------------------------------------------- neighbor mean
 // neighbor_mean.wgsl:75-203. One wave per task (<= 64 query samples of one
-// cell). For every grid row the task's window touches, the wave (uniformly)
-// splits the row's cells into
+// cell). The grid rows of the task's window are set up in parallel, one row
+// per lane: each row's cells split (uniformly for the wave) into
 //   - cells fully inside the disk of every query sample (conservative, with
-//     an eps margin): their samples all pass the neighbor test, so their
+//     an eps margin): all their samples pass the neighbor test, so their
 //     fixed-point sums come in O(1) from the prefix sums of the sorted
 //     coordinates (the role of the reference's CellSummary, :56-59);
-//   - boundary cells, whose samples are streamed through scalar loads and
-//     tested by every lane against its own sample:
-//       t = (q - p) * texel_scale; accept when fmaf(tx, tx, ty*ty) <= r^2;
+//   - boundary cells, whose samples must be tested;
 //   - cells beyond every disk, skipped.
-// Within a row the cells are contiguous in sorted order, so each part is one
-// range. All sums are exact int64, so the result equals the all-pairs test.
+// Within a row the cells are contiguous in sorted order, so a row yields at
+// most two boundary ranges. The ranges of up to 64 rows form one candidate
+// stream, fetched 64 samples at a time with coalesced loads into a
+// double-buffered wave-private LDS tile (the next chunk is in flight while
+// the current one is tested) and broadcast to every lane, which tests its own
+// sample:  t = (q - p) * texel_scale; accept when fmaf(tx, tx, ty*ty) <= r^2.
+// All sums are exact int64, so the result equals the all-pairs test.
 // Output is written in sorted order (the next iteration's input order), and
 // every sample, fixed or not, is splatted into the density counts.
-__device__ __forceinline__ void scan_range(const float4* S, uint32_t b, uint32_t e, float px,
-                                           float py, float tsx, float tsy, float r2,
-                                           long long& ax, long long& ay, int& n) {
-  for (uint32_t c0 = b; c0 < e; c0 += 64u) {
-    const uint32_t c1 = min(c0 + 64u, e);
-    int sx = 0, sy = 0, sc = 0;
-    uint32_t k = c0;
-    for (; k + 4u <= c1; k += 4u) {
-#pragma unroll
-      for (uint32_t u = 0; u < 4u; ++u) {
-        const float4 s = ld_uniform(S, k + u);
-        const float dx = (s.x - px) * tsx;
-        const float dy = (s.y - py) * tsy;
-        const bool ok = fmaf(dx, dx, dy * dy) <= r2;
-        sx += ok ? __float_as_int(s.z) : 0;
-        sy += ok ? __float_as_int(s.w) : 0;
-        sc += ok ? 1 : 0;
-      }
-    }
-    for (; k < c1; ++k) {
-      const float4 s = ld_uniform(S, k);
-      const float dx = (s.x - px) * tsx;
-      const float dy = (s.y - py) * tsy;
-      const bool ok = fmaf(dx, dx, dy * dy) <= r2;
-      sx += ok ? __float_as_int(s.z) : 0;
-      sy += ok ? __float_as_int(s.w) : 0;
-      sc += ok ? 1 : 0;
-    }
-    ax += sx;
-    ay += sy;
-    n += sc;
-  }
-}
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -421,6 +391,12 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_re
 __device__ __forceinline__ int cell_of_fast(float t, float inv_cell, uint32_t g) {
   float v = fminf(fmaxf(t * inv_cell, 0.0f), (float)(g - 1u));
   return (int)v;
+}
+
+__device__ __forceinline__ long long wave_sum64(long long v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
 }
 
 // Work distribution: the task list is cut into BT_XCDS contiguous parts (tasks
@@ -444,9 +420,15 @@ __device__ __forceinline__ uint32_t claim_task(BtControl* ctl, uint32_t total, u
   return 0xffffffffu;
 }
 
+#define NM_WAVES 4          // waves per workgroup
+#define NM_RANGES 128       // ranges per row group (two per row)
+
 extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl) {
-  const uint32_t lane = LANE();
+  __shared__ float4 tile[NM_WAVES][2][64];
+  __shared__ uint32_t rng_start[NM_WAVES][NM_RANGES];
+  __shared__ uint32_t rng_pref[NM_WAVES][NM_RANGES + 1];
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6;
   const uint32_t total = uni(__atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
   uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
   for (;;) {
@@ -485,50 +467,125 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
       const uint32_t cy1 = uni((uint32_t)cell_of_fast(ty1 + rr, inv_c, gy));
       const float4* S = J.sorted;
       const uint32_t* cells = J.cells;
-      const uint64_t* PX = J.pre_x;
-      const uint64_t* PY = J.pre_y;
-      long long ax = 0, ay = 0;  // per-lane sums of scanned neighbors
+      long long ax = 0, ay = 0;  // per-lane sums of tested neighbors
       int n = 0;
-      unsigned long long ux = 0, uy = 0;  // wave-uniform sums of inside runs
-      uint32_t un = 0;
-      for (uint32_t cy = cy0; cy <= cy1; ++cy) {
-        const float Y0 = (float)cy * c, Y1 = Y0 + c;
-        const float dyn = fmaxf(fmaxf(Y0 - eps - ty1, ty0 - (Y1 + eps)), 0.0f);
-        if (dyn >= rr) continue;
-        const float wout = __builtin_sqrtf(rr * rr - dyn * dyn);
-        const uint32_t xlo = uni((uint32_t)cell_of_fast(tx0 - wout, inv_c, gx));
-        const uint32_t xhi = uni((uint32_t)cell_of_fast(tx1 + wout, inv_c, gx));
-        int xa = 1, xb = 0;
-        const float dyf = fmaxf(Y1 + eps - ty0, ty1 - (Y0 - eps));
-        if (cy > 0u && cy + 1u < gy && dyf < ri) {
-          const float win = __builtin_sqrtf(ri * ri - dyf * dyf);
-          const float fa = ceilf((tx1 - win + eps) * inv_c);
-          const float fb = floorf((tx0 + win - eps) * inv_c) - 1.0f;
-          xa = (int)fminf(fmaxf(fa, fmaxf((float)xlo, 1.0f)), (float)gx);
-          xb = (int)fmaxf(fminf(fb, fminf((float)xhi, (float)gx - 2.0f)), -1.0f);
+      long long ux = 0, uy = 0;  // wave-uniform sums of inside runs
+      long long un = 0;
+      for (uint32_t rb = cy0; rb <= cy1; rb += 64u) {
+        // ---- row setup, one row per lane
+        const uint32_t cy = rb + lane;
+        uint32_t r0b = 0, r0e = 0, r1b = 0, r1e = 0;
+        long long lx = 0, ly = 0, ln = 0;
+        if (cy <= cy1) {
+          const float Y0 = (float)cy * c, Y1 = Y0 + c;
+          const float dyn = fmaxf(fmaxf(Y0 - eps - ty1, ty0 - (Y1 + eps)), 0.0f);
+          if (dyn < rr) {
+            const float wout = __builtin_sqrtf(rr * rr - dyn * dyn);
+            const uint32_t xlo = (uint32_t)cell_of_fast(tx0 - wout, inv_c, gx);
+            const uint32_t xhi = (uint32_t)cell_of_fast(tx1 + wout, inv_c, gx);
+            int xa = 1, xb = 0;
+            const float dyf = fmaxf(Y1 + eps - ty0, ty1 - (Y0 - eps));
+            if (cy > 0u && cy + 1u < gy && dyf < ri) {
+              const float win = __builtin_sqrtf(ri * ri - dyf * dyf);
+              const float fa = ceilf((tx1 - win + eps) * inv_c);
+              const float fb = floorf((tx0 + win - eps) * inv_c) - 1.0f;
+              xa = (int)fminf(fmaxf(fa, fmaxf((float)xlo, 1.0f)), (float)gx);
+              xb = (int)fmaxf(fminf(fb, fminf((float)xhi, (float)gx - 2.0f)), -1.0f);
+            }
+            const uint32_t row = cy * gx;
+            const uint32_t s_lo = cells[row + xlo];
+            const uint32_t s_hi = cells[row + xhi + 1u];
+            if (xa <= xb) {
+              const uint32_t s_a = cells[row + (uint32_t)xa];
+              const uint32_t s_b = cells[row + (uint32_t)xb + 1u];
+              lx = (long long)(J.pre_x[s_b] - J.pre_x[s_a]);
+              ly = (long long)(J.pre_y[s_b] - J.pre_y[s_a]);
+              ln = (long long)(s_b - s_a);
+              r0b = s_lo;
+              r0e = s_a;
+              r1b = s_b;
+              r1e = s_hi;
+            } else {
+              r0b = s_lo;
+              r0e = s_hi;
+            }
+          }
         }
-        xa = (int)uni((uint32_t)xa);
-        xb = (int)uni((uint32_t)xb);
-        const uint32_t row = cy * gx;
-        const uint32_t s_lo = ld_uniform(cells, row + xlo);
-        const uint32_t s_hi = ld_uniform(cells, row + xhi + 1u);
-        if (xa <= xb) {
-          const uint32_t s_a = ld_uniform(cells, row + (uint32_t)xa);
-          const uint32_t s_b = ld_uniform(cells, row + (uint32_t)xb + 1u);
-          ux += ((const __attribute__((address_space(4))) unsigned long long*)(uintptr_t)PX)[s_b] -
-                ((const __attribute__((address_space(4))) unsigned long long*)(uintptr_t)PX)[s_a];
-          uy += ((const __attribute__((address_space(4))) unsigned long long*)(uintptr_t)PY)[s_b] -
-                ((const __attribute__((address_space(4))) unsigned long long*)(uintptr_t)PY)[s_a];
-          un += s_b - s_a;
-          scan_range(S, s_lo, s_a, px, py, tsx, tsy, r2, ax, ay, n);
-          scan_range(S, s_b, s_hi, px, py, tsx, tsy, r2, ax, ay, n);
-        } else {
-          scan_range(S, s_lo, s_hi, px, py, tsx, tsy, r2, ax, ay, n);
+        ux += wave_sum64(lx);
+        uy += wave_sum64(ly);
+        un += wave_sum64(ln);
+        // ---- compact the non-empty ranges of the group into LDS
+        const uint32_t l0 = r0e - r0b, l1 = r1e - r1b;
+        const uint32_t nr = (l0 ? 1u : 0u) + (l1 ? 1u : 0u);
+        const uint32_t len = l0 + l1;
+        const uint32_t rpos = wave_incl_scan(nr) - nr;
+        const uint32_t lpos = wave_incl_scan(len) - len;
+        const uint32_t nranges = uni(__shfl(rpos + nr, 63, 64));
+        const uint32_t m = uni(__shfl(lpos + len, 63, 64));
+        if (l0) {
+          rng_start[w][rpos] = r0b;
+          rng_pref[w][rpos] = lpos;
+        }
+        if (l1) {
+          rng_start[w][rpos + (l0 ? 1u : 0u)] = r1b;
+          rng_pref[w][rpos + (l0 ? 1u : 0u)] = lpos + l0;
+        }
+        if (lane == 0) rng_pref[w][nranges] = m;
+        __builtin_amdgcn_wave_barrier();
+        // ---- stream the candidates, 64 per chunk, double-buffered in LDS
+        const uint32_t nch = (m + 63u) >> 6;
+        auto fetch = [&](uint32_t ch) -> float4 {
+          const uint32_t v = (ch << 6) + lane;
+          float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (v < m) {
+            uint32_t lo = 0, hi = nranges - 1u;  // last range with pref <= v
+            while (lo < hi) {
+              const uint32_t mid = (lo + hi + 1u) >> 1;
+              if (rng_pref[w][mid] <= v) lo = mid; else hi = mid - 1u;
+            }
+            s = S[rng_start[w][lo] + (v - rng_pref[w][lo])];
+          }
+          return s;
+        };
+        float4 nxt = nch ? fetch(0) : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t ch = 0; ch < nch; ++ch) {
+          const uint32_t b = ch & 1u;
+          tile[w][b][lane] = nxt;
+          if (ch + 1u < nch) nxt = fetch(ch + 1u);
+          __builtin_amdgcn_wave_barrier();
+          const uint32_t cnt = min(64u, m - (ch << 6));
+          int sx = 0, sy = 0, sc = 0;
+          uint32_t k = 0;
+          for (; k + 4u <= cnt; k += 4u) {
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+              const float4 s = tile[w][b][k + u];
+              const float dx = (s.x - px) * tsx;
+              const float dy = (s.y - py) * tsy;
+              const bool ok = fmaf(dx, dx, dy * dy) <= r2;
+              sx += ok ? __float_as_int(s.z) : 0;
+              sy += ok ? __float_as_int(s.w) : 0;
+              sc += ok ? 1 : 0;
+            }
+          }
+          for (; k < cnt; ++k) {
+            const float4 s = tile[w][b][k];
+            const float dx = (s.x - px) * tsx;
+            const float dy = (s.y - py) * tsy;
+            const bool ok = fmaf(dx, dx, dy * dy) <= r2;
+            sx += ok ? __float_as_int(s.z) : 0;
+            sy += ok ? __float_as_int(s.w) : 0;
+            sc += ok ? 1 : 0;
+          }
+          ax += sx;
+          ay += sy;
+          n += sc;
+          __builtin_amdgcn_wave_barrier();
         }
       }
       if (movable) {
-        const long long tn = (long long)n + (long long)un;
-        const long long sx = ax + (long long)ux, sy = ay + (long long)uy;
+        const long long tn = (long long)n + un;
+        const long long sx = ax + ux, sy = ay + uy;
         const long long pfx = (long long)__float_as_int(q.z), pfy = (long long)__float_as_int(q.w);
         const long long dxs = sx - tn * pfx, dys = sy - tn * pfy;
         const double den = (double)tn * 16777216.0;
