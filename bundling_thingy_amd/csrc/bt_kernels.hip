@@ -598,7 +598,20 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
       out[3u * slot] = mx;
       out[3u * slot + 1u] = my;
       out[3u * slot + 2u] = __uint_as_float(id);
-      if (J.dens_cnt) atomicAdd(&J.dens_cnt[splat_index(mx, my, J.dw, J.dh)], 1u);
+    }
+    if (J.dens_cnt) {
+      // Samples of one task are neighbours, and bundled ones share pixels:
+      // one atomic per distinct pixel of the wave instead of one per sample
+      // (integer adds, so the counts are the same in any grouping).
+      const uint32_t pix = valid ? splat_index(mx, my, J.dw, J.dh) : 0u;
+      uint64_t act = ballot(valid);
+      while (act) {
+        const int leader = __builtin_ctzll(act);
+        const uint32_t key = __builtin_amdgcn_readlane(pix, leader);
+        const uint64_t same = ballot(valid && pix == key);
+        if (lane == (uint32_t)leader) atomicAdd(&J.dens_cnt[key], (uint32_t)__popcll(same));
+        act &= ~same;
+      }
     }
   }
 }
