@@ -93,6 +93,33 @@ struct BtControl {
   uint32_t queue_head;    // unused (kept for layout)
   uint32_t pad[2];
   uint32_t xq[BT_XCDS];   // per-XCD work-queue cursors of the neighbor-mean kernel
+  uint32_t heavy_count;   // heavy tasks recorded for splitting this iteration
+  uint32_t item_count;    // parts of heavy tasks recorded
+  uint32_t item_head;     // work-queue cursor of the heavy-part kernel
+  uint32_t pad2;
+};
+
+// Heavy tasks (candidate stream longer than BT_HEAVY_MIN samples) are split
+// into parts of about BT_PART_CANDS samples, run by many waves; their exact
+// partial sums meet in per-task accumulators, and the last part to finish
+// writes the means.
+#define BT_HEAVY_MIN 16384u
+#define BT_PART_CANDS 4096u
+#define BT_MAX_PARTS 64u
+#define BT_MAX_HEAVY 4096u
+#define BT_MAX_ITEMS 65536u
+
+struct BtHeavy {
+  uint32_t task;          // index into the task list
+  uint32_t parts;         // number of parts
+  uint32_t done;          // parts finished
+  uint32_t pad;
+};
+
+struct BtSplit {          // scratch of the heavy-task split (one per launch)
+  BtHeavy* heavy;         // [BT_MAX_HEAVY]
+  uint2* items;           // [BT_MAX_ITEMS]: (heavy index, part)
+  long long* acc;         // [BT_MAX_HEAVY][3][64]: per-lane x, y, count sums
 };
 
 struct __attribute__((aligned(16))) BtTask {

@@ -17,6 +17,11 @@ static inline size_t align_up(size_t a, size_t b) { return (a + b - 1) / b * b; 
 
 static const uint32_t kWt = BT_MAX_TAPS + 1;  // weights per iteration row
 
+// Layout of the heavy-task split scratch (one per engine).
+static const size_t kSplitItemsOff = (size_t)BT_MAX_HEAVY * sizeof(BtHeavy);
+static const size_t kSplitAccOff = kSplitItemsOff + (size_t)BT_MAX_ITEMS * sizeof(uint2);
+static const size_t kSplitBytes = kSplitAccOff + (size_t)BT_MAX_HEAVY * 192 * sizeof(long long);
+
 BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   BtJob d;
   std::memset(&d, 0, sizeof d);
@@ -109,6 +114,8 @@ Engine::Engine(const bt_options& o) : opts_(o) {
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   ctl_.alloc(sizeof(BtControl));
   HIPCHK(hipMemset(ctl_.p, 0, sizeof(BtControl)));
+  split_.alloc(kSplitBytes);
+  HIPCHK(hipMemset(split_.p, 0, kSplitBytes));
 }
 
 Engine::~Engine() {
@@ -233,7 +240,11 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   mark(5);
   // Persistent grid: 4 workgroups (16 waves) per CU, the kernel's occupancy
   // at its register count; a multiple of the XCD count.
-  btk::neighbor_mean(stream_, (uint32_t)cus_ * 4u, J, tasks_.as<BtTask>(), ctl);
+  BtSplit split;
+  split.heavy = split_.as<BtHeavy>();
+  split.items = reinterpret_cast<uint2*>(split_.as<char>() + kSplitItemsOff);
+  split.acc = reinterpret_cast<long long*>(split_.as<char>() + kSplitAccOff);
+  btk::neighbor_mean(stream_, (uint32_t)cus_ * 4u, J, tasks_.as<BtTask>(), ctl, split);
   mark(6);
   if (density && p.grid[BT_K_ROW]) {
     btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj);

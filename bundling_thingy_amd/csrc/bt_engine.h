@@ -100,7 +100,7 @@ class Engine {
   int cus_ = 256;
   hipStream_t stream_ = nullptr;
   std::vector<std::unique_ptr<Job>> jobs_;
-  DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_;
+  DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_;
   size_t slot_bytes_ = 0;
   uint32_t eager_slot_ = 0;
   uint64_t generation_ = 0;

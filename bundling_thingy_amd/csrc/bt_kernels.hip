@@ -140,6 +140,9 @@ extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restri
       ctl->task_total = 0u;
       ctl->queue_head = 0u;
       for (int x = 0; x < BT_XCDS; ++x) ctl->xq[x] = 0u;
+      ctl->heavy_count = 0u;
+      ctl->item_count = 0u;
+      ctl->item_head = 0u;
     }
   }
 }
@@ -423,195 +426,328 @@ __device__ __forceinline__ uint32_t claim_task(BtControl* ctl, uint32_t total, u
 #define NM_WAVES 4          // waves per workgroup
 #define NM_RANGES 128       // ranges per row group (two per row)
 
-extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
-    const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl) {
-  __shared__ float4 tile[NM_WAVES][2][64];
-  __shared__ uint32_t rng_start[NM_WAVES][NM_RANGES];
-  __shared__ uint32_t rng_pref[NM_WAVES][NM_RANGES + 1];
+struct NmLds {
+  float4 tile[NM_WAVES][2][64];
+  uint32_t start[NM_WAVES][NM_RANGES];
+  uint32_t pref[NM_WAVES][NM_RANGES + 1];
+};
+
+// Per-wave state of one task.
+struct NmTask {
+  const BtJob* J;
+  uint32_t slot, id;
+  bool valid, movable;
+  float px, py;
+  int pfx, pfy;
+  // window
+  float tx0, tx1, ty0, ty1;
+  uint32_t cy0, cy1;
+};
+
+__device__ __forceinline__ void nm_load(NmTask& T, const BtJob* jobs, const BtTask* tasks,
+                                        uint32_t t) {
+  const uint32_t lane = LANE();
+  const uint32_t tj = uni(tasks[t].job);
+  const uint32_t ts = uni(tasks[t].start);
+  const uint32_t te = uni(tasks[t].end);
+  T.J = &jobs[tj];
+  const BtJob& J = *T.J;
+  T.slot = ts + lane;
+  T.valid = T.slot < te;
+  float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+  T.id = 0;
+  if (T.valid) {
+    q = J.sorted[T.slot];
+    T.id = J.sid[T.slot];
+  }
+  T.movable = T.valid && !(T.id & BT_FIXED_BIT);
+  T.px = q.x;
+  T.py = q.y;
+  T.pfx = __float_as_int(q.z);
+  T.pfy = __float_as_int(q.w);
+  const bool mv = T.movable;
+  // query bbox in texels (the same texel() as the binning; monotone)
+  T.tx0 = __uint_as_float(uni(__float_as_uint(texel(wave_min(mv ? T.px : 3.0e38f), J.res_x))));
+  T.tx1 = __uint_as_float(uni(__float_as_uint(texel(wave_max(mv ? T.px : -3.0e38f), J.res_x))));
+  T.ty0 = __uint_as_float(uni(__float_as_uint(texel(wave_min(mv ? T.py : 3.0e38f), J.res_y))));
+  T.ty1 = __uint_as_float(uni(__float_as_uint(texel(wave_max(mv ? T.py : -3.0e38f), J.res_y))));
+  const float rr = J.radius + J.eps;
+  T.cy0 = uni((uint32_t)cell_of_fast(T.ty0 - rr, J.inv_cell, J.gy));
+  T.cy1 = uni((uint32_t)cell_of_fast(T.ty1 + rr, J.inv_cell, J.gy));
+}
+
+// Sets up rows [rb, rb + 64) of the window: the boundary ranges go to LDS,
+// the inside-run sums are added to (ux, uy, un). Returns the number of
+// candidates m and of ranges.
+__device__ __forceinline__ void nm_rows(const NmTask& T, uint32_t rb, NmLds& L, long long& ux,
+                                        long long& uy, long long& un, uint32_t& m,
+                                        uint32_t& nranges) {
   const uint32_t lane = LANE(), w = threadIdx.x >> 6;
+  const BtJob& J = *T.J;
+  const float c = J.cell, inv_c = J.inv_cell, eps = J.eps, r = J.radius;
+  const float rr = r + eps, ri = r - eps;
+  const uint32_t gx = J.gx, gy = J.gy;
+  const uint32_t* cells = J.cells;
+  const uint32_t cy = rb + lane;
+  uint32_t r0b = 0, r0e = 0, r1b = 0, r1e = 0;
+  long long lx = 0, ly = 0, ln = 0;
+  if (cy <= T.cy1) {
+    const float Y0 = (float)cy * c, Y1 = Y0 + c;
+    const float dyn = fmaxf(fmaxf(Y0 - eps - T.ty1, T.ty0 - (Y1 + eps)), 0.0f);
+    if (dyn < rr) {
+      const float wout = __builtin_sqrtf(rr * rr - dyn * dyn);
+      const uint32_t xlo = (uint32_t)cell_of_fast(T.tx0 - wout, inv_c, gx);
+      const uint32_t xhi = (uint32_t)cell_of_fast(T.tx1 + wout, inv_c, gx);
+      int xa = 1, xb = 0;
+      const float dyf = fmaxf(Y1 + eps - T.ty0, T.ty1 - (Y0 - eps));
+      if (cy > 0u && cy + 1u < gy && dyf < ri) {
+        const float win = __builtin_sqrtf(ri * ri - dyf * dyf);
+        const float fa = ceilf((T.tx1 - win + eps) * inv_c);
+        const float fb = floorf((T.tx0 + win - eps) * inv_c) - 1.0f;
+        xa = (int)fminf(fmaxf(fa, fmaxf((float)xlo, 1.0f)), (float)gx);
+        xb = (int)fmaxf(fminf(fb, fminf((float)xhi, (float)gx - 2.0f)), -1.0f);
+      }
+      const uint32_t row = cy * gx;
+      const uint32_t s_lo = cells[row + xlo];
+      const uint32_t s_hi = cells[row + xhi + 1u];
+      if (xa <= xb) {
+        const uint32_t s_a = cells[row + (uint32_t)xa];
+        const uint32_t s_b = cells[row + (uint32_t)xb + 1u];
+        lx = (long long)(J.pre_x[s_b] - J.pre_x[s_a]);
+        ly = (long long)(J.pre_y[s_b] - J.pre_y[s_a]);
+        ln = (long long)(s_b - s_a);
+        r0b = s_lo;
+        r0e = s_a;
+        r1b = s_b;
+        r1e = s_hi;
+      } else {
+        r0b = s_lo;
+        r0e = s_hi;
+      }
+    }
+  }
+  ux += wave_sum64(lx);
+  uy += wave_sum64(ly);
+  un += wave_sum64(ln);
+  const uint32_t l0 = r0e - r0b, l1 = r1e - r1b;
+  const uint32_t nr = (l0 ? 1u : 0u) + (l1 ? 1u : 0u);
+  const uint32_t len = l0 + l1;
+  const uint32_t rpos = wave_incl_scan(nr) - nr;
+  const uint32_t lpos = wave_incl_scan(len) - len;
+  nranges = uni(__shfl(rpos + nr, 63, 64));
+  m = uni(__shfl(lpos + len, 63, 64));
+  if (l0) {
+    L.start[w][rpos] = r0b;
+    L.pref[w][rpos] = lpos;
+  }
+  if (l1) {
+    L.start[w][rpos + (l0 ? 1u : 0u)] = r1b;
+    L.pref[w][rpos + (l0 ? 1u : 0u)] = lpos + l0;
+  }
+  if (lane == 0) L.pref[w][nranges] = m;
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Tests the candidate chunks first, first + step, ... of the stream set up in
+// LDS; adds the lane's accepted sums to (ax, ay, n).
+__device__ __forceinline__ void nm_stream(const NmTask& T, NmLds& L, uint32_t m, uint32_t nranges,
+                                          uint32_t first, uint32_t step, long long& ax,
+                                          long long& ay, long long& n) {
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6;
+  const BtJob& J = *T.J;
+  const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2, px = T.px, py = T.py;
+  const float4* S = J.sorted;
+  const uint32_t nch = (m + 63u) >> 6;
+  auto fetch = [&](uint32_t ch) -> float4 {
+    const uint32_t v = (ch << 6) + lane;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (v < m) {
+      uint32_t lo = 0, hi = nranges - 1u;  // last range with pref <= v
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (L.pref[w][mid] <= v) lo = mid; else hi = mid - 1u;
+      }
+      s = S[L.start[w][lo] + (v - L.pref[w][lo])];
+    }
+    return s;
+  };
+  float4 nxt = first < nch ? fetch(first) : make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t b = 0;
+  for (uint32_t ch = first; ch < nch; ch += step) {
+    L.tile[w][b][lane] = nxt;
+    if (ch + step < nch) nxt = fetch(ch + step);
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t cnt = min(64u, m - (ch << 6));
+    int sx = 0, sy = 0, sc = 0;
+    uint32_t k = 0;
+    for (; k + 4u <= cnt; k += 4u) {
+#pragma unroll
+      for (uint32_t u = 0; u < 4u; ++u) {
+        const float4 s = L.tile[w][b][k + u];
+        const float dx = (s.x - px) * tsx;
+        const float dy = (s.y - py) * tsy;
+        const bool ok = fmaf(dx, dx, dy * dy) <= r2;
+        sx += ok ? __float_as_int(s.z) : 0;
+        sy += ok ? __float_as_int(s.w) : 0;
+        sc += ok ? 1 : 0;
+      }
+    }
+    for (; k < cnt; ++k) {
+      const float4 s = L.tile[w][b][k];
+      const float dx = (s.x - px) * tsx;
+      const float dy = (s.y - py) * tsy;
+      const bool ok = fmaf(dx, dx, dy * dy) <= r2;
+      sx += ok ? __float_as_int(s.z) : 0;
+      sy += ok ? __float_as_int(s.w) : 0;
+      sc += ok ? 1 : 0;
+    }
+    ax += sx;
+    ay += sy;
+    n += sc;
+    b ^= 1u;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Mean of a movable sample from its exact sums, output record, density splat
+// (one atomic per distinct pixel of the wave).
+__device__ __forceinline__ void nm_finish(const NmTask& T, long long sx, long long sy,
+                                          long long tn) {
+  const uint32_t lane = LANE();
+  const BtJob& J = *T.J;
+  float mx = T.px, my = T.py;
+  if (T.movable) {
+    const long long dxs = sx - tn * (long long)T.pfx, dys = sy - tn * (long long)T.pfy;
+    const double den = (double)tn * 16777216.0;
+    mx = (float)((double)T.px + (double)dxs / den);
+    my = (float)((double)T.py + (double)dys / den);
+  }
+  if (T.valid) {
+    float* out = J.rec[J.parity ^ 1u];
+    out[3u * T.slot] = mx;
+    out[3u * T.slot + 1u] = my;
+    out[3u * T.slot + 2u] = __uint_as_float(T.id);
+  }
+  if (J.dens_cnt) {
+    const uint32_t pix = T.valid ? splat_index(mx, my, J.dw, J.dh) : 0u;
+    uint64_t act = ballot(T.valid);
+    while (act) {
+      const int leader = __builtin_ctzll(act);
+      const uint32_t key = __builtin_amdgcn_readlane(pix, leader);
+      const uint64_t same = ballot(T.valid && pix == key);
+      if (lane == (uint32_t)leader) atomicAdd(&J.dens_cnt[key], (uint32_t)__popcll(same));
+      act &= ~same;
+    }
+  }
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
+    const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
+    BtSplit split) {
+  __shared__ NmLds L;
+  const uint32_t lane = LANE();
   const uint32_t total = uni(__atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
   uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
   for (;;) {
     const uint32_t t = uni(claim_task(ctl, total, part, tried));
     if (t >= total) break;
-    const uint32_t tj = uni(tasks[t].job);
-    const uint32_t ts = uni(tasks[t].start);
-    const uint32_t te = uni(tasks[t].end);
-    const BtJob& J = jobs[tj];
-    const uint32_t slot = ts + lane;
-    const bool valid = slot < te;
-    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t id = 0;
-    if (valid) {
-      q = J.sorted[slot];
-      id = J.sid[slot];
-    }
-    const bool movable = valid && !(id & BT_FIXED_BIT);
-    const float px = q.x, py = q.y;
-    float mx = px, my = py;
-    if (ballot(movable)) {
-      const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2;
-      const float c = J.cell, inv_c = J.inv_cell, eps = J.eps, r = J.radius;
-      const uint32_t gx = J.gx, gy = J.gy;
-      // query bbox in texels (the same texel() as the binning; monotone)
-      const float tx0 = __uint_as_float(uni(__float_as_uint(
-          texel(wave_min(movable ? px : 3.0e38f), J.res_x))));
-      const float tx1 = __uint_as_float(uni(__float_as_uint(
-          texel(wave_max(movable ? px : -3.0e38f), J.res_x))));
-      const float ty0 = __uint_as_float(uni(__float_as_uint(
-          texel(wave_min(movable ? py : 3.0e38f), J.res_y))));
-      const float ty1 = __uint_as_float(uni(__float_as_uint(
-          texel(wave_max(movable ? py : -3.0e38f), J.res_y))));
-      const float rr = r + eps, ri = r - eps;
-      const uint32_t cy0 = uni((uint32_t)cell_of_fast(ty0 - rr, inv_c, gy));
-      const uint32_t cy1 = uni((uint32_t)cell_of_fast(ty1 + rr, inv_c, gy));
-      const float4* S = J.sorted;
-      const uint32_t* cells = J.cells;
-      long long ax = 0, ay = 0;  // per-lane sums of tested neighbors
-      int n = 0;
-      long long ux = 0, uy = 0;  // wave-uniform sums of inside runs
-      long long un = 0;
-      for (uint32_t rb = cy0; rb <= cy1; rb += 64u) {
-        // ---- row setup, one row per lane
-        const uint32_t cy = rb + lane;
-        uint32_t r0b = 0, r0e = 0, r1b = 0, r1e = 0;
-        long long lx = 0, ly = 0, ln = 0;
-        if (cy <= cy1) {
-          const float Y0 = (float)cy * c, Y1 = Y0 + c;
-          const float dyn = fmaxf(fmaxf(Y0 - eps - ty1, ty0 - (Y1 + eps)), 0.0f);
-          if (dyn < rr) {
-            const float wout = __builtin_sqrtf(rr * rr - dyn * dyn);
-            const uint32_t xlo = (uint32_t)cell_of_fast(tx0 - wout, inv_c, gx);
-            const uint32_t xhi = (uint32_t)cell_of_fast(tx1 + wout, inv_c, gx);
-            int xa = 1, xb = 0;
-            const float dyf = fmaxf(Y1 + eps - ty0, ty1 - (Y0 - eps));
-            if (cy > 0u && cy + 1u < gy && dyf < ri) {
-              const float win = __builtin_sqrtf(ri * ri - dyf * dyf);
-              const float fa = ceilf((tx1 - win + eps) * inv_c);
-              const float fb = floorf((tx0 + win - eps) * inv_c) - 1.0f;
-              xa = (int)fminf(fmaxf(fa, fmaxf((float)xlo, 1.0f)), (float)gx);
-              xb = (int)fmaxf(fminf(fb, fminf((float)xhi, (float)gx - 2.0f)), -1.0f);
-            }
-            const uint32_t row = cy * gx;
-            const uint32_t s_lo = cells[row + xlo];
-            const uint32_t s_hi = cells[row + xhi + 1u];
-            if (xa <= xb) {
-              const uint32_t s_a = cells[row + (uint32_t)xa];
-              const uint32_t s_b = cells[row + (uint32_t)xb + 1u];
-              lx = (long long)(J.pre_x[s_b] - J.pre_x[s_a]);
-              ly = (long long)(J.pre_y[s_b] - J.pre_y[s_a]);
-              ln = (long long)(s_b - s_a);
-              r0b = s_lo;
-              r0e = s_a;
-              r1b = s_b;
-              r1e = s_hi;
-            } else {
-              r0b = s_lo;
-              r0e = s_hi;
-            }
+    NmTask T;
+    nm_load(T, jobs, tasks, t);
+    long long ax = 0, ay = 0, n = 0, ux = 0, uy = 0, un = 0;
+    if (ballot(T.movable)) {
+      bool deferred = false;
+      for (uint32_t rb = T.cy0; rb <= T.cy1; rb += 64u) {
+        uint32_t m, nranges;
+        nm_rows(T, rb, L, ux, uy, un, m, nranges);
+        if (rb == T.cy0 && T.cy1 - T.cy0 < 64u && m > BT_HEAVY_MIN) {
+          // one row group with a long stream: record it for the split kernel
+          const uint32_t parts = min(BT_MAX_PARTS, (m + BT_PART_CANDS - 1u) / BT_PART_CANDS);
+          uint32_t h = 0, base = 0;
+          if (lane == 0) {
+            h = atomicAdd(&ctl->heavy_count, 1u);
+            base = h < BT_MAX_HEAVY ? atomicAdd(&ctl->item_count, parts) : BT_MAX_ITEMS;
           }
-        }
-        ux += wave_sum64(lx);
-        uy += wave_sum64(ly);
-        un += wave_sum64(ln);
-        // ---- compact the non-empty ranges of the group into LDS
-        const uint32_t l0 = r0e - r0b, l1 = r1e - r1b;
-        const uint32_t nr = (l0 ? 1u : 0u) + (l1 ? 1u : 0u);
-        const uint32_t len = l0 + l1;
-        const uint32_t rpos = wave_incl_scan(nr) - nr;
-        const uint32_t lpos = wave_incl_scan(len) - len;
-        const uint32_t nranges = uni(__shfl(rpos + nr, 63, 64));
-        const uint32_t m = uni(__shfl(lpos + len, 63, 64));
-        if (l0) {
-          rng_start[w][rpos] = r0b;
-          rng_pref[w][rpos] = lpos;
-        }
-        if (l1) {
-          rng_start[w][rpos + (l0 ? 1u : 0u)] = r1b;
-          rng_pref[w][rpos + (l0 ? 1u : 0u)] = lpos + l0;
-        }
-        if (lane == 0) rng_pref[w][nranges] = m;
-        __builtin_amdgcn_wave_barrier();
-        // ---- stream the candidates, 64 per chunk, double-buffered in LDS
-        const uint32_t nch = (m + 63u) >> 6;
-        auto fetch = [&](uint32_t ch) -> float4 {
-          const uint32_t v = (ch << 6) + lane;
-          float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (v < m) {
-            uint32_t lo = 0, hi = nranges - 1u;  // last range with pref <= v
-            while (lo < hi) {
-              const uint32_t mid = (lo + hi + 1u) >> 1;
-              if (rng_pref[w][mid] <= v) lo = mid; else hi = mid - 1u;
+          h = uni(h);
+          base = uni(base);
+          if (h < BT_MAX_HEAVY && base + parts <= BT_MAX_ITEMS) {
+            for (uint32_t p = lane; p < parts; p += 64u) split.items[base + p] = make_uint2(h, p);
+            long long* acc = split.acc + (size_t)h * 192u;
+            acc[lane] = 0;
+            acc[64u + lane] = 0;
+            acc[128u + lane] = 0;
+            if (lane == 0) {
+              BtHeavy hv;
+              hv.task = t;
+              hv.parts = parts;
+              hv.done = 0u;
+              hv.pad = 0u;
+              split.heavy[h] = hv;
             }
-            s = S[rng_start[w][lo] + (v - rng_pref[w][lo])];
+            deferred = true;
+            break;
           }
-          return s;
-        };
-        float4 nxt = nch ? fetch(0) : make_float4(0.f, 0.f, 0.f, 0.f);
-        for (uint32_t ch = 0; ch < nch; ++ch) {
-          const uint32_t b = ch & 1u;
-          tile[w][b][lane] = nxt;
-          if (ch + 1u < nch) nxt = fetch(ch + 1u);
-          __builtin_amdgcn_wave_barrier();
-          const uint32_t cnt = min(64u, m - (ch << 6));
-          int sx = 0, sy = 0, sc = 0;
-          uint32_t k = 0;
-          for (; k + 4u <= cnt; k += 4u) {
-#pragma unroll
-            for (uint32_t u = 0; u < 4u; ++u) {
-              const float4 s = tile[w][b][k + u];
-              const float dx = (s.x - px) * tsx;
-              const float dy = (s.y - py) * tsy;
-              const bool ok = fmaf(dx, dx, dy * dy) <= r2;
-              sx += ok ? __float_as_int(s.z) : 0;
-              sy += ok ? __float_as_int(s.w) : 0;
-              sc += ok ? 1 : 0;
-            }
-          }
-          for (; k < cnt; ++k) {
-            const float4 s = tile[w][b][k];
-            const float dx = (s.x - px) * tsx;
-            const float dy = (s.y - py) * tsy;
-            const bool ok = fmaf(dx, dx, dy * dy) <= r2;
-            sx += ok ? __float_as_int(s.z) : 0;
-            sy += ok ? __float_as_int(s.w) : 0;
-            sc += ok ? 1 : 0;
-          }
-          ax += sx;
-          ay += sy;
-          n += sc;
-          __builtin_amdgcn_wave_barrier();
+          // No room: run it here; neutralize the part slots reserved in range.
+          for (uint32_t p = lane; p < parts; p += 64u)
+            if (base + p < BT_MAX_ITEMS) split.items[base + p] = make_uint2(0xffffffffu, 0u);
         }
+        nm_stream(T, L, m, nranges, 0u, 1u, ax, ay, n);
       }
-      if (movable) {
-        const long long tn = (long long)n + un;
-        const long long sx = ax + ux, sy = ay + uy;
-        const long long pfx = (long long)__float_as_int(q.z), pfy = (long long)__float_as_int(q.w);
-        const long long dxs = sx - tn * pfx, dys = sy - tn * pfy;
-        const double den = (double)tn * 16777216.0;
-        mx = (float)((double)px + (double)dxs / den);
-        my = (float)((double)py + (double)dys / den);
-      }
+      if (deferred) continue;
     }
-    if (valid) {
-      float* out = J.rec[J.parity ^ 1u];
-      out[3u * slot] = mx;
-      out[3u * slot + 1u] = my;
-      out[3u * slot + 2u] = __uint_as_float(id);
+    nm_finish(T, ax + ux, ay + uy, n + un);
+  }
+}
+
+// Runs the parts of heavy tasks: part p tests chunks p, p + P, p + 2P, ... of
+// the task's candidate stream (recomputed exactly as the main kernel set it
+// up) and adds its exact per-lane sums to the task's accumulators (part 0
+// also the inside-run sums). The part that finishes last (a counter; no wave
+// waits) reads the accumulators and writes the means.
+extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
+    const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
+    BtSplit split) {
+  __shared__ NmLds L;
+  const uint32_t lane = LANE();
+  const uint32_t items = min(uni(__atomic_load_n(&ctl->item_count, __ATOMIC_RELAXED)),
+                             BT_MAX_ITEMS);
+  for (;;) {
+    uint32_t i = 0;
+    if (lane == 0) i = atomicAdd(&ctl->item_head, 1u);
+    i = uni(i);
+    if (i >= items) break;
+    const uint2 it = split.items[i];
+    const uint32_t h = uni(it.x), p = uni(it.y);
+    if (h >= BT_MAX_HEAVY) continue;  // a neutralized slot
+    const uint32_t t = uni(split.heavy[h].task);
+    const uint32_t parts = uni(split.heavy[h].parts);
+    NmTask T;
+    nm_load(T, jobs, tasks, t);
+    long long ax = 0, ay = 0, n = 0, ux = 0, uy = 0, un = 0;
+    uint32_t m, nranges;
+    nm_rows(T, T.cy0, L, ux, uy, un, m, nranges);
+    nm_stream(T, L, m, nranges, p, parts, ax, ay, n);
+    if (p == 0u) {
+      ax += ux;
+      ay += uy;
+      n += un;
     }
-    if (J.dens_cnt) {
-      // Samples of one task are neighbours, and bundled ones share pixels:
-      // one atomic per distinct pixel of the wave instead of one per sample
-      // (integer adds, so the counts are the same in any grouping).
-      const uint32_t pix = valid ? splat_index(mx, my, J.dw, J.dh) : 0u;
-      uint64_t act = ballot(valid);
-      while (act) {
-        const int leader = __builtin_ctzll(act);
-        const uint32_t key = __builtin_amdgcn_readlane(pix, leader);
-        const uint64_t same = ballot(valid && pix == key);
-        if (lane == (uint32_t)leader) atomicAdd(&J.dens_cnt[key], (uint32_t)__popcll(same));
-        act &= ~same;
-      }
+    long long* acc = split.acc + (size_t)h * 192u;
+    atomicAdd((unsigned long long*)&acc[lane], (unsigned long long)ax);
+    atomicAdd((unsigned long long*)&acc[64u + lane], (unsigned long long)ay);
+    atomicAdd((unsigned long long*)&acc[128u + lane], (unsigned long long)n);
+    __threadfence();
+    uint32_t prev = 0;
+    if (lane == 0) prev = atomicAdd(&split.heavy[h].done, 1u);
+    prev = uni(prev);
+    if (prev + 1u == parts) {
+      __threadfence();
+      const long long sx = (long long)__hip_atomic_load((unsigned long long*)&acc[lane],
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const long long sy = (long long)__hip_atomic_load((unsigned long long*)&acc[64u + lane],
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const long long sn = (long long)__hip_atomic_load((unsigned long long*)&acc[128u + lane],
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      nm_finish(T, sx, sy, sn);
     }
   }
 }
@@ -987,8 +1123,9 @@ void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* ba
   hipLaunchKernelGGL(bt_scatter, L256(grid), jobs, base, nj);
 }
 void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
-                   BtControl* ctl) {
-  hipLaunchKernelGGL(bt_neighbor_mean, L256(grid), jobs, tasks_in, ctl);
+                   BtControl* ctl, BtSplit split) {
+  hipLaunchKernelGGL(bt_neighbor_mean, L256(grid), jobs, tasks_in, ctl, split);
+  hipLaunchKernelGGL(bt_neighbor_mean_heavy, L256(grid), jobs, tasks_in, ctl, split);
 }
 void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_blur_rows, L256(grid), jobs, base, nj);

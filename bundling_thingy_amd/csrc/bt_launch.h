@@ -14,7 +14,7 @@ void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base
 void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void pre_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
-                   BtControl* ctl);
+                   BtControl* ctl, BtSplit split);
 void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
