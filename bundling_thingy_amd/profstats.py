@@ -2,12 +2,15 @@
 
   python -m bundling_thingy_amd.profstats stats <kernel_trace.csv | results.db>
       per-kernel calls / average / total / min / max durations
-  python -m bundling_thingy_amd.profstats last <kernel_trace.csv> <kernel> <n>
-      average duration of the kernel's last n dispatches
-  python -m bundling_thingy_amd.profstats traffic <fetch.csv> <write.csv> <kernel> <n> <copy_bytes>
+  python -m bundling_thingy_amd.profstats last <kernel_trace.csv> <kernels> <n>
+      average duration of the last n dispatches; <kernels> is one name or a
+      comma-separated list whose averages are added (one stage made of
+      several launches, each dispatched once per stage)
+  python -m bundling_thingy_amd.profstats traffic <fetch.csv> <write.csv> <kernels> <n> <copy_bytes>
       HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
       each scaled by what the counter reads for HIP's device copy of
-      copy_bytes bytes (read once, written once) in the same run; prints JSON
+      copy_bytes bytes (read once, written once) in the same run; a kernel
+      list is summed as in 'last'; prints JSON
 """
 import csv
 import json
@@ -41,9 +44,13 @@ def stats(path):
     return "\n".join(lines)
 
 
-def last(path, kernel, n):
-    d = [t for name, t in _trace_rows(path) if name == kernel][-n:]
-    return sum(d) / len(d) / 1e6
+def last(path, kernels, n):
+    rows = _trace_rows(path)
+    total = 0.0
+    for k in kernels.split(","):
+        d = [t for name, t in rows if name == k][-n:]
+        total += sum(d) / len(d) / 1e6
+    return total
 
 
 def _counter(path, name):
@@ -59,8 +66,10 @@ def traffic(fetch_csv, write_csv, kernel, n, copy_bytes):
         copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
         calib_kib = max(copies)  # the large calibration copy
         factor = (copy_bytes / 1024.0) / calib_kib
-        vals = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"] == kernel][-n:]
-        kib = sum(vals) / len(vals)
+        kib = 0.0
+        for k in kernel.split(","):
+            vals = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"] == k][-n:]
+            kib += sum(vals) / len(vals)
         out[key] = {"kib_per_dispatch": kib, "calibration_kib": calib_kib, "factor": factor,
                     "bytes_per_dispatch": kib * 1024.0 * factor}
         total += kib * 1024.0 * factor
