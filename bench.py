@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 ITERS = 30
-TUNE_CANDIDATES = (1, 2, 3, 4, 6, 8, 12, 16)
+TUNE_CANDIDATES = (1, 2, 3, 4, 6, 8, 12, 16, 20, 24)
 COPY_PROBE_BYTES = 256 << 20
 
 
@@ -293,11 +293,12 @@ def main():
         p = gpu.job_params(i)
         per_iter.append({"radius": round(p.radius, 3), "cell": round(p.cell, 3),
                          "mean_ms": round(st["neighbor_mean"], 3)})
-        k = max(1, int(round(p.radius / p.cell)))
+        ky = max(1, int(round(p.radius / p.cell)))      # rows per radius
+        kx = max(1, int(round(p.radius / p.cell_w)))    # bins per radius
         gy, gx = counts.shape
-        pad = np.zeros(((gy + k - 1) // k * k, (gx + k - 1) // k * k), dtype=np.int64)
+        pad = np.zeros(((gy + ky - 1) // ky * ky, (gx + kx - 1) // kx * kx), dtype=np.int64)
         pad[:gy, :gx] = counts
-        coarse = pad.reshape(pad.shape[0] // k, k, pad.shape[1] // k, k).sum(axis=(1, 3))
+        coarse = pad.reshape(pad.shape[0] // ky, ky, pad.shape[1] // kx, kx).sum(axis=(1, 3))
         flops += candidate_flops(coarse)
     dom = max(stage_ms, key=stage_ms.get)
     mean_ms = stage_ms["neighbor_mean"] / ITERS

@@ -40,20 +40,21 @@ class bt_options(C.Structure):
                 ("radius1", C.c_float), ("sigma0", C.c_float), ("sigma1", C.c_float),
                 ("schedule_len", C.c_uint32), ("density_w", C.c_uint32),
                 ("density_h", C.c_uint32), ("cell_split", C.c_uint32),
-                ("reserved", C.c_uint32 * 3)]
+                ("x_split", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class bt_iter_params(C.Structure):
     _fields_ = [("res_x", C.c_float), ("res_y", C.c_float), ("radius", C.c_float),
                 ("cell", C.c_float), ("gx", C.c_uint32), ("gy", C.c_uint32),
                 ("sigma", C.c_float), ("taps", C.c_uint32), ("density_w", C.c_uint32),
-                ("density_h", C.c_uint32)]
+                ("density_h", C.c_uint32), ("cell_w", C.c_float), ("task_bins", C.c_uint32)]
 
 
 class bt_info(C.Structure):
     _fields_ = [("jobs", C.c_uint32), ("samples", C.c_uint32), ("paths", C.c_uint32),
                 ("iteration", C.c_uint32), ("capacity", C.c_uint64),
-                ("reserved_bytes", C.c_uint64), ("device_bytes", C.c_uint64)]
+                ("reserved_bytes", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("split_tasks", C.c_uint32), ("split_parts", C.c_uint32)]
 
 
 class bt_graph_desc(C.Structure):
@@ -161,7 +162,7 @@ class Options:
     def __init__(self, map_size=1600, neighbor_radius=None, max_graph_memory_mib=1024,
                  color_by_density=True, evaluate_distortion=False, device=0, spacing=0.0,
                  radius=None, sigma=None, schedule_len=1, density=None, cell_split=0,
-                 map_h=None):
+                 map_h=None, x_split=0):
         o = bt_options()
         lib().bt_options_default(C.byref(o))
         o.map_w = map_size
@@ -183,6 +184,7 @@ class Options:
             dw, dh = density if isinstance(density, (tuple, list)) else (density, density)
             o.density_w, o.density_h = dw, dh
         o.cell_split = cell_split
+        o.x_split = x_split
         self.c = o
 
     def iter_params(self, iteration):

@@ -184,7 +184,8 @@ int bt_get_info(bt_ctx* ctx, bt_info* info) {
     info->capacity = j.capacity;
     info->reserved_bytes = j.reserved;
   }
-  return BT_OK;
+  return guarded(&ctx->err,
+                 [&] { ctx->engine->split_counts(&info->split_tasks, &info->split_parts); });
 }
 
 int bt_iter_params_for(const bt_options* opts, uint32_t iteration, bt_iter_params* out) {

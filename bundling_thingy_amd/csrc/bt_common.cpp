@@ -111,11 +111,22 @@ void iter_params(const bt_options& o, uint32_t it, uint32_t split, bt_iter_param
   if (p->radius > 0.0f) {
     if (split == 0) split = default_split(p->radius);
     p->cell = split == 1 ? p->radius : p->radius / (float)split;
-    p->gx = grid_dim(p->res_x, p->cell);
     p->gy = grid_dim(p->res_y, p->cell);
+    // Rows keep the cell height; bins along x are x_split times narrower, so
+    // a row's candidate ranges end within a fraction of a cell of the disk.
+    // Capped so a job's grid stays within kMaxGridCells bins.
+    uint32_t a = std::min(o.x_split ? o.x_split : kDefaultXSplit, 64u);
+    const uint64_t g1 = (uint64_t)grid_dim(p->res_x, p->cell) * p->gy;
+    while (a > 1 && g1 * a > kMaxGridCells) a >>= 1;
+    p->cell_w = a == 1 ? p->cell : p->cell / (float)a;
+    p->gx = grid_dim(p->res_x, p->cell_w);
+    // Task blocks: runs of bins about a quarter radius wide (at least one
+    // cell), cut into chunks of up to 64 samples.
+    p->task_bins = a * std::max(1u, split / 4u);
   } else {
-    p->cell = 0.0f;
+    p->cell = p->cell_w = 0.0f;
     p->gx = p->gy = 1;
+    p->task_bins = 1;
   }
 }
 

@@ -41,11 +41,17 @@ bt_options resolve_options(const bt_options* in);
 
 bool has_radius(const bt_options& o);
 
-// Grid cells per radius when none is given: cells about 4 texels wide, at
+// Grid rows per radius when none is given: rows about 4 texels high, at
 // most 8 per radius (the autotuner refines it per iteration).
 uint32_t default_split(float radius);
 
-// Parameters of iteration `it`; split = grid cells per radius (0 = default).
+// Bins per row height along x when bt_options::x_split is 0.
+constexpr uint32_t kDefaultXSplit = 8;
+// Upper bound on one job's grid bins (4 bytes each); x_split is halved until
+// the grid fits.
+constexpr uint64_t kMaxGridCells = 1ull << 22;
+
+// Parameters of iteration `it`; split = grid rows per radius (0 = default).
 void iter_params(const bt_options& o, uint32_t it, uint32_t split, bt_iter_params* p);
 
 // Number of distinct parameter sets of the schedule (reference mode: 1).

@@ -16,7 +16,7 @@
 // Block granularities (work items per workgroup) of the batched kernels.
 #define BT_BIN_BLOCK 256           // points per workgroup in bin/scatter
 #define BT_SCAN_ITEMS 4096         // cells per workgroup in the cell scan (256 x 16)
-#define BT_TASK_CELLS 1024         // cells per workgroup in task emission (256 x 4)
+#define BT_TASK_BLOCKS 1024        // task blocks per workgroup in task emission (256 x 4)
 #define BT_CLEAR_ITEMS 4096        // words per workgroup in the clear kernel
 #define BT_ROW_TILE_W 64           // row blur tile: 64 columns x 4 rows
 #define BT_ROW_TILE_H 4
@@ -52,13 +52,17 @@ struct __attribute__((aligned(16))) BtJob {
   uint32_t n_paths;
   uint32_t dw, dh;        // density size
   // ---- per-iteration parameters
-  uint32_t gx, gy;        // grid dims (cells)
+  uint32_t gx, gy;        // grid dims: bins per row, rows
   uint32_t n_cells;       // gx * gy
   uint32_t taps;          // blur half-width
   float res_x, res_y;     // texel resolution per axis (reference: map size - 1)
   float ts_x, ts_y;       // texel scale = res * 0.5
-  float cell;             // cell side in texels
+  float cell;             // row height in texels
   float inv_cell;         // 1 / cell (window bounds only; binning divides)
+  float cell_w;           // bin width along x in texels
+  float inv_cell_w;       // 1 / cell_w
+  uint32_t task_bins;     // bins per task block along x
+  uint32_t task_bx;       // task blocks per row: ceil(gx / task_bins)
   float radius;           // texels
   float eps;              // geometric safety margin of the run classification, texels
   float r2;               // radius^2 (texels^2)
@@ -88,26 +92,34 @@ enum BtKind {
 
 #define BT_XCDS 8  // MI355X: workgroups are dealt round-robin to 8 XCDs (one L2 each)
 
+#define BT_CTR_STRIDE 32  // words between work-queue cursors: one 128-byte line each
+
 struct BtControl {
   uint32_t task_total;    // tasks emitted this iteration
-  uint32_t queue_head;    // unused (kept for layout)
-  uint32_t pad[2];
-  uint32_t xq[BT_XCDS];   // per-XCD work-queue cursors of the neighbor-mean kernel
   uint32_t heavy_count;   // heavy tasks recorded for splitting this iteration
   uint32_t item_count;    // parts of heavy tasks recorded
-  uint32_t item_head;     // work-queue cursor of the heavy-part kernel
-  uint32_t pad2;
+  uint32_t pad0;
+  unsigned long long cand_prev;  // candidates tested by the previous neighbor-mean
+                                 // launch (0: none since upload or rewind)
+  uint32_t pad[BT_CTR_STRIDE - 6];
+  // per-XCD work-queue cursors (part x at [x * BT_CTR_STRIDE]) of the
+  // neighbor-mean kernel (xq) and of the heavy-part kernel (iq)
+  uint32_t xq[BT_XCDS * BT_CTR_STRIDE];
+  uint32_t iq[BT_XCDS * BT_CTR_STRIDE];
 };
 
-// Heavy tasks (candidate stream longer than BT_HEAVY_MIN samples) are split
-// into parts of about BT_PART_CANDS samples, run by many waves; their exact
-// partial sums meet in per-task accumulators, and the last part to finish
-// writes the means.
-#define BT_HEAVY_MIN 16384u
-#define BT_PART_CANDS 4096u
+// Heavy tasks (candidate stream longer than a threshold H) are split into
+// parts of about H / 4 samples, run by many waves; their exact partial sums
+// meet in per-task accumulators, and the last part to finish writes the
+// means. H is twice the previous launch's candidates per wave, clamped to
+// [BT_HEAVY_LO, BT_HEAVY_HI] (BT_HEAVY_HI when there is no previous launch):
+// a task much longer than the average would otherwise run alone at the end.
+#define BT_HEAVY_LO 4096u
+#define BT_HEAVY_HI 16384u
 #define BT_MAX_PARTS 64u
 #define BT_MAX_HEAVY 4096u
 #define BT_MAX_ITEMS 65536u
+#define BT_MAX_WG 8192u       // workgroups of a neighbor-mean launch (per-WG candidate sums)
 
 struct BtHeavy {
   uint32_t task;          // index into the task list
@@ -120,6 +132,7 @@ struct BtSplit {          // scratch of the heavy-task split (one per launch)
   BtHeavy* heavy;         // [BT_MAX_HEAVY]
   uint2* items;           // [BT_MAX_ITEMS]: (heavy index, part)
   long long* acc;         // [BT_MAX_HEAVY][3][64]: per-lane x, y, count sums
+  unsigned long long* wg_cand;  // [BT_MAX_WG]: candidates tested per workgroup
 };
 
 struct __attribute__((aligned(16))) BtTask {

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 
 #include "bt_launch.h"
@@ -20,7 +21,8 @@ static const uint32_t kWt = BT_MAX_TAPS + 1;  // weights per iteration row
 // Layout of the heavy-task split scratch (one per engine).
 static const size_t kSplitItemsOff = (size_t)BT_MAX_HEAVY * sizeof(BtHeavy);
 static const size_t kSplitAccOff = kSplitItemsOff + (size_t)BT_MAX_ITEMS * sizeof(uint2);
-static const size_t kSplitBytes = kSplitAccOff + (size_t)BT_MAX_HEAVY * 192 * sizeof(long long);
+static const size_t kSplitCandOff = kSplitAccOff + (size_t)BT_MAX_HEAVY * 192 * sizeof(long long);
+static const size_t kSplitBytes = kSplitCandOff + (size_t)BT_MAX_WG * sizeof(unsigned long long);
 
 BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   BtJob d;
@@ -61,6 +63,10 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   d.ts_y = p.res_y * 0.5f;
   d.cell = p.cell;
   d.inv_cell = p.cell > 0.0f ? 1.0f / p.cell : 0.0f;
+  d.cell_w = p.cell_w;
+  d.inv_cell_w = p.cell_w > 0.0f ? 1.0f / p.cell_w : 0.0f;
+  d.task_bins = std::max(p.task_bins, 1u);
+  d.task_bx = (p.gx + d.task_bins - 1u) / d.task_bins;
   d.radius = p.radius;
   d.eps = std::max(1e-3f * p.cell, (std::max(p.res_x, p.res_y) + 1.0f) * 4e-6f);
   d.r2 = p.radius * p.radius;
@@ -128,6 +134,14 @@ Engine::~Engine() {
 
 void Engine::sync() { HIPCHK(hipStreamSynchronize(stream_)); }
 
+void Engine::split_counts(uint32_t* tasks, uint32_t* parts) {
+  uint32_t w[3] = {0, 0, 0};
+  HIPCHK(hipMemcpyAsync(w, ctl_.p, sizeof w, hipMemcpyDeviceToHost, stream_));
+  sync();
+  *tasks = std::min(w[1], (uint32_t)BT_MAX_HEAVY);
+  *parts = std::min(w[2], (uint32_t)BT_MAX_ITEMS);
+}
+
 void Engine::drop_graphs() {
   if (!graphs_.empty() && stream_) (void)hipStreamSynchronize(stream_);
   for (auto& kv : graphs_) {
@@ -164,7 +178,7 @@ static uint32_t kind_blocks(int k, const Job& j, const BtJob& d) {
     case BT_K_CLEAR: return cdiv(std::max<uint64_t>(d.n_cells + 1ull, npix), BT_CLEAR_ITEMS);
     case BT_K_BIN: return cdiv(j.n, BT_BIN_BLOCK);
     case BT_K_SCAN: return cdiv(d.n_cells + 1ull, BT_SCAN_ITEMS);
-    case BT_K_TASK: return cdiv(d.n_cells, BT_TASK_CELLS);
+    case BT_K_TASK: return cdiv((uint64_t)d.task_bx * d.gy, BT_TASK_BLOCKS);
     case BT_K_ROW: return cdiv(d.dw, BT_ROW_TILE_W) * cdiv(d.dh, BT_ROW_TILE_H);
     case BT_K_COL: return cdiv(d.dw, BT_COL_TILE_W) * cdiv(d.dh, BT_COL_TILE_H);
     case BT_K_NORM: return cdiv(npix, BT_NORM_ITEMS);
@@ -244,6 +258,7 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   split.heavy = split_.as<BtHeavy>();
   split.items = reinterpret_cast<uint2*>(split_.as<char>() + kSplitItemsOff);
   split.acc = reinterpret_cast<long long*>(split_.as<char>() + kSplitAccOff);
+  split.wg_cand = reinterpret_cast<unsigned long long*>(split_.as<char>() + kSplitCandOff);
   btk::neighbor_mean(stream_, (uint32_t)cus_ * 4u, J, tasks_.as<BtTask>(), ctl, split);
   mark(6);
   if (density && p.grid[BT_K_ROW]) {
@@ -269,6 +284,7 @@ void Engine::upload(const std::vector<GraphIn>& gs, std::vector<uint32_t>* sampl
   std::vector<std::unique_ptr<Job>> out;
   upload_jobs(gs, &out);
   jobs_ = std::move(out);
+  reset_split_estimate();
   if (samples) {
     samples->clear();
     for (auto& j : jobs_) samples->push_back(j->n);
@@ -455,7 +471,13 @@ void Engine::rewind() {
     j->iteration = 0;
     j->parity = 0;
   }
+  reset_split_estimate();
   sync();
+}
+
+void Engine::reset_split_estimate() {
+  HIPCHK(hipMemsetAsync(ctl_.as<char>() + offsetof(BtControl, cand_prev), 0,
+                        sizeof(unsigned long long), stream_));
 }
 
 static void check_iterable(const std::vector<Job*>& js) {

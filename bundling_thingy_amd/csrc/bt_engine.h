@@ -78,6 +78,9 @@ class Engine {
   static void read_points_of(Engine& e, Job& j, void* rec12, size_t n);
   static void read_density_of(Engine& e, Job& j, uint32_t kind, void* out, size_t n);
 
+  // Heavy tasks split by the last iteration run, and their parts.
+  void split_counts(uint32_t* tasks, uint32_t* parts);
+
   size_t njobs() const { return jobs_.size(); }
   Job& job(uint32_t i);
   size_t device_bytes() const;
@@ -92,6 +95,9 @@ class Engine {
   void launch_iteration(const Plan& p, const unsigned char* dev, bool density,
                         hipEvent_t* ev = nullptr);
   void ensure_tasks(const std::vector<Job*>& js);
+  // Forgets the previous launch's candidate count (the heavy-task threshold
+  // falls back to its upper bound).
+  void reset_split_estimate();
   void drop_graphs();
   std::vector<Job*> all_jobs();
 

@@ -138,11 +138,12 @@ extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restri
     }
     if (j == 0) {
       ctl->task_total = 0u;
-      ctl->queue_head = 0u;
-      for (int x = 0; x < BT_XCDS; ++x) ctl->xq[x] = 0u;
       ctl->heavy_count = 0u;
       ctl->item_count = 0u;
-      ctl->item_head = 0u;
+      for (int x = 0; x < BT_XCDS; ++x) {
+        ctl->xq[x * BT_CTR_STRIDE] = 0u;
+        ctl->iq[x * BT_CTR_STRIDE] = 0u;
+      }
     }
   }
 }
@@ -162,7 +163,7 @@ extern "C" __global__ __launch_bounds__(256) void bt_bin(const BtJob* __restrict
   uint32_t c = 0;
   if (valid) {
     float x = rec[3u * i], y = rec[3u * i + 1u];
-    uint32_t cx = cell_axis(texel(x, J.res_x), J.cell, J.gx);
+    uint32_t cx = cell_axis(texel(x, J.res_x), J.cell_w, J.gx);
     uint32_t cy = cell_axis(texel(y, J.res_y), J.cell, J.gy);
     c = cy * J.gx + cx;
   }
@@ -276,9 +277,13 @@ extern "C" __global__ __launch_bounds__(256) void bt_cell_scan_apply(
 }
 
 // ------------------------------------------------------------------ tasks
-// One task per <= 64 samples of a cell (the reference's Task, one per <= 128,
-// spatial_grid.wgsl:117-123). Task slots are reserved with one atomic per
-// workgroup; task order is irrelevant to the results.
+// Neighbor-mean tasks (the reference's Task, spatial_grid.wgsl:117-123, one
+// per <= 128 samples of a cell). A row's bins are grouped into blocks of
+// task_bins consecutive bins, whose samples are contiguous in sorted order;
+// each block is cut into tasks of up to 64 consecutive samples. Slots are
+// reserved with one atomic per workgroup and written by all its threads (one
+// block of a collapsed bundle can hold thousands of tasks). Task order is
+// irrelevant to the results.
 extern "C" __global__ __launch_bounds__(256) void bt_tasks(const BtJob* __restrict__ jobs,
                                                            const uint32_t* __restrict__ base,
                                                            int njobs, BtControl* ctl,
@@ -286,40 +291,54 @@ extern "C" __global__ __launch_bounds__(256) void bt_tasks(const BtJob* __restri
   JOB_PROLOGUE(BT_K_TASK);
   __shared__ uint32_t part[4];
   __shared__ uint32_t block_base;
-  const uint32_t c0 = lb * BT_TASK_CELLS + threadIdx.x * 4u;
+  __shared__ uint32_t first[BT_TASK_BLOCKS];
+  __shared__ uint32_t bstart[BT_TASK_BLOCKS];
+  __shared__ uint32_t bend[BT_TASK_BLOCKS];
+  const uint32_t nblocks = J.task_bx * J.gy;
+  const uint32_t b0 = lb * BT_TASK_BLOCKS + threadIdx.x * 4u;
   const uint32_t* cells = J.cells;
-  uint32_t st[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    uint32_t c = c0 + (uint32_t)k;
-    st[k] = c <= J.n_cells ? cells[c] : 0u;
-  }
-  uint32_t nt[4];
+  uint32_t st[4], en[4], nt[4];
   uint32_t mine = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    uint32_t c = c0 + (uint32_t)k;
-    uint32_t cnt = c < J.n_cells ? st[k + 1] - st[k] : 0u;
-    nt[k] = (cnt + BT_TASK_POINTS - 1u) / BT_TASK_POINTS;
+    const uint32_t b = b0 + (uint32_t)k;
+    st[k] = en[k] = 0u;
+    if (b < nblocks) {
+      const uint32_t row = b / J.task_bx, bx = b - row * J.task_bx;
+      const uint32_t x0 = bx * J.task_bins, x1 = min(x0 + J.task_bins, J.gx);
+      st[k] = cells[row * J.gx + x0];
+      en[k] = cells[row * J.gx + x1];
+    }
+    nt[k] = (en[k] - st[k] + BT_TASK_POINTS - 1u) / BT_TASK_POINTS;
     mine += nt[k];
   }
   uint32_t total;
   uint32_t before = block_excl_scan(mine, part, &total);
-  if (threadIdx.x == 0) block_base = total ? atomicAdd(&ctl->task_total, total) : 0u;
-  __syncthreads();
-  uint32_t pos = block_base + before;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    for (uint32_t t = 0; t < nt[k]; ++t) {
-      uint32_t s = st[k] + t * BT_TASK_POINTS;
-      uint32_t e = min(s + BT_TASK_POINTS, st[k + 1]);
-      BtTask task;
-      task.job = (uint32_t)j;
-      task.start = s;
-      task.end = e;
-      task.pad = 0u;
-      tasks[pos++] = task;
+    const uint32_t l = threadIdx.x * 4u + (uint32_t)k;
+    first[l] = before;
+    bstart[l] = st[k];
+    bend[l] = en[k];
+    before += nt[k];
+  }
+  if (threadIdx.x == 0) block_base = total ? atomicAdd(&ctl->task_total, total) : 0u;
+  __syncthreads();
+  const uint32_t pos0 = block_base;
+  for (uint32_t i = threadIdx.x; i < total; i += 256u) {
+    // the block holding task i: the last one whose first task is <= i
+    uint32_t lo = 0, hi = BT_TASK_BLOCKS - 1u;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1u) >> 1;
+      if (first[mid] <= i) lo = mid; else hi = mid - 1u;
     }
+    const uint32_t s = bstart[lo] + (i - first[lo]) * BT_TASK_POINTS;
+    BtTask task;
+    task.job = (uint32_t)j;
+    task.start = s;
+    task.end = min(s + BT_TASK_POINTS, bend[lo]);
+    task.pad = 0u;
+    tasks[pos0 + i] = task;
   }
 }
 
@@ -368,17 +387,19 @@ extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_apply(
 }
 
 // ---------------------------------------------------------- neighbor mean
-// neighbor_mean.wgsl:75-203. One wave per task (<= 64 query samples of one
-// cell). The grid rows of the task's window are set up in parallel, one row
-// per lane: each row's cells split (uniformly for the wave) into
-//   - cells fully inside the disk of every query sample (conservative, with
+// neighbor_mean.wgsl:75-203. One wave per task (<= 64 query samples that are
+// consecutive in sorted order, from a few neighbouring bins of one row). The
+// grid rows of the task's window are set up in parallel, one row per lane:
+// each row's bins split (uniformly for the wave) into
+//   - bins fully inside the disk of every query sample (conservative, with
 //     an eps margin): all their samples pass the neighbor test, so their
 //     fixed-point sums come in O(1) from the prefix sums of the sorted
 //     coordinates (the role of the reference's CellSummary, :56-59);
-//   - boundary cells, whose samples must be tested;
-//   - cells beyond every disk, skipped.
-// Within a row the cells are contiguous in sorted order, so a row yields at
-// most two boundary ranges. The ranges of up to 64 rows form one candidate
+//   - boundary bins, whose samples must be tested;
+//   - bins beyond every disk, skipped.
+// Within a row the bins are contiguous in sorted order, so a row yields at
+// most two boundary ranges; bins narrower than the row height (x_split) end
+// those ranges within a fraction of a row height of the disk's edge. The ranges of up to 64 rows form one candidate
 // stream, fetched 64 samples at a time with coalesced loads into a
 // double-buffered wave-private LDS tile (the next chunk is in flight while
 // the current one is tested) and broadcast to every lane, which tests its own
@@ -396,27 +417,36 @@ __device__ __forceinline__ int cell_of_fast(float t, float inv_cell, uint32_t g)
   return (int)v;
 }
 
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
 __device__ __forceinline__ long long wave_sum64(long long v) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
   return v;
 }
 
-// Work distribution: the task list is cut into BT_XCDS contiguous parts (tasks
-// are emitted in roughly spatial order, so a part is a spatially coherent
-// region); workgroup b runs on XCD b % BT_XCDS and first drains that XCD's
-// part, keeping its candidate ranges in one L2, then takes tasks from the
-// other parts. Every cursor only grows, so every wave ends after at most one
-// failed claim per part.
-__device__ __forceinline__ uint32_t claim_task(BtControl* ctl, uint32_t total, uint32_t& part,
+// Work distribution: a work list (tasks, or parts of heavy tasks) is cut into
+// BT_XCDS contiguous parts (tasks are emitted in roughly spatial order, so a
+// part is a spatially coherent region); workgroup b runs on XCD b % BT_XCDS
+// and first drains that XCD's part, keeping its candidate ranges in one L2,
+// then takes work from the other parts. The cursors sit on separate cache
+// lines; each only grows, so every wave ends after at most one failed claim
+// per part.
+__device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total, uint32_t& part,
                                                uint32_t& tried) {
   while (tried < BT_XCDS) {
     const uint32_t lo = (uint32_t)(((uint64_t)total * part) / BT_XCDS);
     const uint32_t hi = (uint32_t)(((uint64_t)total * (part + 1u)) / BT_XCDS);
-    uint32_t k = 0;
-    if (LANE() == 0) k = atomicAdd(&ctl->xq[part], 1u);
-    k = __builtin_amdgcn_readfirstlane(k);
-    if (lo + k < hi) return lo + k;
+    if (lo < hi) {
+      uint32_t k = 0;
+      if (LANE() == 0) k = atomicAdd(&cursors[part * BT_CTR_STRIDE], 1u);
+      k = __builtin_amdgcn_readfirstlane(k);
+      if (lo + k < hi) return lo + k;
+    }
     part = (part + 1u) % BT_XCDS;
     ++tried;
   }
@@ -484,7 +514,7 @@ __device__ __forceinline__ void nm_rows(const NmTask& T, uint32_t rb, NmLds& L, 
                                         uint32_t& nranges) {
   const uint32_t lane = LANE(), w = threadIdx.x >> 6;
   const BtJob& J = *T.J;
-  const float c = J.cell, inv_c = J.inv_cell, eps = J.eps, r = J.radius;
+  const float c = J.cell, inv_w = J.inv_cell_w, eps = J.eps, r = J.radius;
   const float rr = r + eps, ri = r - eps;
   const uint32_t gx = J.gx, gy = J.gy;
   const uint32_t* cells = J.cells;
@@ -496,14 +526,14 @@ __device__ __forceinline__ void nm_rows(const NmTask& T, uint32_t rb, NmLds& L, 
     const float dyn = fmaxf(fmaxf(Y0 - eps - T.ty1, T.ty0 - (Y1 + eps)), 0.0f);
     if (dyn < rr) {
       const float wout = __builtin_sqrtf(rr * rr - dyn * dyn);
-      const uint32_t xlo = (uint32_t)cell_of_fast(T.tx0 - wout, inv_c, gx);
-      const uint32_t xhi = (uint32_t)cell_of_fast(T.tx1 + wout, inv_c, gx);
+      const uint32_t xlo = (uint32_t)cell_of_fast(T.tx0 - wout, inv_w, gx);
+      const uint32_t xhi = (uint32_t)cell_of_fast(T.tx1 + wout, inv_w, gx);
       int xa = 1, xb = 0;
       const float dyf = fmaxf(Y1 + eps - T.ty0, T.ty1 - (Y0 - eps));
       if (cy > 0u && cy + 1u < gy && dyf < ri) {
         const float win = __builtin_sqrtf(ri * ri - dyf * dyf);
-        const float fa = ceilf((T.tx1 - win + eps) * inv_c);
-        const float fb = floorf((T.tx0 + win - eps) * inv_c) - 1.0f;
+        const float fa = ceilf((T.tx1 - win + eps) * inv_w);
+        const float fb = floorf((T.tx0 + win - eps) * inv_w) - 1.0f;
         xa = (int)fminf(fmaxf(fa, fmaxf((float)xlo, 1.0f)), (float)gx);
         xb = (int)fmaxf(fminf(fb, fminf((float)xhi, (float)gx - 2.0f)), -1.0f);
       }
@@ -645,11 +675,22 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split) {
   __shared__ NmLds L;
+  __shared__ unsigned long long wave_cand[NM_WAVES];
   const uint32_t lane = LANE();
   const uint32_t total = uni(__atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
+  // split threshold from the previous launch's candidates per wave
+  const unsigned long long prev = ctl->cand_prev;
+  uint32_t heavy = BT_HEAVY_HI;
+  if (prev) {
+    const unsigned long long per_wave = prev / ((unsigned long long)gridDim.x * NM_WAVES);
+    heavy = (uint32_t)min(max(2ull * per_wave, (unsigned long long)BT_HEAVY_LO),
+                          (unsigned long long)BT_HEAVY_HI);
+  }
+  const uint32_t part_cands = heavy / 4u;
+  unsigned long long cand = 0;
   uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
   for (;;) {
-    const uint32_t t = uni(claim_task(ctl, total, part, tried));
+    const uint32_t t = uni(claim_work(ctl->xq, total, part, tried));
     if (t >= total) break;
     NmTask T;
     nm_load(T, jobs, tasks, t);
@@ -659,9 +700,10 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
       for (uint32_t rb = T.cy0; rb <= T.cy1; rb += 64u) {
         uint32_t m, nranges;
         nm_rows(T, rb, L, ux, uy, un, m, nranges);
-        if (rb == T.cy0 && T.cy1 - T.cy0 < 64u && m > BT_HEAVY_MIN) {
+        cand += m;
+        if (rb == T.cy0 && T.cy1 - T.cy0 < 64u && m > heavy) {
           // one row group with a long stream: record it for the split kernel
-          const uint32_t parts = min(BT_MAX_PARTS, (m + BT_PART_CANDS - 1u) / BT_PART_CANDS);
+          const uint32_t parts = min(BT_MAX_PARTS, (m + part_cands - 1u) / part_cands);
           uint32_t h = 0, base = 0;
           if (lane == 0) {
             h = atomicAdd(&ctl->heavy_count, 1u);
@@ -696,6 +738,14 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
     }
     nm_finish(T, ax + ux, ay + uy, n + un);
   }
+  // this workgroup's candidates (every wave leaves the loop above)
+  if (lane == 0) wave_cand[threadIdx.x >> 6] = cand;
+  __syncthreads();
+  if (threadIdx.x == 0 && blockIdx.x < BT_MAX_WG) {
+    unsigned long long s = 0;
+    for (int w = 0; w < NM_WAVES; ++w) s += wave_cand[w];
+    split.wg_cand[blockIdx.x] = s;
+  }
 }
 
 // Runs the parts of heavy tasks: part p tests chunks p, p + P, p + 2P, ... of
@@ -710,10 +760,19 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
   const uint32_t lane = LANE();
   const uint32_t items = min(uni(__atomic_load_n(&ctl->item_count, __ATOMIC_RELAXED)),
                              BT_MAX_ITEMS);
+  if (blockIdx.x == 0) {
+    // total candidates of the main launch (same grid), for the next threshold
+    __shared__ unsigned long long red[4];
+    unsigned long long s = 0;
+    for (uint32_t b = threadIdx.x; b < min(gridDim.x, BT_MAX_WG); b += 256u) s += split.wg_cand[b];
+    s = wave_sum_u64(s);
+    if (lane == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) ctl->cand_prev = red[0] + red[1] + red[2] + red[3];
+  }
+  uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
   for (;;) {
-    uint32_t i = 0;
-    if (lane == 0) i = atomicAdd(&ctl->item_head, 1u);
-    i = uni(i);
+    const uint32_t i = uni(claim_work(ctl->iq, items, part, tried));
     if (i >= items) break;
     const uint2 it = split.items[i];
     const uint32_t h = uni(it.x), p = uni(it.y);

@@ -55,19 +55,23 @@ typedef struct bt_options {
   float sigma0, sigma1;            /* generalized: blur sigma schedule, pixels */
   uint32_t schedule_len;           /* generalized: iterations the schedules span */
   uint32_t density_w, density_h;   /* generalized: density size */
-  uint32_t cell_split;             /* grid cells per radius; 0 = autotuned */
-  uint32_t reserved[3];
+  uint32_t cell_split;             /* grid rows per radius; 0 = autotuned */
+  uint32_t x_split;                /* grid bins per row height along x; 0 = 8,
+                                      1 = square cells as in the reference */
+  uint32_t reserved[2];
 } bt_options;
 
 /* Exact parameters of one iteration (what the kernels use). */
 typedef struct bt_iter_params {
   float res_x, res_y;   /* texel resolution: texel_scale = res * 0.5 */
   float radius;         /* texels */
-  float cell;           /* grid cell side, texels */
-  uint32_t gx, gy;      /* grid dims */
+  float cell;           /* grid row height, texels (the reference's cell side) */
+  uint32_t gx, gy;      /* grid dims: bins per row, rows */
   float sigma;          /* pixels */
   uint32_t taps;        /* blur half-width = ceil(3 sigma) */
   uint32_t density_w, density_h;
+  float cell_w;         /* grid bin width along x, texels (cell / x_split) */
+  uint32_t task_bins;   /* bins per neighbor-mean task block along x */
 } bt_iter_params;
 
 typedef struct bt_info {
@@ -78,6 +82,9 @@ typedef struct bt_info {
   uint64_t capacity;    /* job 0: sample capacity of the budget model */
   uint64_t reserved_bytes;
   uint64_t device_bytes;/* device memory held by the context */
+  uint32_t split_tasks; /* neighbor-mean tasks of the last iteration run that were
+                           split across waves (long candidate streams) */
+  uint32_t split_parts; /* their parts */
 } bt_info;
 
 typedef struct bt_ctx bt_ctx;

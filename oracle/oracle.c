@@ -370,11 +370,15 @@ static inline uint32_t cell_axis(float t, float cell, uint32_t g) {
     return v >= (float)(g - 1) ? g - 1 : (uint32_t)v;
 }
 
+/* Bin width along x: the reference's grid is square (cell_w = 0). Any bin
+ * width gives the same neighbor sums; only the candidate windows change. */
+static inline float bin_w(const or_grid* g) { return g->cell_w > 0.0f ? g->cell_w : g->cell; }
+
 void or_grid_bin(const float* pts, uint32_t n, const or_grid* g, uint32_t* cell_of,
                  uint32_t* counts) {
     memset(counts, 0, (size_t)g->gx * g->gy * sizeof(uint32_t));
     for (uint32_t i = 0; i < n; i++) {
-        uint32_t cx = cell_axis(texel(pts[2 * i], g->res_x), g->cell, g->gx);
+        uint32_t cx = cell_axis(texel(pts[2 * i], g->res_x), bin_w(g), g->gx);
         uint32_t cy = cell_axis(texel(pts[2 * i + 1], g->res_y), g->cell, g->gy);
         uint32_t c = cy * g->gx + cx;
         if (cell_of) cell_of[i] = c;
@@ -463,8 +467,8 @@ static void neighbor_mean_impl(const float* in, const uint8_t* fixed, uint32_t n
             continue;
         }
         float tx = texel(px, g->res_x), ty = texel(py, g->res_y);
-        uint32_t x0 = cell_axis(tx - d.reach_x, g->cell, g->gx);
-        uint32_t x1 = cell_axis(tx + d.reach_x, g->cell, g->gx);
+        uint32_t x0 = cell_axis(tx - d.reach_x, bin_w(g), g->gx);
+        uint32_t x1 = cell_axis(tx + d.reach_x, bin_w(g), g->gx);
         uint32_t y0 = cell_axis(ty - d.reach_y, g->cell, g->gy);
         uint32_t y1 = cell_axis(ty + d.reach_y, g->cell, g->gy);
         int64_t ax = 0, ay = 0;
@@ -510,8 +514,8 @@ void or_neighbor_mean_f32(const float* in, const uint8_t* fixed, uint32_t n,
             continue;
         }
         float tx = texel(px, g->res_x), ty = texel(py, g->res_y);
-        uint32_t x0 = cell_axis(tx - reach_x, g->cell, g->gx);
-        uint32_t x1 = cell_axis(tx + reach_x, g->cell, g->gx);
+        uint32_t x0 = cell_axis(tx - reach_x, bin_w(g), g->gx);
+        uint32_t x1 = cell_axis(tx + reach_x, bin_w(g), g->gx);
         uint32_t y0 = cell_axis(ty - reach_y, g->cell, g->gy);
         uint32_t y1 = cell_axis(ty + reach_y, g->cell, g->gy);
         float sx = 0.0f, sy = 0.0f;

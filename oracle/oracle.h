@@ -42,8 +42,9 @@ typedef struct {
 typedef struct {
     float res_x, res_y;   /* samples per clip half-extent * 2: reference map size - 1 */
     float radius;         /* neighbor radius in texels */
-    float cell;           /* grid cell side in texels (reference: == radius) */
-    uint32_t gx, gy;      /* grid dims: ceil((res + 1) / cell) */
+    float cell;           /* grid cell side (row height) in texels (reference: == radius) */
+    uint32_t gx, gy;      /* grid dims: ceil((res_x + 1) / cell_w), ceil((res_y + 1) / cell) */
+    float cell_w;         /* bin width along x in texels; 0 = cell (the reference's square cells) */
 } or_grid;
 
 /* Derived per-iteration constants; identical formulas on the device. */

@@ -17,7 +17,8 @@ class OrNorm(C.Structure):
 
 class OrGrid(C.Structure):
     _fields_ = [("res_x", C.c_float), ("res_y", C.c_float), ("radius", C.c_float),
-                ("cell", C.c_float), ("gx", C.c_uint32), ("gy", C.c_uint32)]
+                ("cell", C.c_float), ("gx", C.c_uint32), ("gy", C.c_uint32),
+                ("cell_w", C.c_float)]
 
 
 class OrDerived(C.Structure):
@@ -106,14 +107,20 @@ def normalization(bbox):
 
 
 def grid(params):
-    """OrGrid from a bt_iter_params-like object (res_x, res_y, radius, cell, gx, gy)."""
-    return OrGrid(params.res_x, params.res_y, params.radius, params.cell, params.gx, params.gy)
+    """OrGrid from a bt_iter_params-like object (res_x, res_y, radius, cell, gx,
+    gy and, if present, the bin width cell_w)."""
+    return OrGrid(params.res_x, params.res_y, params.radius, params.cell, params.gx, params.gy,
+                  getattr(params, "cell_w", 0.0))
 
 
-def make_grid(res, radius, cell=None):
+def make_grid(res, radius, cell=None, cell_w=None):
+    """Square cells of side `cell` (default: the radius, as the reference), or
+    rows of height `cell` split into bins of width `cell_w`."""
     cell = radius if cell is None else cell
-    g = int(np.ceil((np.float64(np.float32(res)) + 1.0) / np.float64(np.float32(cell))))
-    return OrGrid(res, res, radius, cell, g, g)
+    w = cell if cell_w is None else cell_w
+    gy = int(np.ceil((np.float64(np.float32(res)) + 1.0) / np.float64(np.float32(cell))))
+    gx = int(np.ceil((np.float64(np.float32(res)) + 1.0) / np.float64(np.float32(w))))
+    return OrGrid(res, res, radius, cell, gx, gy, 0.0 if cell_w is None else cell_w)
 
 
 def resample(points, starts, bbox, res_x, res_y, capacity=0xfffffffe, density=None):

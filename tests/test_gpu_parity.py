@@ -106,23 +106,46 @@ def test_gpu_case_resample_and_last_radius(bt, orc):
         _assert_density(gpu, pl, "iteration 1")
 
 
-@pytest.mark.parametrize("split", [1, 2, 3, 5, 8, 16])
-def test_cell_split_does_not_change_results(bt, orc, cpu_case, split):
-    """Any grid cell size gives the same bits (inside runs + scanned borders
-    equal the all-pairs test), including the reference grid (split 1)."""
+@pytest.mark.parametrize("split,xs", [(1, 1), (1, 8), (2, 8), (3, 2), (5, 8), (8, 4), (16, 8),
+                                      (16, 16)])
+def test_cell_split_does_not_change_results(bt, orc, cpu_case, split, xs):
+    """Any grid row height and bin width give the same bits (inside runs +
+    scanned borders equal the all-pairs test), including the reference grid
+    (square cells of side r: split 1, x_split 1)."""
     base = workloads.plain_cpu_options()
     opts = bt.Options(spacing=0.005, radius=0.05, sigma=4.0, density=512, schedule_len=10,
-                      cell_split=split)
+                      cell_split=split, x_split=xs)
     pl = _oracle_pipeline(orc, cpu_case, base)
     with bt.Gpu(opts) as gpu:
         gpu.upload(cpu_case)
         for it in range(2):
             gpu.iterate()
             ref = pl.iterate(base.iter_params(it))
-            _assert_points(gpu.points()[0], ref, f"split {split} iteration {it + 1}")
-        if split == 1:
+            _assert_points(gpu.points()[0], ref, f"split {split}/{xs} iteration {it + 1}")
+        if split == 1 and xs == 1:
             cells, counts = gpu.grid()
             assert counts.shape == (21, 21)  # ceil(201 / 10): grid of cell = radius
+
+
+def test_split_heavy_tasks_bitexact(bt, orc, cpu_case):
+    """Square cells as wide as the radius leave long candidate streams; those
+    tasks are split into parts run by many waves, whose exact partial sums
+    must give the same bits as the oracle."""
+    opts = bt.Options(spacing=0.005, radius=0.1, sigma=4.0, density=512, cell_split=1,
+                      x_split=1)
+    pl = _oracle_pipeline(orc, cpu_case, opts)
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(cpu_case)
+        for it in range(3):
+            gpu.iterate()
+            info = gpu.info()
+            # after the first launch the threshold follows the candidates per
+            # wave, which are few here: long streams are split
+            if it > 0:
+                assert info.split_tasks > 0 and info.split_parts > info.split_tasks
+            ref = pl.iterate(opts.iter_params(it))
+            _assert_points(gpu.points()[0], ref, f"iteration {it + 1}")
+            _assert_density(gpu, pl, f"iteration {it + 1}")
 
 
 def test_autotune_keeps_results(bt, cpu_case):

@@ -67,9 +67,9 @@ def test_blur_weights_match_oracle(bt, orc):
 def test_reference_mode_grid_dims(bt):
     # neighbor_mean.rs:27: grid dims ceil(S / r) per axis
     for S, r in [(1600, 10), (201, 10), (1600, 7), (512, 512), (100, 1)]:
-        p = bt.Options(map_size=S, neighbor_radius=r, cell_split=1).iter_params(0)
+        p = bt.Options(map_size=S, neighbor_radius=r, cell_split=1, x_split=1).iter_params(0)
         assert (p.gx, p.gy) == (math.ceil(S / r),) * 2
-        assert p.res_x == S - 1 and p.radius == r and p.cell == r
+        assert p.res_x == S - 1 and p.radius == r and p.cell == r and p.cell_w == r
         assert p.sigma == 6.0 and p.taps == 18 and p.density_w == S
 
 
@@ -79,8 +79,31 @@ def test_generalized_reduces_to_reference_mode(bt, S, r):
     gen = bt.Options(spacing=1.0 / (S - 1), radius=r / (S - 1), sigma=6.0, density=S,
                      schedule_len=1).iter_params(0)
     for f in ["res_x", "res_y", "radius", "cell", "gx", "gy", "sigma", "taps", "density_w",
-              "density_h"]:
+              "density_h", "cell_w", "task_bins"]:
         assert getattr(ref, f) == getattr(gen, f), f
+
+
+@pytest.mark.parametrize("split,xs", [(1, 1), (1, 8), (4, 8), (16, 8), (6, 3), (2, 64)])
+def test_x_split_bins(bt, split, xs):
+    """Rows keep the cell height; bins along x are x_split times narrower; task
+    blocks span about a quarter radius (at least one cell)."""
+    o = bt.Options(spacing=0.002, radius=0.06, sigma=8.0, density=1024, cell_split=split,
+                   x_split=xs)
+    p = o.iter_params(0)
+    cell = np.float32(30.0) if split == 1 else np.float32(np.float32(30.0) / np.float32(split))
+    w = cell if xs == 1 else np.float32(cell / np.float32(xs))
+    assert p.cell == cell and p.cell_w == w
+    assert p.gy == math.ceil(501.0 / float(cell)) and p.gx == math.ceil(501.0 / float(w))
+    assert p.task_bins == xs * max(1, split // 4)
+
+
+def test_x_split_capped_by_grid_size(bt):
+    # 2^22 bins at most: x_split halves until the grid fits
+    o = bt.Options(spacing=0.001, radius=0.01, sigma=2.0, density=1024, cell_split=8,
+                   x_split=64)
+    p = o.iter_params(0)  # square grid 801 x 801; 64 -> 32 -> 16 -> 8 -> 4
+    assert p.gy == 801 and round(p.cell / p.cell_w) == 4
+    assert p.gx * p.gy <= 1 << 22 < p.gx * 2 * p.gy
 
 
 def test_schedule_is_geometric(bt):

@@ -153,11 +153,12 @@ def test_neighbor_mean_bitexact_vs_bruteforce(orc, seed, res, radius):
     np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("cell", [10.0, 5.0, 2.5, 33.0])
-def test_neighbor_mean_independent_of_cell_size(orc, cell):
+@pytest.mark.parametrize("cell,cell_w", [(10.0, None), (5.0, None), (2.5, None), (33.0, None),
+                                         (10.0, 1.25), (2.5, 0.3125), (5.0, 0.7)])
+def test_neighbor_mean_independent_of_cell_size(orc, cell, cell_w):
     pts, fixed = _tiny(5, 400)
     a = orc.neighbor_mean(pts, fixed, orc.make_grid(200.0, 10.0))
-    b = orc.neighbor_mean(pts, fixed, orc.make_grid(200.0, 10.0, cell))
+    b = orc.neighbor_mean(pts, fixed, orc.make_grid(200.0, 10.0, cell, cell_w))
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
