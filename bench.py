@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 ITERS = 30
-TUNE_CANDIDATES = (1, 2, 3, 4, 6, 8, 12, 16, 20, 24)
+TUNE_CANDIDATES = (1, 2, 3, 4, 6, 8, 12, 16, 20, 24, 28)
 COPY_PROBE_BYTES = 256 << 20
 
 
@@ -291,8 +291,10 @@ def main():
             stage_ms[k] += v
         _, counts = gpu.grid()
         p = gpu.job_params(i)
+        info = gpu.info()
         per_iter.append({"radius": round(p.radius, 3), "cell": round(p.cell, 3),
-                         "mean_ms": round(st["neighbor_mean"], 3)})
+                         "mean_ms": round(st["neighbor_mean"], 3),
+                         "split_tasks": info.split_tasks, "split_parts": info.split_parts})
         ky = max(1, int(round(p.radius / p.cell)))      # rows per radius
         kx = max(1, int(round(p.radius / p.cell_w)))    # bins per radius
         gy, gx = counts.shape

@@ -109,16 +109,20 @@ struct BtControl {
 };
 
 // Heavy tasks (candidate stream longer than a threshold H) are split into
-// parts of about H / 4 samples, run by many waves; their exact partial sums
-// meet in per-task accumulators, and the last part to finish writes the
-// means. H is twice the previous launch's candidates per wave, clamped to
-// [BT_HEAVY_LO, BT_HEAVY_HI] (BT_HEAVY_HI when there is no previous launch):
-// a task much longer than the average would otherwise run alone at the end.
-#define BT_HEAVY_LO 4096u
+// parts run by many waves; their exact partial sums meet in per-task
+// accumulators, and the last part to finish writes the means. H is half the
+// previous launch's candidates per wave, clamped to [BT_HEAVY_LO, BT_HEAVY_HI]
+// (BT_HEAVY_HI when there is no previous launch): a task much longer than the
+// average would otherwise run alone at the end. The main kernel stores each
+// heavy task's set-up (lane-group ranges and inside sums, BT_REC_WORDS words)
+// so the parts only stream.
+#define BT_HEAVY_LO 2048u
 #define BT_HEAVY_HI 16384u
+#define BT_PART_STEPS_MIN 512u
+#define BT_REC_WORDS 1088u
 #define BT_MAX_PARTS 64u
-#define BT_MAX_HEAVY 4096u
-#define BT_MAX_ITEMS 65536u
+#define BT_MAX_HEAVY 16384u
+#define BT_MAX_ITEMS 262144u
 #define BT_MAX_WG 8192u       // workgroups of a neighbor-mean launch (per-WG candidate sums)
 
 struct BtHeavy {
@@ -133,6 +137,7 @@ struct BtSplit {          // scratch of the heavy-task split (one per launch)
   uint2* items;           // [BT_MAX_ITEMS]: (heavy index, part)
   long long* acc;         // [BT_MAX_HEAVY][3][64]: per-lane x, y, count sums
   unsigned long long* wg_cand;  // [BT_MAX_WG]: candidates tested per workgroup
+  uint32_t* rec;          // [BT_MAX_HEAVY][BT_REC_WORDS]: set-up of each heavy task
 };
 
 struct __attribute__((aligned(16))) BtTask {

@@ -22,7 +22,8 @@ static const uint32_t kWt = BT_MAX_TAPS + 1;  // weights per iteration row
 static const size_t kSplitItemsOff = (size_t)BT_MAX_HEAVY * sizeof(BtHeavy);
 static const size_t kSplitAccOff = kSplitItemsOff + (size_t)BT_MAX_ITEMS * sizeof(uint2);
 static const size_t kSplitCandOff = kSplitAccOff + (size_t)BT_MAX_HEAVY * 192 * sizeof(long long);
-static const size_t kSplitBytes = kSplitCandOff + (size_t)BT_MAX_WG * sizeof(unsigned long long);
+static const size_t kSplitRecOff = kSplitCandOff + (size_t)BT_MAX_WG * sizeof(unsigned long long);
+static const size_t kSplitBytes = kSplitRecOff + (size_t)BT_MAX_HEAVY * BT_REC_WORDS * 4;
 
 BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   BtJob d;
@@ -259,6 +260,7 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   split.items = reinterpret_cast<uint2*>(split_.as<char>() + kSplitItemsOff);
   split.acc = reinterpret_cast<long long*>(split_.as<char>() + kSplitAccOff);
   split.wg_cand = reinterpret_cast<unsigned long long*>(split_.as<char>() + kSplitCandOff);
+  split.rec = reinterpret_cast<uint32_t*>(split_.as<char>() + kSplitRecOff);
   btk::neighbor_mean(stream_, (uint32_t)cus_ * 4u, J, tasks_.as<BtTask>(), ctl, split);
   mark(6);
   if (density && p.grid[BT_K_ROW]) {

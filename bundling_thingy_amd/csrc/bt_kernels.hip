@@ -455,11 +455,18 @@ __device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total
 
 #define NM_WAVES 4          // waves per workgroup
 #define NM_RANGES 128       // ranges per row group (two per row)
+#define NM_GROUPS 4         // lane groups of the grouped stream (16 lanes each)
+#define NM_GLANES 16
+#define NM_GCHUNK 64        // candidates per group per chunk of the grouped stream
+#define NM_GSTRIDE 65       // float4 per lane group in a tile (one of padding: no bank conflicts)
+#define NM_GROUP_MIN 2048u  // candidates above which a task streams per lane group
 
+// The tile holds either two chunks of 64 candidates (plain stream, double
+// buffered) or one chunk of 64 candidates per lane group (grouped stream).
 struct NmLds {
-  float4 tile[NM_WAVES][2][64];
-  uint32_t start[NM_WAVES][NM_RANGES];
-  uint32_t pref[NM_WAVES][NM_RANGES + 1];
+  float4 tile[NM_WAVES][NM_GROUPS * NM_GSTRIDE];
+  uint32_t start[NM_WAVES][NM_GROUPS][NM_RANGES];
+  uint32_t pref[NM_WAVES][NM_GROUPS][NM_RANGES + 1];
 };
 
 // Per-wave state of one task.
@@ -506,14 +513,15 @@ __device__ __forceinline__ void nm_load(NmTask& T, const BtJob* jobs, const BtTa
   T.cy1 = uni((uint32_t)cell_of_fast(T.ty1 + rr, J.inv_cell, J.gy));
 }
 
-// Sets up rows [rb, rb + 64) of the window: the boundary ranges go to LDS,
-// the inside-run sums are added to (ux, uy, un). Returns the number of
-// candidates m and of ranges.
-__device__ __forceinline__ void nm_rows(const NmTask& T, uint32_t rb, NmLds& L, long long& ux,
-                                        long long& uy, long long& un, uint32_t& m,
-                                        uint32_t& nranges) {
-  const uint32_t lane = LANE(), w = threadIdx.x >> 6;
-  const BtJob& J = *T.J;
+// Sets up rows [rb, min(rb + 64, cy1 + 1)) of the window of the query box
+// [tx0, tx1] x [ty0, ty1] (texels, wave-uniform): the boundary ranges go to
+// start/pref (LDS), the inside-run sums to (sx, sy, sn) (wave-uniform).
+// Returns the number of candidates m and of ranges.
+__device__ __forceinline__ void nm_rows(const BtJob& J, float tx0, float tx1, float ty0, float ty1,
+                                        uint32_t cy1, uint32_t rb, uint32_t* start,
+                                        uint32_t* pref, long long& sx, long long& sy,
+                                        long long& sn, uint32_t& m, uint32_t& nranges) {
+  const uint32_t lane = LANE();
   const float c = J.cell, inv_w = J.inv_cell_w, eps = J.eps, r = J.radius;
   const float rr = r + eps, ri = r - eps;
   const uint32_t gx = J.gx, gy = J.gy;
@@ -521,19 +529,19 @@ __device__ __forceinline__ void nm_rows(const NmTask& T, uint32_t rb, NmLds& L, 
   const uint32_t cy = rb + lane;
   uint32_t r0b = 0, r0e = 0, r1b = 0, r1e = 0;
   long long lx = 0, ly = 0, ln = 0;
-  if (cy <= T.cy1) {
+  if (cy <= cy1) {
     const float Y0 = (float)cy * c, Y1 = Y0 + c;
-    const float dyn = fmaxf(fmaxf(Y0 - eps - T.ty1, T.ty0 - (Y1 + eps)), 0.0f);
+    const float dyn = fmaxf(fmaxf(Y0 - eps - ty1, ty0 - (Y1 + eps)), 0.0f);
     if (dyn < rr) {
       const float wout = __builtin_sqrtf(rr * rr - dyn * dyn);
-      const uint32_t xlo = (uint32_t)cell_of_fast(T.tx0 - wout, inv_w, gx);
-      const uint32_t xhi = (uint32_t)cell_of_fast(T.tx1 + wout, inv_w, gx);
+      const uint32_t xlo = (uint32_t)cell_of_fast(tx0 - wout, inv_w, gx);
+      const uint32_t xhi = (uint32_t)cell_of_fast(tx1 + wout, inv_w, gx);
       int xa = 1, xb = 0;
-      const float dyf = fmaxf(Y1 + eps - T.ty0, T.ty1 - (Y0 - eps));
+      const float dyf = fmaxf(Y1 + eps - ty0, ty1 - (Y0 - eps));
       if (cy > 0u && cy + 1u < gy && dyf < ri) {
         const float win = __builtin_sqrtf(ri * ri - dyf * dyf);
-        const float fa = ceilf((T.tx1 - win + eps) * inv_w);
-        const float fb = floorf((T.tx0 + win - eps) * inv_w) - 1.0f;
+        const float fa = ceilf((tx1 - win + eps) * inv_w);
+        const float fb = floorf((tx0 + win - eps) * inv_w) - 1.0f;
         xa = (int)fminf(fmaxf(fa, fmaxf((float)xlo, 1.0f)), (float)gx);
         xb = (int)fmaxf(fminf(fb, fminf((float)xhi, (float)gx - 2.0f)), -1.0f);
       }
@@ -556,9 +564,9 @@ __device__ __forceinline__ void nm_rows(const NmTask& T, uint32_t rb, NmLds& L, 
       }
     }
   }
-  ux += wave_sum64(lx);
-  uy += wave_sum64(ly);
-  un += wave_sum64(ln);
+  sx = wave_sum64(lx);
+  sy = wave_sum64(ly);
+  sn = wave_sum64(ln);
   const uint32_t l0 = r0e - r0b, l1 = r1e - r1b;
   const uint32_t nr = (l0 ? 1u : 0u) + (l1 ? 1u : 0u);
   const uint32_t len = l0 + l1;
@@ -567,19 +575,52 @@ __device__ __forceinline__ void nm_rows(const NmTask& T, uint32_t rb, NmLds& L, 
   nranges = uni(__shfl(rpos + nr, 63, 64));
   m = uni(__shfl(lpos + len, 63, 64));
   if (l0) {
-    L.start[w][rpos] = r0b;
-    L.pref[w][rpos] = lpos;
+    start[rpos] = r0b;
+    pref[rpos] = lpos;
   }
   if (l1) {
-    L.start[w][rpos + (l0 ? 1u : 0u)] = r1b;
-    L.pref[w][rpos + (l0 ? 1u : 0u)] = lpos + l0;
+    start[rpos + (l0 ? 1u : 0u)] = r1b;
+    pref[rpos + (l0 ? 1u : 0u)] = lpos + l0;
   }
-  if (lane == 0) L.pref[w][nranges] = m;
+  if (lane == 0) pref[nranges] = m;
   __builtin_amdgcn_wave_barrier();
 }
 
+typedef __attribute__((address_space(1))) const f4v gf4v;
+
+// A load from the global address space (its wait counter is separate from
+// LDS waits; a generic pointer would make it a flat load, counted with both).
+__device__ __forceinline__ float4 ld_global(const float4* p, uint32_t i) {
+  const f4v v = ((gf4v*)(uintptr_t)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Candidate v of a stream set up by nm_rows (v < m). `r` is the lane's range
+// cursor: the range holding the lane's previous candidate (0 at the start);
+// a lane's candidates only move forward, so the cursor advances past the
+// ranges that end at or before v (usually none or one per call). Loads from
+// the global address space, so the wait for them is separate from LDS waits.
+__device__ __forceinline__ float4 nm_candidate(const float4* S, const uint32_t* start,
+                                               const uint32_t* pref, uint32_t nranges,
+                                               uint32_t v, uint32_t& r) {
+  while (r + 1u < nranges && pref[r + 1u] <= v) ++r;
+  return ld_global(S, start[r] + (v - pref[r]));
+}
+
+// The neighbor test of one candidate against the lane's sample, adding the
+// candidate's fixed-point coordinates and 1 to the chunk sums when accepted.
+#define NM_TEST(s)                                            \
+  {                                                           \
+    const float dx = ((s).x - px) * tsx;                      \
+    const float dy = ((s).y - py) * tsy;                      \
+    const bool ok = fmaf(dx, dx, dy * dy) <= r2;              \
+    sx += ok ? __float_as_int((s).z) : 0;                     \
+    sy += ok ? __float_as_int((s).w) : 0;                     \
+    sc += ok ? 1 : 0;                                         \
+  }
+
 // Tests the candidate chunks first, first + step, ... of the stream set up in
-// LDS; adds the lane's accepted sums to (ax, ay, n).
+// group 0's LDS ranges; adds the lane's accepted sums to (ax, ay, n).
 __device__ __forceinline__ void nm_stream(const NmTask& T, NmLds& L, uint32_t m, uint32_t nranges,
                                           uint32_t first, uint32_t step, long long& ax,
                                           long long& ay, long long& n) {
@@ -587,24 +628,20 @@ __device__ __forceinline__ void nm_stream(const NmTask& T, NmLds& L, uint32_t m,
   const BtJob& J = *T.J;
   const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2, px = T.px, py = T.py;
   const float4* S = J.sorted;
+  const uint32_t* start = L.start[w][0];
+  const uint32_t* pref = L.pref[w][0];
   const uint32_t nch = (m + 63u) >> 6;
+  uint32_t cur = 0;
   auto fetch = [&](uint32_t ch) -> float4 {
     const uint32_t v = (ch << 6) + lane;
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (v < m) {
-      uint32_t lo = 0, hi = nranges - 1u;  // last range with pref <= v
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1u) >> 1;
-        if (L.pref[w][mid] <= v) lo = mid; else hi = mid - 1u;
-      }
-      s = S[L.start[w][lo] + (v - L.pref[w][lo])];
-    }
-    return s;
+    return v < m ? nm_candidate(S, start, pref, nranges, v, cur)
+                 : make_float4(0.f, 0.f, 0.f, 0.f);
   };
   float4 nxt = first < nch ? fetch(first) : make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t b = 0;
   for (uint32_t ch = first; ch < nch; ch += step) {
-    L.tile[w][b][lane] = nxt;
+    float4* tile = &L.tile[w][b * 64u];
+    tile[lane] = nxt;
     if (ch + step < nch) nxt = fetch(ch + step);
     __builtin_amdgcn_wave_barrier();
     const uint32_t cnt = min(64u, m - (ch << 6));
@@ -613,23 +650,13 @@ __device__ __forceinline__ void nm_stream(const NmTask& T, NmLds& L, uint32_t m,
     for (; k + 4u <= cnt; k += 4u) {
 #pragma unroll
       for (uint32_t u = 0; u < 4u; ++u) {
-        const float4 s = L.tile[w][b][k + u];
-        const float dx = (s.x - px) * tsx;
-        const float dy = (s.y - py) * tsy;
-        const bool ok = fmaf(dx, dx, dy * dy) <= r2;
-        sx += ok ? __float_as_int(s.z) : 0;
-        sy += ok ? __float_as_int(s.w) : 0;
-        sc += ok ? 1 : 0;
+        const float4 s = tile[k + u];
+        NM_TEST(s)
       }
     }
     for (; k < cnt; ++k) {
-      const float4 s = L.tile[w][b][k];
-      const float dx = (s.x - px) * tsx;
-      const float dy = (s.y - py) * tsy;
-      const bool ok = fmaf(dx, dx, dy * dy) <= r2;
-      sx += ok ? __float_as_int(s.z) : 0;
-      sy += ok ? __float_as_int(s.w) : 0;
-      sc += ok ? 1 : 0;
+      const float4 s = tile[k];
+      NM_TEST(s)
     }
     ax += sx;
     ay += sy;
@@ -637,6 +664,170 @@ __device__ __forceinline__ void nm_stream(const NmTask& T, NmLds& L, uint32_t m,
     b ^= 1u;
     __builtin_amdgcn_wave_barrier();
   }
+}
+
+__device__ __forceinline__ float group_min(float v) {
+#pragma unroll
+  for (int d = NM_GLANES / 2; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ float group_max(float v) {
+#pragma unroll
+  for (int d = NM_GLANES / 2; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+
+// Grouped streams of a task whose window is one row group: the wave's lanes
+// form NM_GROUPS groups of 16 consecutive samples (neighbouring along x),
+// each with its own query box, inside sums and boundary ranges, which are
+// narrower than the whole task's. Per step, each group's lanes test the next
+// candidate of their group's stream (exhausted streams read candidates at
+// infinity, which fail the test), so the task takes max over groups of the
+// stream length instead of the length of the task's union stream.
+struct NmGroups {
+  uint32_t my_m, my_n;  // the lane's group: stream length and ranges
+  uint32_t mmax;        // longest group stream (wave-uniform)
+  long long sx, sy, sn; // inside sums of the lane's group
+};
+
+// Sets up every group's ranges (LDS) and inside sums.
+__device__ __forceinline__ NmGroups nm_group_rows(const NmTask& T, NmLds& L) {
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / NM_GLANES;
+  const BtJob& J = *T.J;
+  const bool mv = T.movable;
+  const float gx0 = group_min(mv ? T.px : 3.0e38f), gx1 = group_max(mv ? T.px : -3.0e38f);
+  const float gy0 = group_min(mv ? T.py : 3.0e38f), gy1 = group_max(mv ? T.py : -3.0e38f);
+  const float rr = J.radius + J.eps;
+  NmGroups G;
+  G.my_m = G.my_n = G.mmax = 0u;
+  G.sx = G.sy = G.sn = 0;
+  const uint64_t movable = ballot(mv);
+#pragma unroll
+  for (uint32_t g = 0; g < NM_GROUPS; ++g) {
+    const uint32_t l = g * NM_GLANES;
+    if (!(movable & (0xffffull << l))) continue;  // no movable sample in the group
+    const float tx0 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gx0, (int)l, 64), J.res_x))));
+    const float tx1 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gx1, (int)l, 64), J.res_x))));
+    const float ty0 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gy0, (int)l, 64), J.res_y))));
+    const float ty1 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gy1, (int)l, 64), J.res_y))));
+    const uint32_t c0 = uni((uint32_t)cell_of_fast(ty0 - rr, J.inv_cell, J.gy));
+    const uint32_t c1 = uni((uint32_t)cell_of_fast(ty1 + rr, J.inv_cell, J.gy));
+    long long sx, sy, sn;
+    uint32_t m, nr;
+    nm_rows(J, tx0, tx1, ty0, ty1, c1, c0, L.start[w][g], L.pref[w][g], sx, sy, sn, m, nr);
+    if (grp == g) {
+      G.sx = sx;
+      G.sy = sy;
+      G.sn = sn;
+      G.my_m = m;
+      G.my_n = nr;
+    }
+    G.mmax = max(G.mmax, m);
+  }
+  return G;
+}
+
+// Tests the group chunks first, first + step, ... (NM_GCHUNK steps each);
+// adds the lane's accepted sums to (ax, ay, n).
+__device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds& L, const NmGroups& G,
+                                                uint32_t first, uint32_t step, long long& ax,
+                                                long long& ay, long long& n) {
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / NM_GLANES;
+  const BtJob& J = *T.J;
+  const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2, px = T.px, py = T.py;
+  const float4* S = J.sorted;
+  const uint32_t* start = L.start[w][grp];
+  const uint32_t* pref = L.pref[w][grp];
+  const float inf = __int_as_float(0x7f800000);
+  const uint32_t nch = (G.mmax + NM_GCHUNK - 1u) / NM_GCHUNK;
+  const uint32_t sub = lane % NM_GLANES;
+  uint32_t cur = 0;
+  auto fetch = [&](uint32_t ch, uint32_t i) -> float4 {
+    const uint32_t v = ch * NM_GCHUNK + i * NM_GLANES + sub;
+    return v < G.my_m ? nm_candidate(S, start, pref, G.my_n, v, cur)
+                      : make_float4(inf, inf, 0.f, 0.f);
+  };
+  float4* tile = &L.tile[w][grp * NM_GSTRIDE];
+  float4 nxt[NM_GCHUNK / NM_GLANES];
+  if (first < nch) {
+#pragma unroll
+    for (uint32_t i = 0; i < NM_GCHUNK / NM_GLANES; ++i) nxt[i] = fetch(first, i);
+  }
+  for (uint32_t ch = first; ch < nch; ch += step) {
+    // one buffer: the previous chunk's reads are complete (wave barrier below)
+#pragma unroll
+    for (uint32_t i = 0; i < NM_GCHUNK / NM_GLANES; ++i) tile[i * NM_GLANES + sub] = nxt[i];
+    __builtin_amdgcn_wave_barrier();
+    if (ch + step < nch) {
+#pragma unroll
+      for (uint32_t i = 0; i < NM_GCHUNK / NM_GLANES; ++i) nxt[i] = fetch(ch + step, i);
+    }
+    const uint32_t cnt = min((uint32_t)NM_GCHUNK, G.mmax - ch * (uint32_t)NM_GCHUNK);
+    int sx = 0, sy = 0, sc = 0;
+    uint32_t k = 0;
+    for (; k + 4u <= cnt; k += 4u) {
+#pragma unroll
+      for (uint32_t u = 0; u < 4u; ++u) {
+        const float4 s = tile[k + u];
+        NM_TEST(s)
+      }
+    }
+    for (; k < cnt; ++k) {
+      const float4 s = tile[k];
+      NM_TEST(s)
+    }
+    ax += sx;
+    ay += sy;
+    n += sc;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Record of a heavy task's set-up (BT_REC_WORDS words): per lane group its
+// ranges (start[128], pref[129]), then stream lengths m[4], range counts
+// n[4], then per group the inside sums x, y, count (int64, 24 words).
+#define NM_REC_GROUP (NM_RANGES + NM_RANGES + 1u)
+#define NM_REC_M (NM_GROUPS * NM_REC_GROUP)
+#define NM_REC_N (NM_REC_M + NM_GROUPS)
+#define NM_REC_SUMS (NM_REC_N + NM_GROUPS)
+static_assert(NM_REC_SUMS + 6 * NM_GROUPS <= BT_REC_WORDS, "heavy record layout");
+static_assert(NM_REC_SUMS % 2 == 0, "int64 sums must be 8-byte aligned");
+
+__device__ __forceinline__ void nm_store_groups(const NmGroups& G, NmLds& L, uint32_t* rec) {
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / NM_GLANES;
+  for (uint32_t i = lane; i < NM_REC_M; i += 64u) {
+    const uint32_t g = i / NM_REC_GROUP, o = i - g * NM_REC_GROUP;
+    rec[i] = o < NM_RANGES ? L.start[w][g][o] : L.pref[w][g][o - NM_RANGES];
+  }
+  if (lane % NM_GLANES == 0) {
+    rec[NM_REC_M + grp] = G.my_m;
+    rec[NM_REC_N + grp] = G.my_n;
+    long long* sums = (long long*)(rec + NM_REC_SUMS) + 3u * grp;
+    sums[0] = G.sx;
+    sums[1] = G.sy;
+    sums[2] = G.sn;
+  }
+}
+
+__device__ __forceinline__ NmGroups nm_load_groups(NmLds& L, const uint32_t* rec) {
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / NM_GLANES;
+  for (uint32_t i = lane; i < NM_REC_M; i += 64u) {
+    const uint32_t g = i / NM_REC_GROUP, o = i - g * NM_REC_GROUP;
+    const uint32_t v = rec[i];
+    if (o < NM_RANGES) L.start[w][g][o] = v; else L.pref[w][g][o - NM_RANGES] = v;
+  }
+  NmGroups G;
+  G.mmax = 0u;
+#pragma unroll
+  for (uint32_t g = 0; g < NM_GROUPS; ++g) G.mmax = max(G.mmax, uni(rec[NM_REC_M + g]));
+  G.my_m = rec[NM_REC_M + grp];
+  G.my_n = rec[NM_REC_N + grp];
+  const long long* sums = (const long long*)(rec + NM_REC_SUMS) + 3u * grp;
+  G.sx = sums[0];
+  G.sy = sums[1];
+  G.sn = sums[2];
+  __builtin_amdgcn_wave_barrier();
+  return G;
 }
 
 // Mean of a movable sample from its exact sums, output record, density splat
@@ -671,22 +862,22 @@ __device__ __forceinline__ void nm_finish(const NmTask& T, long long sx, long lo
   }
 }
 
-extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
+extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split) {
   __shared__ NmLds L;
   __shared__ unsigned long long wave_cand[NM_WAVES];
-  const uint32_t lane = LANE();
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6;
   const uint32_t total = uni(__atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
   // split threshold from the previous launch's candidates per wave
   const unsigned long long prev = ctl->cand_prev;
   uint32_t heavy = BT_HEAVY_HI;
   if (prev) {
     const unsigned long long per_wave = prev / ((unsigned long long)gridDim.x * NM_WAVES);
-    heavy = (uint32_t)min(max(2ull * per_wave, (unsigned long long)BT_HEAVY_LO),
+    heavy = (uint32_t)min(max(per_wave / 2ull, (unsigned long long)BT_HEAVY_LO),
                           (unsigned long long)BT_HEAVY_HI);
   }
-  const uint32_t part_cands = heavy / 4u;
+  const uint32_t part_steps = max(heavy / 4u, BT_PART_STEPS_MIN);
   unsigned long long cand = 0;
   uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
   for (;;) {
@@ -699,39 +890,54 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean(
       bool deferred = false;
       for (uint32_t rb = T.cy0; rb <= T.cy1; rb += 64u) {
         uint32_t m, nranges;
-        nm_rows(T, rb, L, ux, uy, un, m, nranges);
+        long long sx, sy, sn;
+        nm_rows(*T.J, T.tx0, T.tx1, T.ty0, T.ty1, T.cy1, rb, L.start[w][0], L.pref[w][0], sx, sy,
+                sn, m, nranges);
         cand += m;
-        if (rb == T.cy0 && T.cy1 - T.cy0 < 64u && m > heavy) {
-          // one row group with a long stream: record it for the split kernel
-          const uint32_t parts = min(BT_MAX_PARTS, (m + part_cands - 1u) / part_cands);
-          uint32_t h = 0, base = 0;
-          if (lane == 0) {
-            h = atomicAdd(&ctl->heavy_count, 1u);
-            base = h < BT_MAX_HEAVY ? atomicAdd(&ctl->item_count, parts) : BT_MAX_ITEMS;
-          }
-          h = uni(h);
-          base = uni(base);
-          if (h < BT_MAX_HEAVY && base + parts <= BT_MAX_ITEMS) {
-            for (uint32_t p = lane; p < parts; p += 64u) split.items[base + p] = make_uint2(h, p);
-            long long* acc = split.acc + (size_t)h * 192u;
-            acc[lane] = 0;
-            acc[64u + lane] = 0;
-            acc[128u + lane] = 0;
+        const bool one_group = rb == T.cy0 && T.cy1 - T.cy0 < 64u;
+        if (one_group && m > NM_GROUP_MIN) {
+          // grouped streams; a long one is recorded for the split kernel
+          const NmGroups G = nm_group_rows(T, L);  // its own ranges and inside sums
+          if (m > heavy) {
+            const uint32_t parts = min(BT_MAX_PARTS, (G.mmax + part_steps - 1u) / part_steps);
+            uint32_t h = 0, base = 0;
             if (lane == 0) {
-              BtHeavy hv;
-              hv.task = t;
-              hv.parts = parts;
-              hv.done = 0u;
-              hv.pad = 0u;
-              split.heavy[h] = hv;
+              h = atomicAdd(&ctl->heavy_count, 1u);
+              base = h < BT_MAX_HEAVY ? atomicAdd(&ctl->item_count, parts) : BT_MAX_ITEMS;
             }
-            deferred = true;
-            break;
+            h = uni(h);
+            base = uni(base);
+            if (h < BT_MAX_HEAVY && base + parts <= BT_MAX_ITEMS) {
+              nm_store_groups(G, L, split.rec + (size_t)h * BT_REC_WORDS);
+              for (uint32_t p = lane; p < parts; p += 64u) split.items[base + p] = make_uint2(h, p);
+              long long* acc = split.acc + (size_t)h * 192u;
+              acc[lane] = 0;
+              acc[64u + lane] = 0;
+              acc[128u + lane] = 0;
+              if (lane == 0) {
+                BtHeavy hv;
+                hv.task = t;
+                hv.parts = parts;
+                hv.done = 0u;
+                hv.pad = 0u;
+                split.heavy[h] = hv;
+              }
+              deferred = true;
+              break;
+            }
+            // No room: run it here; neutralize the part slots reserved in range.
+            for (uint32_t p = lane; p < parts; p += 64u)
+              if (base + p < BT_MAX_ITEMS) split.items[base + p] = make_uint2(0xffffffffu, 0u);
           }
-          // No room: run it here; neutralize the part slots reserved in range.
-          for (uint32_t p = lane; p < parts; p += 64u)
-            if (base + p < BT_MAX_ITEMS) split.items[base + p] = make_uint2(0xffffffffu, 0u);
+          ax += G.sx;
+          ay += G.sy;
+          n += G.sn;
+          nm_group_stream(T, L, G, 0u, 1u, ax, ay, n);
+          break;
         }
+        ux += sx;
+        uy += sy;
+        un += sn;
         nm_stream(T, L, m, nranges, 0u, 1u, ax, ay, n);
       }
       if (deferred) continue;
@@ -782,9 +988,13 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
     NmTask T;
     nm_load(T, jobs, tasks, t);
     long long ax = 0, ay = 0, n = 0, ux = 0, uy = 0, un = 0;
-    uint32_t m, nranges;
-    nm_rows(T, T.cy0, L, ux, uy, un, m, nranges);
-    nm_stream(T, L, m, nranges, p, parts, ax, ay, n);
+    // the set-up the main kernel stored; part p takes group chunks p,
+    // p + parts, ...
+    const NmGroups G = nm_load_groups(L, split.rec + (size_t)h * BT_REC_WORDS);
+    ux = G.sx;
+    uy = G.sy;
+    un = G.sn;
+    nm_group_stream(T, L, G, p, parts, ax, ay, n);
     if (p == 0u) {
       ax += ux;
       ay += uy;
