@@ -173,43 +173,47 @@ def served_section(bt, workloads, n_jobs, rate_hz=200.0):
     plain_s = time.perf_counter() - t0
     gpu.close()
     log(f"served: plain FIFO makespan {plain_s:.2f} s")
-    # continuous-batching server
-    srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128)
-    clk0 = srv.clock_ms()
-    t0 = time.perf_counter()
-    pending, slat, i, done = {}, [], 0, 0
-    ticks = 0
-    while done < n_jobs:
-        now = time.perf_counter() - t0
-        while i < n_jobs and jobs[i][0] <= now:
-            ta, g, o, it = jobs[i]
-            pending[srv.submit(g, o, it)] = (ta, i)
-            i += 1
-        running, queued = srv.step()
-        ticks += 1
-        for tk in list(pending):
-            r = srv.poll(tk)
-            if r.state >= 2:
-                ta, k = pending.pop(tk)
-                slat.append((k, r.done_ms - (clk0 + ta * 1e3)))
-                srv.release(tk)
-                done += 1
-        if running == 0 and queued == 0 and i < n_jobs:
-            time.sleep(max(0.0, jobs[i][0] - (time.perf_counter() - t0)))
-    srv_s = time.perf_counter() - t0
-    srv.close()
-    log(f"served: server makespan {srv_s:.2f} s in {ticks} ticks")
-    small = [k for k, (_, g, _, _) in enumerate(jobs) if g.edge_count <= 20000]
-    return {"jobs": n_jobs, "rate_hz": rate_hz, "updates": total_updates,
-            "plain_fifo": {"updates_per_s": total_updates / plain_s, "makespan_s": plain_s,
-                           "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                           "p99_ms_le20k_edges": pct([lat[k] for k in small], 99)},
-            "server": {"updates_per_s": total_updates / srv_s, "makespan_s": srv_s,
-                       "p50_ms": pct([v for _, v in slat], 50),
-                       "p99_ms": pct([v for _, v in slat], 99),
-                       "p99_ms_le20k_edges": pct([v for k, v in slat if k in set(small)], 99),
-                       "ticks": ticks},
-            "speedup": plain_s / srv_s}
+    # continuous-batching server, both policies
+    runs = {}
+    for name, policy in (("server", bt.Server.FIFO), ("server_shortest", bt.Server.SHORTEST)):
+        srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy)
+        clk0 = srv.clock_ms()
+        t0 = time.perf_counter()
+        pending, slat, i, done = {}, [], 0, 0
+        ticks = 0
+        while done < n_jobs:
+            now = time.perf_counter() - t0
+            while i < n_jobs and jobs[i][0] <= now:
+                ta, g, o, it = jobs[i]
+                pending[srv.submit(g, o, it)] = (ta, i)
+                i += 1
+            running, queued = srv.step()
+            ticks += 1
+            for tk in list(pending):
+                r = srv.poll(tk)
+                if r.state >= 2:
+                    ta, k = pending.pop(tk)
+                    slat.append((k, r.done_ms - (clk0 + ta * 1e3)))
+                    srv.release(tk)
+                    done += 1
+            if running == 0 and queued == 0 and i < n_jobs:
+                time.sleep(max(0.0, jobs[i][0] - (time.perf_counter() - t0)))
+        runs[name] = (time.perf_counter() - t0, slat, ticks)
+        srv.close()
+        log(f"served: {name} makespan {runs[name][0]:.2f} s in {ticks} ticks")
+    small = set(k for k, (_, g, _, _) in enumerate(jobs) if g.edge_count <= 20000)
+    out = {"jobs": n_jobs, "rate_hz": rate_hz, "updates": total_updates,
+           "plain_fifo": {"updates_per_s": total_updates / plain_s, "makespan_s": plain_s,
+                          "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                          "p99_ms_le20k_edges": pct([lat[k] for k in sorted(small)], 99)}}
+    for name, (srv_s, slat, ticks) in runs.items():
+        out[name] = {"updates_per_s": total_updates / srv_s, "makespan_s": srv_s,
+                     "p50_ms": pct([v for _, v in slat], 50),
+                     "p99_ms": pct([v for _, v in slat], 99),
+                     "p50_ms_le20k_edges": pct([v for k, v in slat if k in small], 50),
+                     "p99_ms_le20k_edges": pct([v for k, v in slat if k in small], 99),
+                     "ticks": ticks, "speedup": plain_s / srv_s}
+    return out
 
 
 def main():

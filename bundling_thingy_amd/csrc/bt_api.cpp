@@ -306,6 +306,8 @@ struct ServedJob {
   bt_options opts{};
   uint32_t iterations = 0;
   uint64_t estimate = 0;  // device bytes expected before admission
+  uint64_t est_samples = 0;  // samples expected (the resample length model)
+  double last_ms = 0;     // server clock of admission or of the last iteration
   std::unique_ptr<bt::Job> job;
   std::vector<unsigned char> points;  // result records (12 B each)
   std::vector<float> density;         // normalized density at completion
@@ -315,6 +317,9 @@ struct ServedJob {
 };
 
 }  // namespace
+
+static const double kStarveMs = 1000.0;       // SHORTEST: a job waiting longer goes first
+static const uint64_t kTickSamples = 4ull << 20;  // SHORTEST: default sample-updates per tick
 
 struct bt_server {
   bt_server_options o{};
@@ -336,7 +341,8 @@ namespace {
 
 // Device bytes a job will hold, estimated before resampling from the polyline
 // lengths in texels (the count pass yields the exact figure after admission).
-uint64_t estimate_bytes(const ServedJob& s) {
+// Device bytes of a job (and, in *samples, its expected sample count).
+uint64_t estimate_bytes(const ServedJob& s, uint64_t* samples_out) {
   bt_options o = bt::resolve_options(&s.opts);
   float bbox[4] = {3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f};
   const uint32_t np = (uint32_t)(s.xy.size() / 2);
@@ -359,13 +365,19 @@ uint64_t estimate_bytes(const ServedJob& s) {
     }
     samples += 2.0;
   }
+  const uint64_t n = (uint64_t)(samples * 1.02) + 64;
+  *samples_out = n;
   uint64_t cells = 1;
   for (uint32_t i = 0; i < bt::schedule_len(o); ++i) {
     bt_iter_params q;
     bt::iter_params(o, i, 1, &q);
+    const uint32_t k = o.cell_split ? o.cell_split
+                                    : (q.radius > 0.0f ? bt::density_split(q.radius, n, q.res_x,
+                                                                           q.res_y)
+                                                       : 1u);
+    bt::iter_params(o, i, k, &q);
     cells = std::max<uint64_t>(cells, (uint64_t)q.gx * q.gy);
   }
-  const uint64_t n = (uint64_t)(samples * 1.02) + 64;
   uint64_t bytes = n * (12 * 3 + 16 + 4 + 8 + 4) + (cells + 1) * 4 + s.xy.size() * 4 +
                    s.starts.size() * 24 + 4096;
   if (o.flags & BT_FLAG_DENSITY) bytes += (uint64_t)p.density_w * p.density_h * 32;
@@ -418,7 +430,7 @@ int bt_server_submit(bt_server* srv, const float* xy, uint32_t n_points, const u
   s->opts = bt::resolve_options(opts ? opts : &srv->o.defaults);
   s->iterations = iterations;
   s->submit_ms = srv->now();
-  s->estimate = estimate_bytes(*s);
+  s->estimate = estimate_bytes(*s, &s->est_samples);
   if (s->estimate == 0 || s->estimate > srv->o.memory_budget) {
     srv->err = "job footprint exceeds the server memory budget";
     return BT_E_CAPACITY;
@@ -434,7 +446,18 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
   srv->err.clear();
   return guarded(&srv->err, [&] {
     bt::Engine& e = *srv->engine;
-    // Admission: FIFO while the footprint fits the budget and the batch cap.
+    const bool shortest = srv->o.policy == BT_POLICY_SHORTEST;
+    auto remaining = [](const ServedJob* j) -> uint64_t {
+      const uint64_t n = j->job ? j->job->n : j->est_samples;
+      const uint32_t done = j->job ? j->job->iteration : 0u;
+      return n * (uint64_t)(j->iterations - std::min(done, j->iterations));
+    };
+    if (shortest)  // least work first; arrival order among equals
+      std::stable_sort(srv->queue.begin(), srv->queue.end(),
+                       [&](const ServedJob* a, const ServedJob* b) {
+                         return remaining(a) < remaining(b);
+                       });
+    // Admission while the footprint fits the budget and the in-flight cap.
     std::vector<ServedJob*> admit;
     uint64_t used = srv->used;
     while (!srv->queue.empty() && srv->running.size() + admit.size() < srv->o.max_inflight) {
@@ -476,17 +499,39 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
         s->job = std::move(jobs[i]);
         s->state = 1;
         s->admit_ms = t;
+        s->last_ms = t;
         s->samples = s->job->n;
         s->estimate = std::max<uint64_t>(s->estimate, s->job->device_bytes());
         srv->used += s->estimate;
         srv->running.push_back(s);
       }
     }
-    // One iteration of every running job in one batched launch per stage.
+    // One iteration of the advanced jobs in one batched launch per stage.
     if (!srv->running.empty()) {
+      std::vector<ServedJob*> adv;
+      if (shortest) {
+        const double now = srv->now();
+        std::vector<ServedJob*> order(srv->running);
+        std::stable_sort(order.begin(), order.end(), [&](const ServedJob* a, const ServedJob* b) {
+          const bool sa = now - a->last_ms > kStarveMs, sb = now - b->last_ms > kStarveMs;
+          if (sa != sb) return sa;
+          return remaining(a) < remaining(b);
+        });
+        const uint64_t budget = srv->o.tick_samples ? srv->o.tick_samples : kTickSamples;
+        uint64_t taken = 0;
+        for (ServedJob* s : order) {
+          if (!adv.empty() && taken + s->job->n > budget) continue;
+          adv.push_back(s);
+          taken += s->job->n;
+        }
+      } else {
+        adv = srv->running;
+      }
       std::vector<bt::Job*> js;
-      for (ServedJob* s : srv->running) js.push_back(s->job.get());
+      for (ServedJob* s : adv) js.push_back(s->job.get());
       e.iterate_jobs(js);
+      const double t = srv->now();
+      for (ServedJob* s : adv) s->last_ms = t;
       std::vector<ServedJob*> still;
       for (ServedJob* s : srv->running) {
         if (s->job->iteration < s->iterations) {

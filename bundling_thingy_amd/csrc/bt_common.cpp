@@ -84,6 +84,14 @@ uint32_t default_split(float radius) {
   return (uint32_t)std::min(8.0f, std::max(1.0f, k));
 }
 
+uint32_t density_split(float radius, uint64_t samples, float res_x, float res_y) {
+  const double area = ((double)res_x + 1.0) * ((double)res_y + 1.0);
+  const double rho = std::max((double)samples / area, 1e-3);
+  const double h = std::max(2.9 / std::sqrt(rho), 0.25);
+  const double k = std::nearbyint((double)radius / h);
+  return (uint32_t)std::min<double>((double)kMaxSplit, std::max(1.0, k));
+}
+
 void iter_params(const bt_options& o, uint32_t it, uint32_t split, bt_iter_params* p) {
   float sigma;
   if (o.spacing > 0.0f) {

@@ -45,6 +45,13 @@ bool has_radius(const bt_options& o);
 // most 8 per radius (the autotuner refines it per iteration).
 uint32_t default_split(float radius);
 
+// Grid rows per radius from the sample density once the sample count is
+// known: rows about 2.9 / sqrt(samples per texel^2) texels high (the set-up
+// cost per task grows with the rows of its window, the tested band with the
+// row height times the density), 1 to kMaxSplit rows per radius.
+uint32_t density_split(float radius, uint64_t samples, float res_x, float res_y);
+constexpr uint32_t kMaxSplit = 28;  // windows stay within 64 rows
+
 // Bins per row height along x when bt_options::x_split is 0.
 constexpr uint32_t kDefaultXSplit = 8;
 // Upper bound on one job's grid bins (4 bytes each); x_split is halved until

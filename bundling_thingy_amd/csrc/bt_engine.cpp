@@ -389,6 +389,11 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
     if (n > 0x7fffffffull) fail(BT_E_CAPACITY, "sample count exceeds 2^31");
     j.n = (uint32_t)n;
     j.valid_paths = (uint32_t)valid;
+    if (j.o.cell_split == 0)  // no split given: rows sized for the sample density
+      for (uint32_t e = 0; e < j.sched; ++e)
+        if (j.params[e].radius > 0.0f)
+          j.set_split(e, density_split(j.params[e].radius, n, j.params[e].res_x,
+                                       j.params[e].res_y));
     j.rec0.alloc((size_t)n * 12);
     j.rec1.alloc((size_t)n * 12);
     j.rec_saved.alloc((size_t)n * 12);
@@ -400,7 +405,8 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
     j.pre_x.alloc(((size_t)n + 1) * 8);
     j.pre_y.alloc(((size_t)n + 1) * 8);
     if (n + 1 > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS) fail(BT_E_CAPACITY, "too many samples");
-    const size_t nb = std::max<size_t>(j.bsum.bytes / 8, cdiv(n + 1, BT_SCAN_ITEMS));
+    const size_t nb = std::max<size_t>(std::max<size_t>(j.bsum.bytes / 8, cdiv(n + 1, BT_SCAN_ITEMS)),
+                                       cdiv(j.max_cells + 1ull, BT_SCAN_ITEMS));
     if (nb * 8 > j.bsum.bytes) j.bsum.alloc(nb * 8);
     if (j.o.flags & BT_FLAG_DENSITY) {
       const size_t pb = (size_t)j.dw * j.dh * 4;

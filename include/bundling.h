@@ -215,11 +215,16 @@ int bt_distortion(const float* original_xy, const float* current_xy, const uint3
 
 typedef struct bt_server bt_server;
 
+#define BT_POLICY_FIFO 0      /* admit in arrival order; every running job advances each tick */
+#define BT_POLICY_SHORTEST 1  /* admit and advance the least remaining work first, up to
+                                 tick_samples sample-updates per tick */
+
 typedef struct bt_server_options {
   bt_options defaults;       /* device, flags */
   uint64_t memory_budget;    /* device bytes the in-flight jobs may hold */
-  uint32_t max_inflight;     /* jobs per tick (continuous batch size cap) */
-  uint32_t use_graph;        /* reserved */
+  uint32_t max_inflight;     /* jobs holding device state at once */
+  uint32_t policy;           /* BT_POLICY_* */
+  uint64_t tick_samples;     /* BT_POLICY_SHORTEST: sample-updates per tick (0 = 4M) */
 } bt_server_options;
 
 typedef struct bt_job_result {
@@ -240,9 +245,13 @@ int bt_server_submit(bt_server* srv, const float* xy, uint32_t n_points, const u
                      uint32_t n_starts, const bt_options* opts, uint32_t iterations,
                      uint64_t* ticket);
 
-/* One scheduler tick: admits queued jobs that fit the memory budget (and
- * resamples them in one batched launch), advances every running job by one
- * iteration in one batched launch per stage, retires finished jobs. */
+/* One scheduler tick: admits queued jobs that fit the memory budget and the
+ * in-flight cap (and resamples them in one batched launch), advances running
+ * jobs by one iteration in one batched launch per stage, retires finished
+ * jobs. FIFO advances every running job; SHORTEST advances the jobs with the
+ * least remaining work (samples x iterations left) until tick_samples
+ * sample-updates are taken (at least one job), except that a job left
+ * waiting for more than a second goes first. */
 int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued);
 
 int bt_server_poll(bt_server* srv, uint64_t ticket, bt_job_result* out);

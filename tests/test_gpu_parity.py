@@ -72,7 +72,7 @@ def test_grid_cells_and_counts_bitexact(bt, orc, cpu_case):
         before, _ = gpu.points()
         gpu.iterate()
         cells, counts = gpu.grid()
-        ocell, ocounts = orc.grid_bin(before, orc.grid(opts.iter_params(0)))
+        ocell, ocounts = orc.grid_bin(before, orc.grid(gpu.job_params(0)))
         np.testing.assert_array_equal(cells, ocell)
         np.testing.assert_array_equal(counts.reshape(-1), ocounts)
         assert counts.sum() == len(before)
@@ -249,7 +249,10 @@ def test_error_paths(bt, cpu_case):
         assert e.value.code == 3 and "sample buffer capacity" in e.value.msg
 
 
-def test_server_results_match_alone(bt):
+@pytest.mark.parametrize("policy,tick", [(0, 0), (1, 0), (1, 1)])
+def test_server_results_match_alone(bt, policy, tick):
+    """Co-scheduled jobs give the bits of each job alone, under both policies
+    (tick 1: the shortest-first policy advances one job per tick)."""
     graphs = workloads.batched_graphs(count=4, lo=60, hi=400)
     opts = bt.Options(spacing=0.004, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=256,
                       schedule_len=6)
@@ -260,9 +263,10 @@ def test_server_results_match_alone(bt):
             gpu.upload(g)
             gpu.iterate(it)
             alone.append(gpu.points()[0])
-    srv = bt.Server(opts, memory_budget=1 << 30, max_inflight=3)
+    srv = bt.Server(opts, memory_budget=1 << 30, max_inflight=3, policy=policy,
+                    tick_samples=tick)
     tickets = [srv.submit(g, opts, it) for g, it in zip(graphs, iters)]
-    for _ in range(64):
+    for _ in range(256):
         r, q = srv.step()
         if r == 0 and q == 0:
             break
