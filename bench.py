@@ -148,6 +148,37 @@ def batched_section(bt, workloads, n_graphs, iters=30):
             "speedup": plain_s / batch_s}
 
 
+def large_section(bt, workloads, steps=2, iters=50):
+    """The scale case: 100,000 edges over 20,000 nodes (~5M samples), 50
+    iterations radius 0.02 -> 0.005, 2048^2 density, sigma 8 -> 2; autotuned
+    row splits, the schedule replayed as one hipGraph from the uploaded state."""
+    t = time.perf_counter()
+    graph = workloads.large_graph()
+    log(f"large: generated {graph.edge_count} edges in {time.perf_counter() - t:.1f} s")
+    gpu = bt.Gpu(workloads.large_options(iters))
+    t = time.perf_counter()
+    n = gpu.upload(graph)
+    upload_ms = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    chosen, _ = gpu.autotune(iters, TUNE_CANDIDATES)
+    tune_s = time.perf_counter() - t
+    gpu.rewind()
+    gpu.iterate(iters, graph=True)
+    times = []
+    for _ in range(steps):
+        gpu.rewind()
+        t = time.perf_counter()
+        gpu.iterate(iters, graph=True)
+        times.append(time.perf_counter() - t)
+    gpu.close()
+    sec = sum(times) / len(times)
+    log(f"large: {n} samples, {sec * 1e3:.1f} ms per {iters}-iteration schedule")
+    return {"edges": graph.edge_count, "samples": n, "iterations": iters,
+            "density": [2048, 2048], "updates_per_s": iters * n / sec,
+            "ms_per_schedule": sec * 1e3, "upload_ms": upload_ms, "autotune_s": tune_s,
+            "splits": chosen.tolist()}
+
+
 def served_section(bt, workloads, n_jobs, rate_hz=200.0):
     """Poisson arrivals at rate_hz: the plain per-job form (FIFO, one job at a
     time, as the drop-in does) against the continuous-batching server."""
@@ -225,6 +256,7 @@ def main():
     ap.add_argument("--cell-split", type=int, default=0)
     ap.add_argument("--batched", type=int, default=256, help="graphs in the batched case (0: skip)")
     ap.add_argument("--served", type=int, default=100, help="jobs in the served case (0: skip)")
+    ap.add_argument("--large", type=int, default=1, help="run the 100,000-edge case (0: skip)")
     ap.add_argument("--profile-only", action="store_true",
                     help="for rocprofv3 counter runs: a 256 MiB device copy (calibration), "
                          "then upload + autotune + one eager 30-iteration schedule")
@@ -332,6 +364,7 @@ def main():
     gpu.close()
     batched = batched_section(bt, workloads, args.batched) if args.batched else None
     served = served_section(bt, workloads, args.served) if args.served else None
+    large = large_section(bt, workloads) if args.large else None
 
     cpu = None
     if not args.no_cpu:
@@ -369,6 +402,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "large": large,
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]

@@ -118,6 +118,12 @@ void iter_params(const bt_options& o, uint32_t it, uint32_t split, bt_iter_param
   p->taps = blur_weights(sigma, w, BT_MAX_TAPS);
   if (p->radius > 0.0f) {
     if (split == 0) split = default_split(p->radius);
+    // fewer rows per radius while even square cells exceed kMaxGridCells
+    while (split > 1) {
+      const float c = p->radius / (float)split;
+      if ((uint64_t)grid_dim(p->res_x, c) * grid_dim(p->res_y, c) <= kMaxGridCells) break;
+      --split;
+    }
     p->cell = split == 1 ? p->radius : p->radius / (float)split;
     p->gy = grid_dim(p->res_y, p->cell);
     // Rows keep the cell height; bins along x are x_split times narrower, so
@@ -128,9 +134,10 @@ void iter_params(const bt_options& o, uint32_t it, uint32_t split, bt_iter_param
     while (a > 1 && g1 * a > kMaxGridCells) a >>= 1;
     p->cell_w = a == 1 ? p->cell : p->cell / (float)a;
     p->gx = grid_dim(p->res_x, p->cell_w);
-    // Task blocks: runs of bins about a quarter radius wide (at least one
-    // cell), cut into chunks of up to 64 samples.
-    p->task_bins = a * std::max(1u, split / 4u);
+    // Task blocks: runs of bins one radius wide, cut into chunks of up to 64
+    // consecutive samples (wider blocks fill more lanes; lane groups keep the
+    // query boxes narrow).
+    p->task_bins = a * split;
   } else {
     p->cell = p->cell_w = 0.0f;
     p->gx = p->gy = 1;

@@ -234,6 +234,36 @@ def test_full_size_properties(bt):
         assert 0 < d[0] < 0.1
 
 
+def test_large_case_split_invariance(bt):
+    """The 100,000-edge case (~5M samples, 2048^2 density): two grids (the
+    density default, and 3 rows x 2 bins per radius) give the same bits after
+    two iterations; endpoints stay fixed; the grid partitions every sample."""
+    g = workloads.large_graph()
+    base = workloads.large_options(50)
+    other = bt.Options(spacing=0.0035, radius=(0.02, 0.005), sigma=(8.0, 2.0), density=2048,
+                       schedule_len=50, cell_split=3, x_split=2)
+    res = []
+    for opts in (base, other):
+        with bt.Gpu(opts) as gpu:
+            n = gpu.upload(g)
+            layout = gpu.distortion_layout()
+            before = gpu.distortion_snapshot(layout)
+            gpu.iterate(2)
+            after = gpu.distortion_snapshot(layout)
+            ends = np.concatenate([layout.paths[layout.edges, 0],
+                                   layout.paths[layout.edges, 0] + layout.paths[layout.edges, 2]
+                                   - 1])
+            np.testing.assert_array_equal(after.positions[ends], before.positions[ends])
+            cells, counts = gpu.grid()
+            assert counts.sum() == n
+            np.testing.assert_array_equal(np.bincount(cells, minlength=counts.size),
+                                          counts.reshape(-1))
+            res.append((gpu.points()[0], gpu.density(0)))
+    assert n > 4_000_000
+    np.testing.assert_array_equal(res[0][0].view(np.uint32), res[1][0].view(np.uint32))
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+
+
 def test_error_paths(bt, cpu_case):
     with bt.Gpu(bt.Options(map_size=201)) as gpu:
         with pytest.raises(bt.BtError) as e:

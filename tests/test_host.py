@@ -86,7 +86,7 @@ def test_generalized_reduces_to_reference_mode(bt, S, r):
 @pytest.mark.parametrize("split,xs", [(1, 1), (1, 8), (4, 8), (16, 8), (6, 3), (2, 64)])
 def test_x_split_bins(bt, split, xs):
     """Rows keep the cell height; bins along x are x_split times narrower; task
-    blocks span about a quarter radius (at least one cell)."""
+    blocks span one radius."""
     o = bt.Options(spacing=0.002, radius=0.06, sigma=8.0, density=1024, cell_split=split,
                    x_split=xs)
     p = o.iter_params(0)
@@ -94,7 +94,18 @@ def test_x_split_bins(bt, split, xs):
     w = cell if xs == 1 else np.float32(cell / np.float32(xs))
     assert p.cell == cell and p.cell_w == w
     assert p.gy == math.ceil(501.0 / float(cell)) and p.gx == math.ceil(501.0 / float(w))
-    assert p.task_bins == xs * max(1, split // 4)
+    assert p.task_bins == xs * split
+
+
+def test_row_split_capped_by_grid_size(bt):
+    # 28 rows per radius of 1.43 texels on a 286-texel map would need 31M
+    # square cells: the split drops until square cells fit 2^22
+    o = bt.Options(spacing=0.0035, radius=0.005, sigma=2.0, density=2048, cell_split=28)
+    p = o.iter_params(0)
+    assert p.gx * p.gy <= 1 << 22 and p.cell == p.cell_w
+    k = round(p.radius / p.cell)
+    c = np.float32(p.radius / np.float32(k + 1))
+    assert math.ceil((p.res_x + 1) / c) * math.ceil((p.res_y + 1) / c) > 1 << 22
 
 
 def test_x_split_capped_by_grid_size(bt):
