@@ -118,6 +118,7 @@ Engine::Engine(const bt_options& o) : opts_(o) {
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device_));
   cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  nm_per_cu_ = btk::neighbor_mean_blocks_per_cu();
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   ctl_.alloc(sizeof(BtControl));
   HIPCHK(hipMemset(ctl_.p, 0, sizeof(BtControl)));
@@ -261,7 +262,7 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   split.acc = reinterpret_cast<long long*>(split_.as<char>() + kSplitAccOff);
   split.wg_cand = reinterpret_cast<unsigned long long*>(split_.as<char>() + kSplitCandOff);
   split.rec = reinterpret_cast<uint32_t*>(split_.as<char>() + kSplitRecOff);
-  btk::neighbor_mean(stream_, (uint32_t)cus_ * 4u, J, tasks_.as<BtTask>(), ctl, split);
+  btk::neighbor_mean(stream_, (uint32_t)cus_ * nm_per_cu_, J, tasks_.as<BtTask>(), ctl, split);
   mark(6);
   if (density && p.grid[BT_K_ROW]) {
     btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj);

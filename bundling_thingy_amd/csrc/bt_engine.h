@@ -104,6 +104,7 @@ class Engine {
   bt_options opts_;
   int device_ = 0;
   int cus_ = 256;
+  uint32_t nm_per_cu_ = 4;  // resident neighbor-mean workgroups per CU
   hipStream_t stream_ = nullptr;
   std::vector<std::unique_ptr<Job>> jobs_;
   DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_;

@@ -14,6 +14,8 @@
 // fixed-point int64 sums, so results do not depend on visiting order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bt_device.h"
 
 #define LANE() (threadIdx.x & 63u)
@@ -454,17 +456,19 @@ __device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total
 }
 
 #define NM_WAVES 4          // waves per workgroup
-#define NM_RANGES 128       // ranges per row group (two per row)
-#define NM_GROUPS 4         // lane groups of the grouped stream (16 lanes each)
-#define NM_GLANES 16
+#define NM_ROWS 64          // window rows set up per pass (one per lane)
+#define NM_RANGES (2 * NM_ROWS)  // ranges per row pass (two per row)
+#define NM_GROUPS 4         // lane groups of the grouped stream
+#define NM_GLANES 16        // lanes per group
 #define NM_GCHUNK 64        // candidates per group per chunk of the grouped stream
 #define NM_GSTRIDE 65       // float4 per lane group in a tile (one of padding: no bank conflicts)
+#define NM_TILE (NM_GROUPS * NM_GSTRIDE > 128 ? NM_GROUPS * NM_GSTRIDE : 128)
 #define NM_GROUP_MIN 2048u  // candidates above which a task streams per lane group
 
 // The tile holds either two chunks of 64 candidates (plain stream, double
 // buffered) or one chunk of 64 candidates per lane group (grouped stream).
 struct NmLds {
-  float4 tile[NM_WAVES][NM_GROUPS * NM_GSTRIDE];
+  float4 tile[NM_WAVES][NM_TILE];
   uint32_t start[NM_WAVES][NM_GROUPS][NM_RANGES];
   uint32_t pref[NM_WAVES][NM_GROUPS][NM_RANGES + 1];
 };
@@ -513,7 +517,7 @@ __device__ __forceinline__ void nm_load(NmTask& T, const BtJob* jobs, const BtTa
   T.cy1 = uni((uint32_t)cell_of_fast(T.ty1 + rr, J.inv_cell, J.gy));
 }
 
-// Sets up rows [rb, min(rb + 64, cy1 + 1)) of the window of the query box
+// Sets up rows [rb, min(rb + NM_ROWS, cy1 + 1)) of the window of the query box
 // [tx0, tx1] x [ty0, ty1] (texels, wave-uniform): the boundary ranges go to
 // start/pref (LDS), the inside-run sums to (sx, sy, sn) (wave-uniform).
 // Returns the number of candidates m and of ranges.
@@ -529,7 +533,7 @@ __device__ __forceinline__ void nm_rows(const BtJob& J, float tx0, float tx1, fl
   const uint32_t cy = rb + lane;
   uint32_t r0b = 0, r0e = 0, r1b = 0, r1e = 0;
   long long lx = 0, ly = 0, ln = 0;
-  if (cy <= cy1) {
+  if (lane < NM_ROWS && cy <= cy1) {
     const float Y0 = (float)cy * c, Y1 = Y0 + c;
     const float dyn = fmaxf(fmaxf(Y0 - eps - ty1, ty0 - (Y1 + eps)), 0.0f);
     if (dyn < rr) {
@@ -678,7 +682,7 @@ __device__ __forceinline__ float group_max(float v) {
 }
 
 // Grouped streams of a task whose window is one row group: the wave's lanes
-// form NM_GROUPS groups of 16 consecutive samples (neighbouring along x),
+// form NM_GROUPS groups of NM_GLANES consecutive samples (neighbours along x),
 // each with its own query box, inside sums and boundary ranges, which are
 // narrower than the whole task's. Per step, each group's lanes test the next
 // candidate of their group's stream (exhausted streams read candidates at
@@ -705,7 +709,7 @@ __device__ __forceinline__ NmGroups nm_group_rows(const NmTask& T, NmLds& L) {
 #pragma unroll
   for (uint32_t g = 0; g < NM_GROUPS; ++g) {
     const uint32_t l = g * NM_GLANES;
-    if (!(movable & (0xffffull << l))) continue;  // no movable sample in the group
+    if (!(movable & (((1ull << NM_GLANES) - 1ull) << l))) continue;  // no movable sample
     const float tx0 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gx0, (int)l, 64), J.res_x))));
     const float tx1 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gx1, (int)l, 64), J.res_x))));
     const float ty0 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gy0, (int)l, 64), J.res_y))));
@@ -888,13 +892,13 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
     long long ax = 0, ay = 0, n = 0, ux = 0, uy = 0, un = 0;
     if (ballot(T.movable)) {
       bool deferred = false;
-      for (uint32_t rb = T.cy0; rb <= T.cy1; rb += 64u) {
+      for (uint32_t rb = T.cy0; rb <= T.cy1; rb += NM_ROWS) {
         uint32_t m, nranges;
         long long sx, sy, sn;
         nm_rows(*T.J, T.tx0, T.tx1, T.ty0, T.ty1, T.cy1, rb, L.start[w][0], L.pref[w][0], sx, sy,
                 sn, m, nranges);
         cand += m;
-        const bool one_group = rb == T.cy0 && T.cy1 - T.cy0 < 64u;
+        const bool one_group = rb == T.cy0 && T.cy1 - T.cy0 < NM_ROWS;
         if (one_group && m > NM_GROUP_MIN) {
           // grouped streams; a long one is recorded for the split kernel
           const NmGroups G = nm_group_rows(T, L);  // its own ranges and inside sums
@@ -1390,6 +1394,15 @@ void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base
 }
 void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_scatter, L256(grid), jobs, base, nj);
+}
+uint32_t neighbor_mean_blocks_per_cu() {
+  int a = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, bt_neighbor_mean, 256, 0) != hipSuccess)
+    a = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bt_neighbor_mean_heavy, 256, 0) !=
+      hipSuccess)
+    b = 1;
+  return (uint32_t)std::max(1, std::min(a, b));
 }
 void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
                    BtControl* ctl, BtSplit split) {

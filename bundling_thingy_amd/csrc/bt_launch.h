@@ -15,6 +15,8 @@ void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* ba
 void pre_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
                    BtControl* ctl, BtSplit split);
+// Workgroups of the (persistent) neighbor-mean kernels resident per CU.
+uint32_t neighbor_mean_blocks_per_cu();
 void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
