@@ -153,8 +153,13 @@ extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restri
 // -------------------------------------------------------------------- bin
 // spatial_grid.wgsl:72-81 (count_cells) fused with the rank half of
 // scatter_points (:127-138): each sample gets its cell and its rank among the
-// cell's samples from a wave-aggregated atomic add. Ranks depend on scheduling,
-// which is harmless: every consumer is order-independent.
+// cell's samples. The input is in the previous iteration's sorted order, so
+// equal cells mostly sit in runs of adjacent lanes: the first lane of each
+// run adds the run's length to the cell's counter (one atomic instruction
+// for the whole wave, no serial loop) and the run's lanes take consecutive
+// ranks from the returned base. A cell split over several runs gets one
+// disjoint range per run. Ranks depend on scheduling, which is harmless:
+// every consumer is order-independent.
 extern "C" __global__ __launch_bounds__(256) void bt_bin(const BtJob* __restrict__ jobs,
                                                          const uint32_t* __restrict__ base,
                                                          int njobs) {
@@ -162,29 +167,28 @@ extern "C" __global__ __launch_bounds__(256) void bt_bin(const BtJob* __restrict
   const uint32_t i = lb * BT_BIN_BLOCK + threadIdx.x;
   const bool valid = i < J.n;
   const float* rec = J.rec[J.parity];
-  uint32_t c = 0;
+  uint32_t c = 0xffffffffu;
   if (valid) {
     float x = rec[3u * i], y = rec[3u * i + 1u];
     uint32_t cx = cell_axis(texel(x, J.res_x), J.cell_w, J.gx);
     uint32_t cy = cell_axis(texel(y, J.res_y), J.cell, J.gy);
     c = cy * J.gx + cx;
   }
-  uint32_t* cells = J.cells;
-  uint64_t active = ballot(valid);
-  uint32_t rank = 0;
   const uint32_t lane = LANE();
-  while (active) {
-    const int leader = __builtin_ctzll(active);
-    const uint32_t key = __builtin_amdgcn_readlane(c, leader);
-    const bool mine = valid && c == key;
-    const uint64_t same = ballot(mine);
-    uint32_t b = 0;
-    if (lane == (uint32_t)leader) b = atomicAdd(&cells[key], (uint32_t)__popcll(same));
-    b = __builtin_amdgcn_readlane(b, leader);
-    if (mine) rank = b + mbcnt64(same);
-    active &= ~same;
-  }
-  if (valid) J.cellrank[i] = make_uint2(c, rank);
+  const uint32_t prev = __shfl(c, (int)(lane == 0u ? 0u : lane - 1u), 64);
+  const bool leader = valid && (lane == 0u || prev != c);
+  const uint64_t leaders = ballot(leader);
+  // run of the lane: from its leader (highest leader <= lane) to the next
+  // leader or the end of the valid lanes (invalid lanes are a trailing block)
+  const uint64_t upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+  const uint32_t head = 63u - (uint32_t)__builtin_clzll(leaders & upto | 1ull);
+  const uint64_t after = leaders & ~upto;
+  const uint32_t n_valid = (uint32_t)__popcll(ballot(valid));
+  const uint32_t next = after ? (uint32_t)__builtin_ctzll(after) : n_valid;
+  uint32_t b = 0;
+  if (leader) b = atomicAdd(&J.cells[c], next - lane);
+  b = __shfl(b, (int)head, 64);
+  if (valid) J.cellrank[i] = make_uint2(c, b + (lane - head));
 }
 
 // -------------------------------------------------------------- cell scan

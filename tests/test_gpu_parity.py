@@ -107,11 +107,12 @@ def test_gpu_case_resample_and_last_radius(bt, orc):
 
 
 @pytest.mark.parametrize("split,xs", [(1, 1), (1, 8), (2, 8), (3, 2), (5, 8), (8, 4), (16, 8),
-                                      (16, 16)])
+                                      (16, 16), (1, 64), (40, 2)])
 def test_cell_split_does_not_change_results(bt, orc, cpu_case, split, xs):
     """Any grid row height and bin width give the same bits (inside runs +
     scanned borders equal the all-pairs test), including the reference grid
-    (square cells of side r: split 1, x_split 1)."""
+    (square cells of side r: split 1, x_split 1) and windows taller than one
+    row pass (split 40: 82 rows, set up in two passes, plain streams)."""
     base = workloads.plain_cpu_options()
     opts = bt.Options(spacing=0.005, radius=0.05, sigma=4.0, density=512, schedule_len=10,
                       cell_split=split, x_split=xs)
