@@ -308,6 +308,11 @@ struct ServedJob {
   uint64_t estimate = 0;  // device bytes expected before admission
   uint64_t est_samples = 0;  // samples expected (the resample length model)
   double last_ms = 0;     // server clock of admission or of the last iteration
+  // retirement in flight: results copied into a pinned buffer, `done` marks
+  // the end of the copies
+  hipEvent_t done = nullptr;
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
   std::unique_ptr<bt::Job> job;
   std::vector<unsigned char> points;  // result records (12 B each)
   std::vector<float> density;         // normalized density at completion
@@ -319,6 +324,7 @@ struct ServedJob {
 }  // namespace
 
 static const double kStarveMs = 1000.0;       // SHORTEST: a job waiting longer goes first
+static const size_t kTicksAhead = 2;          // ticks queued on the GPU at most
 static const uint64_t kTickSamples = 4ull << 20;  // SHORTEST: default sample-updates per tick
 
 struct bt_server {
@@ -329,6 +335,9 @@ struct bt_server {
   uint64_t used = 0;
   std::deque<ServedJob*> queue;
   std::vector<ServedJob*> running;
+  std::vector<ServedJob*> finishing;          // results being copied out
+  std::multimap<size_t, void*> pinned_free;   // pinned result buffers for reuse
+  std::deque<hipEvent_t> ticks;               // ends of the ticks still on the GPU
   std::map<uint64_t, std::unique_ptr<ServedJob>> all;
   std::string err;
   double now() const {
@@ -384,6 +393,50 @@ uint64_t estimate_bytes(const ServedJob& s, uint64_t* samples_out) {
   return bytes;
 }
 
+// Finishes retirements whose copies are done (all of them if `block`):
+// results move to the job's host vectors, device state and pinned buffers
+// are released.
+void reap(bt_server* srv, bool block) {
+  std::vector<ServedJob*> still;
+  for (ServedJob* s : srv->finishing) {
+    const hipError_t q = block ? hipEventSynchronize(s->done) : hipEventQuery(s->done);
+    if (q == hipErrorNotReady) {
+      still.push_back(s);
+      continue;
+    }
+    if (q != hipSuccess) bt::fail(BT_E_DEVICE, std::string("HIP: ") + hipGetErrorString(q));
+    const size_t pb = (size_t)s->samples * 12;
+    s->points.assign((const unsigned char*)s->pinned, (const unsigned char*)s->pinned + pb);
+    if (!s->density.empty())
+      std::memcpy(s->density.data(), (const unsigned char*)s->pinned + pb, s->density.size() * 4);
+    srv->pinned_free.emplace(s->pinned_bytes, s->pinned);
+    s->pinned = nullptr;
+    (void)hipEventDestroy(s->done);
+    s->done = nullptr;
+    s->job.reset();
+    srv->used -= std::min(srv->used, s->estimate);
+    s->state = 2;
+    s->done_ms = srv->now();
+  }
+  srv->finishing.swap(still);
+}
+
+// A pinned host buffer of at least `bytes` (reused when one is free).
+void* get_pinned(bt_server* srv, size_t bytes, size_t* got) {
+  auto it = srv->pinned_free.lower_bound(bytes);
+  if (it != srv->pinned_free.end() && it->first <= 2 * bytes + (1 << 20)) {
+    void* p = it->second;
+    *got = it->first;
+    srv->pinned_free.erase(it);
+    return p;
+  }
+  void* p = nullptr;
+  const size_t b = (bytes + (1 << 20) - 1) / (1 << 20) * (1 << 20);
+  HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
+  *got = b;
+  return p;
+}
+
 }  // namespace
 
 extern "C" {
@@ -408,6 +461,13 @@ int bt_server_create(const bt_server_options* opts, bt_server** out) {
 void bt_server_destroy(bt_server* srv) {
   if (!srv) return;
   try {
+    if (srv->engine) srv->engine->sync();
+    for (hipEvent_t ev : srv->ticks) (void)hipEventDestroy(ev);
+    for (ServedJob* s : srv->finishing) {
+      if (s->done) (void)hipEventDestroy(s->done);
+      if (s->pinned) (void)hipHostFree(s->pinned);
+    }
+    for (auto& kv : srv->pinned_free) (void)hipHostFree(kv.second);
     srv->all.clear();
     srv->engine.reset();
   } catch (...) {
@@ -446,6 +506,17 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
   srv->err.clear();
   return guarded(&srv->err, [&] {
     bt::Engine& e = *srv->engine;
+    // at most kTicksAhead ticks queued on the GPU: the host prepares the next
+    // tick while the previous one runs, and a loop of steps keeps pace
+    while (srv->ticks.size() >= kTicksAhead) {
+      const hipEvent_t ev = srv->ticks.front();
+      srv->ticks.pop_front();
+      const hipError_t q = hipEventSynchronize(ev);
+      (void)hipEventDestroy(ev);
+      HIPCHK(q);
+    }
+    // nothing left to launch: wait for the results being copied out
+    reap(srv, srv->running.empty() && srv->queue.empty());
     const bool shortest = srv->o.policy == BT_POLICY_SHORTEST;
     auto remaining = [](const ServedJob* j) -> uint64_t {
       const uint64_t n = j->job ? j->job->n : j->est_samples;
@@ -529,7 +600,13 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
       }
       std::vector<bt::Job*> js;
       for (ServedJob* s : adv) js.push_back(s->job.get());
-      e.iterate_jobs(js);
+      e.iterate_jobs(js, false);  // queued; the next tick's host work overlaps it
+      hipEvent_t ev = nullptr;
+      HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      const hipError_t rq = hipEventRecord(ev, e.stream());
+      if (rq != hipSuccess) (void)hipEventDestroy(ev);
+      HIPCHK(rq);
+      srv->ticks.push_back(ev);
       const double t = srv->now();
       for (ServedJob* s : adv) s->last_ms = t;
       std::vector<ServedJob*> still;
@@ -538,22 +615,21 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
           still.push_back(s);
           continue;
         }
-        s->points.resize((size_t)s->job->n * 12);
-        bt::Engine::read_points_of(e, *s->job, s->points.data(), s->job->n);
-        if (s->job->dens_cnt.p) {
-          s->dw = s->job->dw;
-          s->dh = s->job->dh;
-          s->density.resize((size_t)s->dw * s->dh);
-          bt::Engine::read_density_of(e, *s->job, 3, s->density.data(), s->density.size());
-        }
-        srv->used -= std::min(srv->used, s->estimate);
-        s->job.reset();
-        s->state = 2;
-        s->done_ms = srv->now();
+        // retire: queue the copies of the results; reap() completes them
+        const bool dens = s->job->dens_cnt.p != nullptr;
+        s->dw = dens ? s->job->dw : 0;
+        s->dh = dens ? s->job->dh : 0;
+        s->density.assign((size_t)s->dw * s->dh, 0.0f);
+        const size_t pb = (size_t)s->samples * 12;
+        s->pinned = get_pinned(srv, pb + s->density.size() * 4, &s->pinned_bytes);
+        HIPCHK(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
+        e.queue_readback(*s->job, s->pinned,
+                         dens ? (float*)((unsigned char*)s->pinned + pb) : nullptr, s->done);
+        srv->finishing.push_back(s);
       }
       srv->running.swap(still);
     }
-    if (running) *running = (uint32_t)srv->running.size();
+    if (running) *running = (uint32_t)(srv->running.size() + srv->finishing.size());
     if (queued) *queued = (uint32_t)srv->queue.size();
   });
 }
@@ -562,6 +638,10 @@ int bt_server_poll(bt_server* srv, uint64_t ticket, bt_job_result* out) {
   if (!srv || !out) return BT_E_INVALID;
   auto it = srv->all.find(ticket);
   if (it == srv->all.end()) return BT_E_INVALID;
+  if (it->second->done) {
+    int rc = guarded(&srv->err, [&] { reap(srv, false); });
+    if (rc != BT_OK) return rc;
+  }
   const ServedJob& s = *it->second;
   out->ticket = s.ticket;
   out->state = s.state;

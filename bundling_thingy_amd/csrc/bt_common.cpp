@@ -19,9 +19,56 @@ void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) fail(BT_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+DevPool::~DevPool() {
+  for (auto& kv : free_) (void)hipFree(kv.second);
+}
+
+void* DevPool::get(size_t bytes, size_t* cap) {
+  // sizes round up to a power of two below 1 MiB, to 1 MiB multiples above
+  size_t c = 256;
+  if (bytes <= (1u << 20)) {
+    while (c < bytes) c <<= 1;
+  } else {
+    c = (bytes + (1u << 20) - 1) >> 20 << 20;
+  }
+  auto it = free_.lower_bound(c);
+  if (it != free_.end() && it->first <= c + c / 4) {
+    void* q = it->second;
+    *cap = it->first;
+    cached_ -= it->first;
+    free_.erase(it);
+    return q;
+  }
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, c);
+  if (e != hipSuccess && !free_.empty()) {
+    // out of memory: return the cache to HIP and try once more
+    (void)hipGetLastError();
+    for (auto& kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+    cached_ = 0;
+    e = hipMalloc(&q, c);
+  }
+  HIPCHK(e);
+  *cap = c;
+  return q;
+}
+
+void DevPool::put(void* p, size_t cap) {
+  free_.emplace(cap, p);
+  cached_ += cap;
+}
+
 void DevBuf::alloc(size_t b) {
   release();
-  if (b) HIPCHK(hipMalloc(&p, b));
+  if (b) {
+    if (pool) {
+      p = pool->get(b, &cap);
+    } else {
+      HIPCHK(hipMalloc(&p, b));
+      cap = b;
+    }
+  }
   bytes = b;
 }
 
@@ -32,9 +79,13 @@ bool DevBuf::reserve(size_t b) {
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p) {
+    if (pool) pool->put(p, cap);
+    else (void)hipFree(p);
+  }
   p = nullptr;
   bytes = 0;
+  cap = 0;
 }
 
 bt_options resolve_options(const bt_options* in) {

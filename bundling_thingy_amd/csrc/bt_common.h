@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -20,10 +21,31 @@ struct Error {
 void hip_check(hipError_t e, const char* what);
 #define HIPCHK(x) ::bt::hip_check((x), #x)
 
-// Owning device allocation.
+// Per-engine cache of device blocks: a freed block is kept and handed out
+// again (same stream, so reuse is ordered after every earlier use) instead of
+// going back through hipFree, which waits for the device. Blocks return to
+// HIP when the pool is destroyed.
+class DevPool {
+ public:
+  DevPool() = default;
+  DevPool(const DevPool&) = delete;
+  DevPool& operator=(const DevPool&) = delete;
+  ~DevPool();
+  void* get(size_t bytes, size_t* cap);
+  void put(void* p, size_t cap);
+  size_t cached() const { return cached_; }
+
+ private:
+  std::multimap<size_t, void*> free_;
+  size_t cached_ = 0;
+};
+
+// Owning device allocation (from `pool` when set, else hipMalloc).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  size_t cap = 0;
+  DevPool* pool = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;

@@ -19,6 +19,11 @@ static inline size_t align_up(size_t a, size_t b) { return (a + b - 1) / b * b; 
 static const uint32_t kWt = BT_MAX_TAPS + 1;  // weights per iteration row
 
 // Layout of the heavy-task split scratch (one per engine).
+// Pinned host slots for plan uploads, and device slots for one-job readback plans.
+static const uint32_t kPinnedSlots = 4;
+static const uint32_t kRetireSlots = 64;
+static const size_t kRetireSlotBytes = 2048;
+
 static const size_t kSplitItemsOff = (size_t)BT_MAX_HEAVY * sizeof(BtHeavy);
 static const size_t kSplitAccOff = kSplitItemsOff + (size_t)BT_MAX_ITEMS * sizeof(uint2);
 static const size_t kSplitCandOff = kSplitAccOff + (size_t)BT_MAX_HEAVY * 192 * sizeof(long long);
@@ -102,6 +107,14 @@ bool Job::set_split(uint32_t e, uint32_t k) {
   return false;
 }
 
+void Job::set_pool(DevPool* pool) {
+  DevBuf* all[] = {&ctrl, &starts, &path_scan, &paths, &bsum, &status, &rec0,
+                   &rec1, &rec_saved, &sorted, &sid, &cellrank, &cells, &edge_of,
+                   &pre_x, &pre_y, &dens_cnt, &dens_row, &dens_col, &dens_norm, &dens_saved,
+                   &peak, &wt};
+  for (DevBuf* b : all) b->pool = pool;
+}
+
 size_t Job::device_bytes() const {
   const DevBuf* all[] = {&ctrl, &starts, &path_scan, &paths, &bsum, &status, &rec0,
                          &rec1, &rec_saved, &sorted, &sid, &cellrank, &cells, &edge_of,
@@ -112,7 +125,7 @@ size_t Job::device_bytes() const {
   return s;
 }
 
-Engine::Engine(const bt_options& o) : opts_(o) {
+Engine::Engine(const bt_options& o) : opts_(o), pool_(new DevPool) {
   device_ = o.device;
   HIPCHK(hipSetDevice(device_));
   hipDeviceProp_t prop;
@@ -124,6 +137,9 @@ Engine::Engine(const bt_options& o) : opts_(o) {
   HIPCHK(hipMemset(ctl_.p, 0, sizeof(BtControl)));
   split_.alloc(kSplitBytes);
   HIPCHK(hipMemset(split_.p, 0, kSplitBytes));
+  pinned_.resize(kPinnedSlots);
+  for (Pinned& s : pinned_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+  retire_plans_.alloc(kRetireSlots * kRetireSlotBytes);
 }
 
 Engine::~Engine() {
@@ -131,6 +147,10 @@ Engine::~Engine() {
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
+  }
+  for (Pinned& s : pinned_) {
+    if (s.ev) (void)hipEventDestroy(s.ev);
+    if (s.p) (void)hipHostFree(s.p);
   }
 }
 
@@ -218,7 +238,42 @@ Plan Engine::make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>&
 }
 
 void Engine::put_plan(const Plan& p, void* dst) {
-  HIPCHK(hipMemcpyAsync(dst, p.host.data(), p.host.size(), hipMemcpyHostToDevice, stream_));
+  Pinned& s = pinned_[pinned_next_];
+  pinned_next_ = (pinned_next_ + 1u) % (uint32_t)pinned_.size();
+  if (s.pending) {
+    HIPCHK(hipEventSynchronize(s.ev));
+    s.pending = false;
+  }
+  if (s.bytes < p.host.size()) {
+    if (s.p) HIPCHK(hipHostFree(s.p));
+    s.p = nullptr;
+    s.bytes = 0;
+    const size_t b = align_up(std::max<size_t>(p.host.size(), 64 << 10), 64 << 10);
+    HIPCHK(hipHostMalloc(&s.p, b, hipHostMallocDefault));
+    s.bytes = b;
+  }
+  std::memcpy(s.p, p.host.data(), p.host.size());
+  HIPCHK(hipMemcpyAsync(dst, s.p, p.host.size(), hipMemcpyHostToDevice, stream_));
+  HIPCHK(hipEventRecord(s.ev, stream_));
+  s.pending = true;
+}
+
+void Engine::queue_readback(Job& j, void* rec12, float* density, hipEvent_t done) {
+  std::vector<Job*> js = {&j};
+  Plan p = make_plan(js, {j.iteration});
+  if (p.host.size() > kRetireSlotBytes) fail(BT_E_DEVICE, "readback plan exceeds its slot");
+  unsigned char* dev = retire_plans_.as<unsigned char>() + (size_t)retire_next_ * kRetireSlotBytes;
+  retire_next_ = (retire_next_ + 1u) % kRetireSlots;
+  put_plan(p, dev);
+  btk::unpermute(stream_, p.grid[BT_K_BIN], reinterpret_cast<const BtJob*>(dev),
+                 reinterpret_cast<const uint32_t*>(dev + p.base_off) + BT_K_BIN * 2, 1,
+                 j.rec_saved.as<float>(), nullptr, 0u);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rec12, j.rec_saved.p, (size_t)j.n * 12, hipMemcpyDeviceToHost, stream_));
+  if (density && j.dens_norm.p)
+    HIPCHK(hipMemcpyAsync(density, j.dens_norm.p, (size_t)j.dw * j.dh * 4, hipMemcpyDeviceToHost,
+                          stream_));
+  HIPCHK(hipEventRecord(done, stream_));
 }
 
 void Engine::ensure_tasks(const std::vector<Job*>& js) {
@@ -299,6 +354,7 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
   std::vector<Job*> js;
   for (const GraphIn& g : gs) {
     auto j = std::make_unique<Job>();
+    j->set_pool(pool_.get());
     j->o = resolve_options(&g.opts);
     if (!g.xy || !g.starts || g.n_starts < 2 || g.n_points < 2)
       fail(BT_E_INVALID, "graph has no drawable edges");
@@ -495,7 +551,7 @@ static void check_iterable(const std::vector<Job*>& js) {
     if (!has_radius(j->o)) fail(BT_E_STATE, "neighbor radius is not configured");
 }
 
-void Engine::iterate_jobs(const std::vector<Job*>& js) {
+void Engine::iterate_jobs(const std::vector<Job*>& js, bool wait) {
   check_iterable(js);
   ensure_tasks(js);
   std::vector<uint32_t> it;
@@ -519,7 +575,7 @@ void Engine::iterate_jobs(const std::vector<Job*>& js) {
     ++j->iteration;
     j->parity ^= 1u;
   }
-  sync();
+  if (wait) sync();
 }
 
 void Engine::iterate(uint32_t n, bool use_graph) {
@@ -530,12 +586,10 @@ void Engine::iterate(uint32_t n, bool use_graph) {
   bool density = false;
   for (Job* j : js) density |= j->dens_cnt.p != nullptr;
   if (!use_graph) {
-    std::vector<Plan> keep;
     for (uint32_t i = 0; i < n; ++i) {
       std::vector<uint32_t> it;
       for (Job* j : js) it.push_back(j->iteration);
-      keep.push_back(make_plan(js, it));
-      const Plan& p = keep.back();
+      const Plan p = make_plan(js, it);
       const size_t slot = align_up(p.host.size(), 256);
       if (2 * slot > plans_.bytes) {
         sync();
@@ -548,10 +602,6 @@ void Engine::iterate(uint32_t n, bool use_graph) {
       for (Job* j : js) {
         ++j->iteration;
         j->parity ^= 1u;
-      }
-      if (keep.size() > 4) {
-        sync();
-        keep.clear();
       }
     }
     sync();

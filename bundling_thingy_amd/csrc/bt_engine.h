@@ -33,6 +33,8 @@ struct Job {
   bool set_split(uint32_t e, uint32_t k);
   BtJob dev(uint32_t it, uint32_t parity_at) const;
   size_t device_bytes() const;
+  // Allocate this job's buffers from `pool` (before any allocation).
+  void set_pool(DevPool* pool);
 };
 
 struct Plan {
@@ -62,8 +64,14 @@ class Engine {
   // Uploads graphs as new jobs appended to `out` (used by the server).
   void upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out);
   void iterate(uint32_t n, bool use_graph);
-  // Advances an explicit job set by one iteration (server tick).
-  void iterate_jobs(const std::vector<Job*>& js);
+  // Advances an explicit job set by one iteration (server tick); with
+  // wait = false it returns once the launches are queued.
+  void iterate_jobs(const std::vector<Job*>& js, bool wait = true);
+  // Queues the readback of a job's points (12-byte records, Layout order)
+  // and, if density is on, its normalized density into host buffers (pinned,
+  // so the copies run asynchronously), then records `done` on the stream.
+  // Uses the job's rewind copy as staging: the job must not be rewound after.
+  void queue_readback(Job& j, void* rec12, float* density, hipEvent_t done);
   void rewind();
   void profile(uint32_t n, double* stage_ms, uint32_t nst, double* total_ms);
   // Picks the grid cell split of each of the first n iterations by timing
@@ -91,6 +99,8 @@ class Engine {
 
  private:
   Plan make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>& iters) const;
+  // Copies a plan to device memory through a ring of pinned host slots (the
+  // copy is asynchronous; a slot is reused once its previous copy is done).
   void put_plan(const Plan& p, void* dst);
   void launch_iteration(const Plan& p, const unsigned char* dev, bool density,
                         hipEvent_t* ev = nullptr);
@@ -102,12 +112,22 @@ class Engine {
   std::vector<Job*> all_jobs();
 
   bt_options opts_;
+  std::unique_ptr<DevPool> pool_;  // declared before the jobs: outlives their buffers
   int device_ = 0;
   int cus_ = 256;
   uint32_t nm_per_cu_ = 4;  // resident neighbor-mean workgroups per CU
   hipStream_t stream_ = nullptr;
   std::vector<std::unique_ptr<Job>> jobs_;
-  DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_;
+  DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_, retire_plans_;
+  struct Pinned {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+  };
+  std::vector<Pinned> pinned_;
+  uint32_t pinned_next_ = 0;
+  uint32_t retire_next_ = 0;
   size_t slot_bytes_ = 0;
   uint32_t eager_slot_ = 0;
   uint64_t generation_ = 0;
