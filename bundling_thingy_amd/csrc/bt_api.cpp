@@ -545,14 +545,14 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
                       (uint32_t)s->starts.size(), s->opts});
       std::vector<std::unique_ptr<bt::Job>> jobs;
       try {
-        e.upload_jobs(gs, &jobs);
+        e.upload_jobs(gs, &jobs, true, false);  // overlaps queued ticks
       } catch (const bt::Error& err) {
         // Isolate failures: retry one by one so one bad graph fails alone.
         jobs.clear();
         for (size_t i = 0; i < admit.size(); ++i) {
           std::vector<std::unique_ptr<bt::Job>> one;
           try {
-            e.upload_jobs({gs[i]}, &one);
+            e.upload_jobs({gs[i]}, &one, true, false);
             jobs.push_back(std::move(one[0]));
           } catch (const bt::Error& err1) {
             admit[i]->state = 3;

@@ -133,8 +133,11 @@ Engine::Engine(const bt_options& o) : opts_(o), pool_(new DevPool) {
   cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   nm_per_cu_ = btk::neighbor_mean_blocks_per_cu();
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
   ctl_.alloc(sizeof(BtControl));
   HIPCHK(hipMemset(ctl_.p, 0, sizeof(BtControl)));
+  ctl_side_.alloc(sizeof(BtControl));
+  HIPCHK(hipMemset(ctl_side_.p, 0, sizeof(BtControl)));
   split_.alloc(kSplitBytes);
   HIPCHK(hipMemset(split_.p, 0, kSplitBytes));
   pinned_.resize(kPinnedSlots);
@@ -147,6 +150,10 @@ Engine::~Engine() {
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
+  }
+  if (side_) {
+    (void)hipStreamSynchronize(side_);
+    (void)hipStreamDestroy(side_);
   }
   for (Pinned& s : pinned_) {
     if (s.ev) (void)hipEventDestroy(s.ev);
@@ -237,7 +244,8 @@ Plan Engine::make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>&
   return p;
 }
 
-void Engine::put_plan(const Plan& p, void* dst) {
+void Engine::put_plan(const Plan& p, void* dst, hipStream_t st) {
+  if (!st) st = stream_;
   Pinned& s = pinned_[pinned_next_];
   pinned_next_ = (pinned_next_ + 1u) % (uint32_t)pinned_.size();
   if (s.pending) {
@@ -253,8 +261,8 @@ void Engine::put_plan(const Plan& p, void* dst) {
     s.bytes = b;
   }
   std::memcpy(s.p, p.host.data(), p.host.size());
-  HIPCHK(hipMemcpyAsync(dst, s.p, p.host.size(), hipMemcpyHostToDevice, stream_));
-  HIPCHK(hipEventRecord(s.ev, stream_));
+  HIPCHK(hipMemcpyAsync(dst, s.p, p.host.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(s.ev, st));
   s.pending = true;
 }
 
@@ -349,7 +357,10 @@ void Engine::upload(const std::vector<GraphIn>& gs, std::vector<uint32_t>* sampl
   }
 }
 
-void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out) {
+void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out,
+                         bool side, bool rewindable) {
+  hipStream_t const st = side ? side_ : stream_;
+  BtControl* const ctl = side ? ctl_side_.as<BtControl>() : ctl_.as<BtControl>();
   std::vector<std::unique_ptr<Job>> nj;
   std::vector<Job*> js;
   for (const GraphIn& g : gs) {
@@ -404,11 +415,11 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
                                  cdiv(j->max_cells + 1ull, BT_SCAN_ITEMS));
     j->bsum.alloc((size_t)nb * 8);
     j->wt.alloc(wt.size() * 4);
-    HIPCHK(hipMemcpyAsync(j->ctrl.p, g.xy, (size_t)g.n_points * 8, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(j->ctrl.p, g.xy, (size_t)g.n_points * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(j->starts.p, g.starts, (size_t)g.n_starts * 4, hipMemcpyHostToDevice,
-                          stream_));
-    HIPCHK(hipMemcpyAsync(j->wt.p, wt.data(), wt.size() * 4, hipMemcpyHostToDevice, stream_));
-    HIPCHK(hipMemsetAsync(j->status.p, 0, 16, stream_));
+                          st));
+    HIPCHK(hipMemcpyAsync(j->wt.p, wt.data(), wt.size() * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(j->status.p, 0, 16, st));
     js.push_back(j.get());
     nj.push_back(std::move(j));
   }
@@ -418,23 +429,24 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
   std::vector<uint32_t> zeros(js.size(), 0);
   Plan p1 = make_plan(js, zeros);
   DevBuf table;
+  table.pool = pool_.get();
   table.alloc(p1.host.size());
-  put_plan(p1, table.p);
+  put_plan(p1, table.p, st);
   {
     const unsigned char* dev = table.as<unsigned char>();
     const BtJob* J = reinterpret_cast<const BtJob*>(dev);
     auto B = [&](int k) {
       return reinterpret_cast<const uint32_t*>(dev + p1.base_off) + (size_t)k * (p1.njobs + 1);
     };
-    btk::resample_count(stream_, p1.grid[BT_K_PATH], J, B(BT_K_PATH), (int)p1.njobs);
-    btk::path_scan(stream_, p1.grid[BT_K_PSCAN], J, B(BT_K_PSCAN), (int)p1.njobs);
+    btk::resample_count(st, p1.grid[BT_K_PATH], J, B(BT_K_PATH), (int)p1.njobs);
+    btk::path_scan(st, p1.grid[BT_K_PSCAN], J, B(BT_K_PSCAN), (int)p1.njobs);
     HIPCHK(hipGetLastError());
   }
   std::vector<uint64_t> totals(js.size());
   for (size_t i = 0; i < js.size(); ++i)
     HIPCHK(hipMemcpyAsync(&totals[i], js[i]->path_scan.as<uint64_t>() + js[i]->n_paths, 8,
-                          hipMemcpyDeviceToHost, stream_));
-  sync();
+                          hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   for (size_t i = 0; i < js.size(); ++i) {
     Job& j = *js[i];
     const uint64_t n = totals[i] >> 32, valid = totals[i] & 0xffffffffull;
@@ -471,14 +483,14 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
       j.dens_row.alloc(pb);
       j.dens_col.alloc(pb);
       j.dens_norm.alloc(pb);
-      j.dens_saved.alloc(pb * 4);
+      if (rewindable) j.dens_saved.alloc(pb * 4);
       j.peak.alloc(8);
     }
   }
   // Pass 2: emit samples (emit_paths, resampling.wgsl:161-183) with the
   // initial density splat, then the density chain (gpu.rs:512-514).
   Plan p2 = make_plan(js, zeros);
-  put_plan(p2, table.p);
+  put_plan(p2, table.p, st);
   {
     const unsigned char* dev = table.as<unsigned char>();
     const BtJob* J = reinterpret_cast<const BtJob*>(dev);
@@ -486,33 +498,33 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
     auto B = [&](int k) {
       return reinterpret_cast<const uint32_t*>(dev + p2.base_off) + (size_t)k * (n2 + 1);
     };
-    btk::clear(stream_, p2.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)n2, ctl_.as<BtControl>(), 1);
-    btk::resample_emit(stream_, p2.grid[BT_K_EMIT], J, B(BT_K_EMIT), (int)n2);
+    btk::clear(st, p2.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)n2, ctl, 1);
+    btk::resample_emit(st, p2.grid[BT_K_EMIT], J, B(BT_K_EMIT), (int)n2);
     if (p2.grid[BT_K_ROW]) {
-      btk::blur_rows(stream_, p2.grid[BT_K_ROW], J, B(BT_K_ROW), (int)n2);
-      btk::blur_cols(stream_, p2.grid[BT_K_COL], J, B(BT_K_COL), (int)n2);
-      btk::normalize(stream_, p2.grid[BT_K_NORM], J, B(BT_K_NORM), (int)n2);
+      btk::blur_rows(st, p2.grid[BT_K_ROW], J, B(BT_K_ROW), (int)n2);
+      btk::blur_cols(st, p2.grid[BT_K_COL], J, B(BT_K_COL), (int)n2);
+      btk::normalize(st, p2.grid[BT_K_NORM], J, B(BT_K_NORM), (int)n2);
     }
     HIPCHK(hipGetLastError());
   }
-  std::vector<uint32_t> st(js.size() * 4);
+  std::vector<uint32_t> status(js.size() * 4);
   for (size_t i = 0; i < js.size(); ++i)
-    HIPCHK(hipMemcpyAsync(&st[i * 4], js[i]->status.p, 16, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(&status[i * 4], js[i]->status.p, 16, hipMemcpyDeviceToHost, st));
   for (Job* j : js) {
     HIPCHK(hipMemcpyAsync(j->rec_saved.p, j->rec0.p, (size_t)j->n * 12, hipMemcpyDeviceToDevice,
-                          stream_));
-    if (j->dens_cnt.p) {
+                          st));
+    if (j->dens_cnt.p && rewindable) {
       const size_t pb = (size_t)j->dw * j->dh * 4;
       char* s = j->dens_saved.as<char>();
-      HIPCHK(hipMemcpyAsync(s, j->dens_cnt.p, pb, hipMemcpyDeviceToDevice, stream_));
-      HIPCHK(hipMemcpyAsync(s + pb, j->dens_row.p, pb, hipMemcpyDeviceToDevice, stream_));
-      HIPCHK(hipMemcpyAsync(s + 2 * pb, j->dens_col.p, pb, hipMemcpyDeviceToDevice, stream_));
-      HIPCHK(hipMemcpyAsync(s + 3 * pb, j->dens_norm.p, pb, hipMemcpyDeviceToDevice, stream_));
+      HIPCHK(hipMemcpyAsync(s, j->dens_cnt.p, pb, hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(s + pb, j->dens_row.p, pb, hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(s + 2 * pb, j->dens_col.p, pb, hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(s + 3 * pb, j->dens_norm.p, pb, hipMemcpyDeviceToDevice, st));
     }
   }
-  sync();
+  HIPCHK(hipStreamSynchronize(st));
   for (size_t i = 0; i < js.size(); ++i)
-    if (st[i * 4] & 2u) fail(BT_E_MISMATCH, "GPU resampling count and write passes disagree");
+    if (status[i * 4] & 2u) fail(BT_E_MISMATCH, "GPU resampling count and write passes disagree");
   for (auto& j : nj) {
     j->iteration = 0;
     j->parity = 0;
@@ -525,7 +537,7 @@ void Engine::rewind() {
   for (auto& j : jobs_) {
     HIPCHK(hipMemcpyAsync(j->rec0.p, j->rec_saved.p, (size_t)j->n * 12, hipMemcpyDeviceToDevice,
                           stream_));
-    if (j->dens_cnt.p) {
+    if (j->dens_cnt.p && j->dens_saved.p) {
       const size_t pb = (size_t)j->dw * j->dh * 4;
       const char* s = j->dens_saved.as<char>();
       HIPCHK(hipMemcpyAsync(j->dens_cnt.p, s, pb, hipMemcpyDeviceToDevice, stream_));
@@ -786,6 +798,7 @@ void Engine::read_points_of(Engine& e, Job& j, void* rec12, size_t n) {
   std::vector<Job*> js = {&j};
   Plan p = e.make_plan(js, {j.iteration});
   DevBuf table;
+  table.pool = e.pool_.get();
   table.alloc(p.host.size());
   e.put_plan(p, table.p);
   const unsigned char* dev = table.as<unsigned char>();
@@ -828,6 +841,7 @@ void Engine::read_grid(uint32_t job_i, uint32_t* cell_of, size_t n, uint32_t* co
   std::vector<Job*> js = {&j};
   Plan pl = make_plan(js, {j.iteration});
   DevBuf table;
+  table.pool = pool_.get();
   table.alloc(pl.host.size());
   put_plan(pl, table.p);
   const unsigned char* dev = table.as<unsigned char>();

@@ -61,8 +61,12 @@ class Engine {
   const bt_options& options() const { return opts_; }
   // Replaces the current jobs with the given graphs; returns sample counts.
   void upload(const std::vector<GraphIn>& gs, std::vector<uint32_t>* samples);
-  // Uploads graphs as new jobs appended to `out` (used by the server).
-  void upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out);
+  // Uploads graphs as new jobs appended to `out`. `side`: on the engine's
+  // second stream (with its own control block), so the upload overlaps
+  // iterations queued on the main stream (the server's admissions).
+  // `rewindable`: keep the post-upload copies bt_rewind restores.
+  void upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out,
+                   bool side = false, bool rewindable = true);
   void iterate(uint32_t n, bool use_graph);
   // Advances an explicit job set by one iteration (server tick); with
   // wait = false it returns once the launches are queued.
@@ -101,7 +105,7 @@ class Engine {
   Plan make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>& iters) const;
   // Copies a plan to device memory through a ring of pinned host slots (the
   // copy is asynchronous; a slot is reused once its previous copy is done).
-  void put_plan(const Plan& p, void* dst);
+  void put_plan(const Plan& p, void* dst, hipStream_t st = nullptr);
   void launch_iteration(const Plan& p, const unsigned char* dev, bool density,
                         hipEvent_t* ev = nullptr);
   void ensure_tasks(const std::vector<Job*>& js);
@@ -117,8 +121,9 @@ class Engine {
   int cus_ = 256;
   uint32_t nm_per_cu_ = 4;  // resident neighbor-mean workgroups per CU
   hipStream_t stream_ = nullptr;
+  hipStream_t side_ = nullptr;  // uploads that overlap queued iterations
   std::vector<std::unique_ptr<Job>> jobs_;
-  DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_, retire_plans_;
+  DevBuf ctl_, ctl_side_, tasks_, plans_, graph_plans_, staging_, staging2_, split_, retire_plans_;
   struct Pinned {
     void* p = nullptr;
     size_t bytes = 0;
