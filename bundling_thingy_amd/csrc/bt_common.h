@@ -75,7 +75,7 @@ uint32_t density_split(float radius, uint64_t samples, float res_x, float res_y)
 constexpr uint32_t kMaxSplit = 28;  // windows stay within 64 rows
 
 // Bins per row height along x when bt_options::x_split is 0.
-constexpr uint32_t kDefaultXSplit = 8;
+constexpr uint32_t kDefaultXSplit = 16;
 // Upper bound on one job's grid bins (4 bytes each); x_split is halved until
 // the grid fits.
 constexpr uint64_t kMaxGridCells = 1ull << 22;

@@ -1368,7 +1368,7 @@ extern "C" __global__ __launch_bounds__(256) void bt_unpermute(const BtJob* __re
 }
 
 // ------------------------------------------------------- launch wrappers
-// Host-callable entry points used by the C++ runtime (bt_host.cpp).
+// Host-callable entry points used by the C++ runtime (bt_engine.cpp).
 #include "bt_launch.h"
 
 namespace btk {

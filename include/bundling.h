@@ -56,7 +56,7 @@ typedef struct bt_options {
   uint32_t schedule_len;           /* generalized: iterations the schedules span */
   uint32_t density_w, density_h;   /* generalized: density size */
   uint32_t cell_split;             /* grid rows per radius; 0 = autotuned */
-  uint32_t x_split;                /* grid bins per row height along x; 0 = 8,
+  uint32_t x_split;                /* grid bins per row height along x; 0 = 16,
                                       1 = square cells as in the reference */
   uint32_t reserved[2];
 } bt_options;
