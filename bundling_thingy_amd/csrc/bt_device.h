@@ -14,7 +14,9 @@
 #define BT_TASK_POINTS 64          // query points per neighbor-mean task (one wave)
 
 // Block granularities (work items per workgroup) of the batched kernels.
-#define BT_BIN_BLOCK 256           // points per workgroup in bin/scatter
+#define BT_BIN_BLOCK 256           // points per workgroup in scatter/readback
+#define BT_RANK_CHUNKS 8           // binning: 64-sample chunks per wave
+#define BT_RANK_BLOCK (256 * BT_RANK_CHUNKS)  // points per workgroup in binning
 #define BT_SCAN_ITEMS 4096         // cells per workgroup in the cell scan (256 x 16)
 #define BT_TASK_BLOCKS 1024        // task blocks per workgroup in task emission (256 x 4)
 #define BT_CLEAR_ITEMS 4096        // words per workgroup in the clear kernel
@@ -87,6 +89,7 @@ enum BtKind {
   BT_K_EMIT,     // resample emit: BT_EMIT_PATHS paths per workgroup
   BT_K_PSCAN,    // path scan: BT_SCAN_ITEMS paths per workgroup
   BT_K_PTS,      // sample prefix scan: BT_SCAN_ITEMS samples per workgroup
+  BT_K_RANK,     // binning: BT_RANK_BLOCK samples per workgroup
   BT_K_COUNT
 };
 

@@ -206,6 +206,7 @@ static uint32_t kind_blocks(int k, const Job& j, const BtJob& d) {
   switch (k) {
     case BT_K_CLEAR: return cdiv(std::max<uint64_t>(d.n_cells + 1ull, npix), BT_CLEAR_ITEMS);
     case BT_K_BIN: return cdiv(j.n, BT_BIN_BLOCK);
+    case BT_K_RANK: return cdiv(j.n, BT_RANK_BLOCK);
     case BT_K_SCAN: return cdiv(d.n_cells + 1ull, BT_SCAN_ITEMS);
     case BT_K_TASK: return cdiv((uint64_t)d.task_bx * d.gy, BT_TASK_BLOCKS);
     case BT_K_ROW: return cdiv(d.dw, BT_ROW_TILE_W) * cdiv(d.dh, BT_ROW_TILE_H);
@@ -308,7 +309,7 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   mark(0);
   btk::clear(stream_, p.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)nj, ctl, 1);
   mark(1);
-  btk::bin(stream_, p.grid[BT_K_BIN], J, B(BT_K_BIN), (int)nj);
+  btk::bin(stream_, p.grid[BT_K_RANK], J, B(BT_K_RANK), (int)nj);
   mark(2);
   btk::cell_scan(stream_, p.grid[BT_K_SCAN], J, B(BT_K_SCAN), (int)nj);
   mark(3);

@@ -154,41 +154,76 @@ extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restri
 // spatial_grid.wgsl:72-81 (count_cells) fused with the rank half of
 // scatter_points (:127-138): each sample gets its cell and its rank among the
 // cell's samples. The input is in the previous iteration's sorted order, so
-// equal cells mostly sit in runs of adjacent lanes: the first lane of each
-// run adds the run's length to the cell's counter (one atomic instruction
-// for the whole wave, no serial loop) and the run's lanes take consecutive
-// ranks from the returned base. A cell split over several runs gets one
-// disjoint range per run. Ranks depend on scheduling, which is harmless:
-// every consumer is order-independent.
+// equal cells mostly come in runs of consecutive samples. Each wave takes
+// BT_RANK_CHUNKS chunks of 64 consecutive samples; the first sample of each
+// run (across chunk boundaries) adds the run's length to the cell counter,
+// all these atomics issued together, and the run's samples take consecutive
+// ranks from the returned base. A collapsed bundle (one cell, 10^5 samples)
+// costs one atomic per 512 samples instead of one per sample or per wave. A
+// cell split over several runs gets one disjoint range per run. Ranks depend
+// on scheduling, which is harmless: every consumer is order-independent.
 extern "C" __global__ __launch_bounds__(256) void bt_bin(const BtJob* __restrict__ jobs,
                                                          const uint32_t* __restrict__ base,
                                                          int njobs) {
-  JOB_PROLOGUE(BT_K_BIN);
-  const uint32_t i = lb * BT_BIN_BLOCK + threadIdx.x;
-  const bool valid = i < J.n;
-  const float* rec = J.rec[J.parity];
-  uint32_t c = 0xffffffffu;
-  if (valid) {
-    float x = rec[3u * i], y = rec[3u * i + 1u];
-    uint32_t cx = cell_axis(texel(x, J.res_x), J.cell_w, J.gx);
-    uint32_t cy = cell_axis(texel(y, J.res_y), J.cell, J.gy);
-    c = cy * J.gx + cx;
-  }
+  JOB_PROLOGUE(BT_K_RANK);
+  constexpr uint32_t C = BT_RANK_CHUNKS;
   const uint32_t lane = LANE();
-  const uint32_t prev = __shfl(c, (int)(lane == 0u ? 0u : lane - 1u), 64);
-  const bool leader = valid && (lane == 0u || prev != c);
-  const uint64_t leaders = ballot(leader);
-  // run of the lane: from its leader (highest leader <= lane) to the next
-  // leader or the end of the valid lanes (invalid lanes are a trailing block)
+  const uint32_t w0 = lb * BT_RANK_BLOCK + (threadIdx.x >> 6) * (C * 64u);
+  const uint32_t nv = w0 < J.n ? min(C * 64u, J.n - w0) : 0u;  // valid samples (a prefix)
+  const float* rec = J.rec[J.parity];
+  uint32_t key[C];
+  uint64_t lead[C];
+#pragma unroll
+  for (uint32_t c = 0; c < C; ++c) {
+    const uint32_t p = c * 64u + lane;
+    key[c] = 0xffffffffu;
+    if (p < nv) {
+      const uint32_t i = w0 + p;
+      const float x = rec[3u * i], y = rec[3u * i + 1u];
+      const uint32_t cx = cell_axis(texel(x, J.res_x), J.cell_w, J.gx);
+      const uint32_t cy = cell_axis(texel(y, J.res_y), J.cell, J.gy);
+      key[c] = cy * J.gx + cx;
+    }
+  }
+#pragma unroll
+  for (uint32_t c = 0; c < C; ++c) {
+    // the previous sample in sequence: lane - 1, or lane 63 of the last chunk
+    uint32_t prev = __shfl(key[c], (int)(lane == 0u ? 0u : lane - 1u), 64);
+    if (lane == 0u)
+      prev = c ? (uint32_t)__builtin_amdgcn_readlane(key[c - 1u], 63) : 0xfffffffeu;
+    lead[c] = ballot(c * 64u + lane < nv && key[c] != prev);
+  }
   const uint64_t upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
-  const uint32_t head = 63u - (uint32_t)__builtin_clzll(leaders & upto | 1ull);
-  const uint64_t after = leaders & ~upto;
-  const uint32_t n_valid = (uint32_t)__popcll(ballot(valid));
-  const uint32_t next = after ? (uint32_t)__builtin_ctzll(after) : n_valid;
-  uint32_t b = 0;
-  if (leader) b = atomicAdd(&J.cells[c], next - lane);
-  b = __shfl(b, (int)head, 64);
-  if (valid) J.cellrank[i] = make_uint2(c, b + (lane - head));
+  // run lengths: from a leader to the next leader (or the end of the samples)
+  uint32_t b[C];
+  uint32_t first_after = nv;
+#pragma unroll
+  for (int c = (int)C - 1; c >= 0; --c) {
+    const uint64_t above = lead[c] & ~upto;
+    const uint32_t next = above ? (uint32_t)c * 64u + (uint32_t)__builtin_ctzll(above) : first_after;
+    b[c] = 0u;
+    if ((lead[c] >> lane) & 1ull) b[c] = atomicAdd(&J.cells[key[c]], next - ((uint32_t)c * 64u + lane));
+    if (lead[c]) first_after = (uint32_t)c * 64u + (uint32_t)__builtin_ctzll(lead[c]);
+  }
+  // ranks: base of the run's leader plus the distance to it
+  uint32_t carry_pos = 0u, carry_base = 0u;
+#pragma unroll
+  for (uint32_t c = 0; c < C; ++c) {
+    const uint32_t p = c * 64u + lane;
+    const uint64_t below = lead[c] & upto;
+    const uint32_t hl = below ? 63u - (uint32_t)__builtin_clzll(below) : 0u;
+    const uint32_t in_base = __shfl(b[c], (int)hl, 64);
+    if (p < nv) {
+      const uint32_t head = below ? c * 64u + hl : carry_pos;
+      const uint32_t rb = below ? in_base : carry_base;
+      J.cellrank[w0 + p] = make_uint2(key[c], rb + (p - head));
+    }
+    if (lead[c]) {
+      const uint32_t L = 63u - (uint32_t)__builtin_clzll(lead[c]);
+      carry_pos = c * 64u + L;
+      carry_base = (uint32_t)__builtin_amdgcn_readlane(b[c], L);
+    }
+  }
 }
 
 // -------------------------------------------------------------- cell scan
@@ -394,9 +429,9 @@ extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_apply(
 
 // ---------------------------------------------------------- neighbor mean
 // neighbor_mean.wgsl:75-203. One wave per task (<= 64 query samples that are
-// consecutive in sorted order, from a few neighbouring bins of one row). The
-// grid rows of the task's window are set up in parallel, one row per lane:
-// each row's bins split (uniformly for the wave) into
+// consecutive in sorted order, from neighbouring bins of one row). The grid
+// rows of the task's window are set up in parallel, one row per lane (up to
+// NM_ROWS per pass): each row's bins split (uniformly for the wave) into
 //   - bins fully inside the disk of every query sample (conservative, with
 //     an eps margin): all their samples pass the neighbor test, so their
 //     fixed-point sums come in O(1) from the prefix sums of the sorted
@@ -405,12 +440,14 @@ extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_apply(
 //   - bins beyond every disk, skipped.
 // Within a row the bins are contiguous in sorted order, so a row yields at
 // most two boundary ranges; bins narrower than the row height (x_split) end
-// those ranges within a fraction of a row height of the disk's edge. The ranges of up to 64 rows form one candidate
-// stream, fetched 64 samples at a time with coalesced loads into a
-// double-buffered wave-private LDS tile (the next chunk is in flight while
-// the current one is tested) and broadcast to every lane, which tests its own
-// sample:  t = (q - p) * texel_scale; accept when fmaf(tx, tx, ty*ty) <= r^2.
-// All sums are exact int64, so the result equals the all-pairs test.
+// those ranges within a fraction of a row height of the disk's edge. The
+// ranges form a candidate stream, fetched in chunks with coalesced loads into
+// a wave-private LDS tile (the next chunk is in flight while the current one
+// is tested) and broadcast to the lanes, which test their own sample:
+//   t = (q - p) * texel_scale; accept when fmaf(tx, tx, ty*ty) <= r^2.
+// Long streams are set up again per lane group (nm_group_rows) and the
+// longest are split across waves (bt_neighbor_mean_heavy). All sums are
+// exact int64, so the result equals the all-pairs test whatever the split.
 // Output is written in sorted order (the next iteration's input order), and
 // every sample, fixed or not, is splatted into the density counts.
 
@@ -962,11 +999,13 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
   }
 }
 
-// Runs the parts of heavy tasks: part p tests chunks p, p + P, p + 2P, ... of
-// the task's candidate stream (recomputed exactly as the main kernel set it
-// up) and adds its exact per-lane sums to the task's accumulators (part 0
-// also the inside-run sums). The part that finishes last (a counter; no wave
-// waits) reads the accumulators and writes the means.
+// Runs the parts of heavy tasks: part p tests group chunks p, p + P, p + 2P,
+// ... of the task's lane-group streams (set up by the main kernel and stored
+// in the task's record) and adds its exact per-lane sums to the task's
+// accumulators (part 0 also the inside-run sums). The part that finishes
+// last (a counter; no wave waits) reads the accumulators and writes the
+// means. Block 0 first sums the main launch's per-workgroup candidate counts
+// into the control block, for the next launch's split threshold.
 extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split) {
