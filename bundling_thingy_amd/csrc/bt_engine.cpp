@@ -475,7 +475,8 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
     j.pre_x.alloc(((size_t)n + 1) * 8);
     j.pre_y.alloc(((size_t)n + 1) * 8);
     if (n + 1 > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS) fail(BT_E_CAPACITY, "too many samples");
-    const size_t nb = std::max<size_t>(std::max<size_t>(j.bsum.bytes / 8, cdiv(n + 1, BT_SCAN_ITEMS)),
+    // block sums: cells (one per block) and sample prefix sums (x, y per block)
+    const size_t nb = std::max<size_t>(std::max<size_t>(j.bsum.bytes / 8, 2 * cdiv(n + 1, BT_SCAN_ITEMS)),
                                        cdiv(j.max_cells + 1ull, BT_SCAN_ITEMS));
     if (nb * 8 > j.bsum.bytes) j.bsum.alloc(nb * 8);
     if (j.o.flags & BT_FLAG_DENSITY) {

@@ -109,6 +109,18 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ __forceinline__ long long wave_sum64(long long v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
 #define JOB_PROLOGUE(KIND)                                                   \
   const int j = find_job(base, njobs, blockIdx.x);                          \
   const uint32_t lb = blockIdx.x - ld_uniform(base, (uint32_t)j);           \
@@ -134,10 +146,6 @@ extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restri
   }
   if (lb == 0 && threadIdx.x == 0) {
     J.peak[0] = 0u;
-    if (J.pre_x) {
-      J.pre_x[J.n] = 0ull;
-      J.pre_y[J.n] = 0ull;
-    }
     if (j == 0) {
       ctl->task_total = 0u;
       ctl->heavy_count = 0u;
@@ -400,31 +408,110 @@ extern "C" __global__ __launch_bounds__(256) void bt_scatter(const BtJob* __rest
   int fx = (int)rintf(x * BT_FIX_ONE), fy = (int)rintf(y * BT_FIX_ONE);
   J.sorted[slot] = make_float4(x, y, __int_as_float(fx), __int_as_float(fy));
   J.sid[slot] = id;
-  J.pre_x[slot] = (uint64_t)(int64_t)fx;
-  J.pre_y[slot] = (uint64_t)(int64_t)fy;
 }
 
-// Exclusive prefix sums of the sorted fixed-point coordinates (two's
-// complement int64 in u64 lanes), so any run of sorted samples sums in O(1).
+// Exclusive prefix sums of the sorted fixed-point coordinates, both axes in
+// one pass (two's complement int64 in u64 lanes), so any run of sorted
+// samples sums in O(1). The coordinates are read from the sorted records
+// (their upper 8 bytes); entry n is the total. Three phases over blocks of
+// BT_SCAN_ITEMS samples: block totals, a scan of the totals (interleaved x, y
+// in bsum), and the block-local scan plus the block's offset.
+__device__ __forceinline__ int2 qf_of(const BtJob& J, uint32_t i) {
+  if (i >= J.n) return make_int2(0, 0);
+  const int2* q = reinterpret_cast<const int2*>(J.sorted);
+  return q[2u * i + 1u];
+}
+
 extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_reduce(
-    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs, int axis) {
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
   JOB_PROLOGUE(BT_K_PTS);
-  scan_reduce_body<unsigned long long>((const unsigned long long*)(axis ? J.pre_y : J.pre_x),
-                                       J.n + 1u, lb, (unsigned long long*)J.bsum);
+  __shared__ long long part[8];
+  const uint32_t first = lb * BT_SCAN_ITEMS;
+  long long sx = 0, sy = 0;
+#pragma unroll 4
+  for (uint32_t k = threadIdx.x; k < BT_SCAN_ITEMS; k += 256u) {
+    const int2 q = qf_of(J, first + k);
+    sx += q.x;
+    sy += q.y;
+  }
+  sx = wave_sum64(sx);
+  sy = wave_sum64(sy);
+  if (LANE() == 0) {
+    part[threadIdx.x >> 6] = sx;
+    part[4 + (threadIdx.x >> 6)] = sy;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long* bs = reinterpret_cast<unsigned long long*>(J.bsum);
+    bs[2u * lb] = (unsigned long long)(part[0] + part[1] + part[2] + part[3]);
+    bs[2u * lb + 1u] = (unsigned long long)(part[4] + part[5] + part[6] + part[7]);
+  }
 }
 
 extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_bsum(const BtJob* __restrict__ jobs,
                                                                    int njobs) {
   const BtJob& J = jobs[blockIdx.x];
   const uint32_t nb = (J.n + 1u + BT_SCAN_ITEMS - 1u) / BT_SCAN_ITEMS;
-  scan_bsum_body<unsigned long long>((unsigned long long*)J.bsum, nb);
+  // nb <= BT_SCAN_ITEMS blocks, x and y interleaved in bsum
+  __shared__ long long part[8];
+  unsigned long long* bs = reinterpret_cast<unsigned long long*>(J.bsum);
+  constexpr uint32_t K = BT_SCAN_ITEMS / 256u;  // blocks per thread
+  long long vx[K], vy[K];
+  long long tx = 0, ty = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) {
+    const uint32_t blk = threadIdx.x * K + k;
+    vx[k] = tx;
+    vy[k] = ty;
+    if (blk < nb) {
+      tx += (long long)bs[2u * blk];
+      ty += (long long)bs[2u * blk + 1u];
+    }
+  }
+  long long totx, toty;
+  const long long bx = block_excl_scan(tx, part, &totx);
+  const long long by = block_excl_scan(ty, part + 4, &toty);
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) {
+    const uint32_t blk = threadIdx.x * K + k;
+    if (blk < nb) {
+      bs[2u * blk] = (unsigned long long)(bx + vx[k]);
+      bs[2u * blk + 1u] = (unsigned long long)(by + vy[k]);
+    }
+  }
 }
 
 extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_apply(
-    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs, int axis) {
+    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
   JOB_PROLOGUE(BT_K_PTS);
-  scan_apply_body<unsigned long long>((unsigned long long*)(axis ? J.pre_y : J.pre_x), J.n + 1u,
-                                      lb, (const unsigned long long*)J.bsum);
+  __shared__ int2 lds[BT_SCAN_ITEMS];
+  __shared__ long long part[8];
+  const uint32_t first = lb * BT_SCAN_ITEMS;
+  for (uint32_t k = threadIdx.x; k < BT_SCAN_ITEMS; k += 256u) lds[k] = qf_of(J, first + k);
+  __syncthreads();
+  long long ex[16], ey[16];
+  long long sx = 0, sy = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    ex[k] = sx;
+    ey[k] = sy;
+    const int2 q = lds[threadIdx.x * 16 + k];
+    sx += q.x;
+    sy += q.y;
+  }
+  long long totx, toty;
+  const unsigned long long* bs = reinterpret_cast<const unsigned long long*>(J.bsum);
+  const long long bx = block_excl_scan(sx, part, &totx) + (long long)bs[2u * lb];
+  const long long by = block_excl_scan(sy, part + 4, &toty) + (long long)bs[2u * lb + 1u];
+  const uint32_t n1 = J.n + 1u;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t i = first + threadIdx.x * 16u + (uint32_t)k;
+    if (i < n1) {
+      J.pre_x[i] = (uint64_t)(bx + ex[k]);
+      J.pre_y[i] = (uint64_t)(by + ey[k]);
+    }
+  }
 }
 
 // ---------------------------------------------------------- neighbor mean
@@ -460,17 +547,6 @@ __device__ __forceinline__ int cell_of_fast(float t, float inv_cell, uint32_t g)
   return (int)v;
 }
 
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
-
-__device__ __forceinline__ long long wave_sum64(long long v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
 
 // Work distribution: a work list (tasks, or parts of heavy tasks) is cut into
 // BT_XCDS contiguous parts (tasks are emitted in roughly spatial order, so a
@@ -1425,11 +1501,9 @@ void cell_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
   hipLaunchKernelGGL(bt_cell_scan_apply, L256(grid), jobs, base, nj);
 }
 void pre_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
-  for (int axis = 0; axis < 2; ++axis) {
-    hipLaunchKernelGGL(bt_pre_scan_reduce, L256(grid), jobs, base, nj, axis);
-    hipLaunchKernelGGL(bt_pre_scan_bsum, L256((uint32_t)nj), jobs, nj);
-    hipLaunchKernelGGL(bt_pre_scan_apply, L256(grid), jobs, base, nj, axis);
-  }
+  hipLaunchKernelGGL(bt_pre_scan_reduce, L256(grid), jobs, base, nj);
+  hipLaunchKernelGGL(bt_pre_scan_bsum, L256((uint32_t)nj), jobs, nj);
+  hipLaunchKernelGGL(bt_pre_scan_apply, L256(grid), jobs, base, nj);
 }
 void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
            BtControl* ctl, BtTask* tasks_out) {
