@@ -91,6 +91,7 @@ SIGNATURES = {
     "bt_read_points": (C.c_int, [_P, _P, _SZ]),
     "bt_read_density": (C.c_int, [_P, _U32, _P, _SZ]),
     "bt_read_grid": (C.c_int, [_P, _PU32, _SZ, _PU32, _SZ]),
+    "bt_read_endpoints": (C.c_int, [_P, _PU32, _SZ, _PU32]),
     "bt_get_info": (C.c_int, [_P, C.POINTER(bt_info)]),
     "bt_iter_params_for": (C.c_int, [C.POINTER(bt_options), _U32, C.POINTER(bt_iter_params)]),
     "bt_profile": (C.c_int, [_P, _U32, _PD, _U32, _PD]),
@@ -469,6 +470,19 @@ class Gpu:
         counts = np.zeros(p.gx * p.gy, dtype=np.uint32)
         self._check(lib().bt_read_grid(self._h, _u32p(cells), n, _u32p(counts), len(counts)))
         return cells, counts.reshape(p.gy, p.gx)
+
+    def endpoints(self):
+        """Sorted sample indices (Layout order) of the distinct path endpoint
+        positions: the deduplicate pass of endpoints.wgsl:9-49 that feeds the
+        endpoint dots (resampling.rs:137-149)."""
+        count = C.c_uint32()
+        rc = lib().bt_read_endpoints(self._h, None, 0, C.byref(count))
+        if rc != BT_E_CAPACITY:
+            self._check(rc)
+        out = np.zeros(count.value, dtype=np.uint32)
+        if count.value:
+            self._check(lib().bt_read_endpoints(self._h, _u32p(out), len(out), C.byref(count)))
+        return out
 
     def profile(self, n):
         ms = (C.c_double * 9)()

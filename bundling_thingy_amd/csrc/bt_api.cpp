@@ -161,6 +161,18 @@ int bt_read_grid(bt_ctx* ctx, uint32_t* cell_of, size_t n_samples, uint32_t* cou
                  [&] { ctx->engine->read_grid(0, cell_of, n_samples, counts, n_cells); });
 }
 
+int bt_read_endpoints(bt_ctx* ctx, uint32_t* indices, size_t capacity, uint32_t* count) {
+  CTX_CHECK();
+  if (!count || (capacity && !indices)) return BT_E_INVALID;
+  *count = 0;
+  return guarded(&ctx->err, [&] {
+    const std::vector<uint32_t> e = ctx->engine->read_endpoints(0);
+    *count = (uint32_t)e.size();
+    std::copy(e.begin(), e.begin() + std::min(capacity, e.size()), indices);
+    if (capacity < e.size()) bt::fail(BT_E_CAPACITY, "endpoint buffer too small");
+  });
+}
+
 int bt_job_samples(bt_ctx* ctx, uint32_t job, uint32_t* samples, uint32_t* iteration) {
   CTX_CHECK();
   return guarded(&ctx->err, [&] {

@@ -858,4 +858,43 @@ void Engine::read_grid(uint32_t job_i, uint32_t* cell_of, size_t n, uint32_t* co
   for (size_t c = 0; c < nc; ++c) counts[c] = starts[c + 1] - starts[c];
 }
 
+// resampling.rs:137-149,240-249 and endpoints.wgsl:9-49: the distinct
+// endpoint positions of the paths with two or more samples, one sample index
+// per position. The reference keeps whichever index wins its insertion race
+// and lists them in completion order; here each position keeps its smallest
+// index and the list is sorted, so the result is the same on every run.
+std::vector<uint32_t> Engine::read_endpoints(uint32_t job_i) {
+  Job& j = job(job_i);
+  std::vector<uint32_t> out;
+  if (j.n_paths == 0 || j.n == 0) return out;
+  size_t slots = 64;
+  while (slots < (size_t)j.n_paths * 4) slots <<= 1;
+  if (slots > 0x80000000ull) fail(BT_E_CAPACITY, "endpoint table exceeds 2^31 slots");
+  staging_.reserve((size_t)j.n * 12);
+  std::vector<Job*> js = {&j};
+  Plan p = make_plan(js, {j.iteration});
+  DevBuf plan, table;
+  plan.pool = pool_.get();
+  table.pool = pool_.get();
+  plan.alloc(p.host.size());
+  table.alloc(slots * 4);
+  put_plan(p, plan.p);
+  const unsigned char* dev = plan.as<unsigned char>();
+  const BtJob* J = reinterpret_cast<const BtJob*>(dev);
+  const uint32_t* base = reinterpret_cast<const uint32_t*>(dev + p.base_off);
+  btk::unpermute(stream_, p.grid[BT_K_BIN], J, base + BT_K_BIN * 2, 1, staging_.as<float>(),
+                 nullptr, 0u);
+  HIPCHK(hipMemsetAsync(table.p, 0, slots * 4, stream_));
+  btk::endpoints(stream_, p.grid[BT_K_PATH], J, base + BT_K_PATH * 2, 1, staging_.as<float>(),
+                 table.as<uint32_t>(), (uint32_t)(slots - 1));
+  HIPCHK(hipGetLastError());
+  std::vector<uint32_t> host(slots);
+  HIPCHK(hipMemcpyAsync(host.data(), table.p, slots * 4, hipMemcpyDeviceToHost, stream_));
+  sync();
+  for (uint32_t v : host)
+    if (v) out.push_back(v - 1u);
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
 }  // namespace bt

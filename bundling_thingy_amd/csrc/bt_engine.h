@@ -87,6 +87,8 @@ class Engine {
   void read_points(uint32_t job, void* rec12, size_t n);
   void read_density(uint32_t job, uint32_t kind, void* out, size_t n);
   void read_grid(uint32_t job, uint32_t* cell_of, size_t n, uint32_t* counts, size_t ncells);
+  // Sorted sample indices of the distinct path endpoint positions.
+  std::vector<uint32_t> read_endpoints(uint32_t job);
   static void read_points_of(Engine& e, Job& j, void* rec12, size_t n);
   static void read_density_of(Engine& e, Job& j, uint32_t kind, void* out, size_t n);
 

@@ -1482,6 +1482,57 @@ extern "C" __global__ __launch_bounds__(256) void bt_unpermute(const BtJob* __re
   }
 }
 
+// ------------------------------------------------------------- endpoints
+// endpoints.wgsl:9-49 (deduplicate, for --endpoint-dots): the distinct
+// positions among the first and last samples of the paths with two or more
+// samples. One thread per path inserts its two endpoints into an
+// open-addressing hash table (linear probing, load <= 1/2) keyed by the
+// position (-0 and 0 hash alike; positions compare with ==, as the
+// reference). A slot holds sample index + 1; a position already present
+// keeps the smallest index (atomicMin), so the table's content does not
+// depend on scheduling. `pts` holds the current records in Layout order
+// (endpoints never move).
+__device__ __forceinline__ void endpoint_insert(const float* pts, uint32_t* table, uint32_t mask,
+                                                uint32_t index) {
+  const float x = pts[3u * index], y = pts[3u * index + 1u];
+  const uint32_t bx = __float_as_uint(x == 0.0f ? 0.0f : x);
+  const uint32_t by = __float_as_uint(y == 0.0f ? 0.0f : y);
+  // 32-bit finalizer mix (MurmurHash3 fmix32) of the combined bits
+  uint32_t h = bx * 0x85ebca6bu + by;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  uint32_t slot = h & mask;
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    uint32_t old = __hip_atomic_load(&table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0u) {
+      old = atomicCAS(&table[slot], 0u, index + 1u);
+      if (old == 0u) return;  // inserted
+    }
+    const float ox = pts[3u * (old - 1u)], oy = pts[3u * (old - 1u) + 1u];
+    if (ox == x && oy == y) {
+      atomicMin(&table[slot], index + 1u);
+      return;
+    }
+    slot = (slot + 1u) & mask;
+  }
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_endpoints(const BtJob* __restrict__ jobs,
+                                                               const uint32_t* __restrict__ base,
+                                                               int njobs, const float* pts,
+                                                               uint32_t* table, uint32_t mask) {
+  JOB_PROLOGUE(BT_K_PATH);
+  const uint32_t i = lb * 256u + threadIdx.x;
+  if (i >= J.n_paths) return;
+  const uint32_t off = J.paths[3u * i], count = J.paths[3u * i + 2u];
+  if (count < 2u) return;
+  endpoint_insert(pts, table, mask, off);
+  endpoint_insert(pts, table, mask, off + count - 1u);
+}
+
 // ------------------------------------------------------- launch wrappers
 // Host-callable entry points used by the C++ runtime (bt_engine.cpp).
 #include "bt_launch.h"
@@ -1549,6 +1600,10 @@ void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32
 void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                float* out, uint32_t* cell_out, uint32_t want_cells) {
   hipLaunchKernelGGL(bt_unpermute, L256(grid), jobs, base, nj, out, cell_out, want_cells);
+}
+void endpoints(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+               const float* pts, uint32_t* table, uint32_t mask) {
+  hipLaunchKernelGGL(bt_endpoints, L256(grid), jobs, base, nj, pts, table, mask);
 }
 #undef L256
 }  // namespace btk

@@ -25,4 +25,6 @@ void path_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
 void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                float* out, uint32_t* cell_out, uint32_t want_cells);
+void endpoints(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+               const float* pts, uint32_t* table, uint32_t mask);
 }  // namespace btk

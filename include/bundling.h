@@ -133,6 +133,15 @@ int bt_read_density(bt_ctx* ctx, uint32_t kind, void* out, size_t n_pixels);
 int bt_read_grid(bt_ctx* ctx, uint32_t* cell_of, size_t n_samples, uint32_t* counts,
                  size_t n_cells);
 
+/* Endpoint dedup (endpoints.wgsl:9-49, run by resampling.rs:137-149,240-249
+ * when --endpoint-dots is set): the sample indices (Layout order) of the
+ * distinct positions among the first and last samples of the paths with two
+ * or more samples. Each position keeps its smallest index and the list is
+ * ascending (the reference keeps the first inserter, in completion order).
+ * *count always receives the number found; indices receives min(capacity,
+ * count) of them and BT_E_CAPACITY is returned when capacity < count. */
+int bt_read_endpoints(bt_ctx* ctx, uint32_t* indices, size_t capacity, uint32_t* count);
+
 int bt_get_info(bt_ctx* ctx, bt_info* info);
 
 /* Parameters iteration `iteration` of a job with these options runs with

@@ -236,6 +236,27 @@ def distortion(original, current, offsets, counts):
     return d.value, r.value
 
 
+def endpoints(points, paths):
+    """endpoints.wgsl:9-49 (deduplicate; run when --endpoint-dots is set,
+    resampling.rs:137-149): for every path {point_offset, segment_offset,
+    count} with count >= 2, its first and last sample; positions equal under
+    float == (so -0 equals 0) count once. The reference keeps the index of
+    whichever insertion wins and lists them in completion order; this
+    canonical form keeps each position's smallest index and sorts them.
+    points: (N, 2) float32 in Layout order; paths: (P, 3) uint32."""
+    xy = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 2)
+    pa = np.asarray(paths, dtype=np.int64).reshape(-1, 3)
+    pa = pa[pa[:, 2] >= 2]
+    cand = np.unique(np.concatenate([pa[:, 0], pa[:, 0] + pa[:, 2] - 1]))
+    if len(cand) == 0:
+        return np.zeros(0, dtype=np.uint32)
+    pos = xy[cand] + np.float32(0.0)  # -0 + 0 = +0: one bit pattern per value
+    key = pos.view(np.uint32).astype(np.uint64)
+    key = (key[:, 0] << np.uint64(32)) | key[:, 1]
+    _, first = np.unique(key, return_index=True)  # first occurrence = smallest index
+    return np.sort(cand[first]).astype(np.uint32)
+
+
 class Pipeline:
     """The oracle's whole pipeline for one graph under given per-iteration
     parameters (from the product's bt_iter_params_for, or built by hand)."""

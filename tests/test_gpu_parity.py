@@ -5,10 +5,12 @@ the FP32 positions and density fields, because the kernels and the oracle
 share one arithmetic contract. Tolerances from the reference's parity bar are
 asserted too (positions within 1e-5 of the bbox diagonal, density relative
 L-inf <= 1e-5) so the bar is visible in the test."""
+import ctypes
+
 import numpy as np
 import pytest
 
-from bundling_thingy_amd import workloads
+from bundling_thingy_amd import api, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -263,6 +265,33 @@ def test_large_case_split_invariance(bt):
     assert n > 4_000_000
     np.testing.assert_array_equal(res[0][0].view(np.uint32), res[1][0].view(np.uint32))
     np.testing.assert_array_equal(res[0][1], res[1][1])
+
+
+@pytest.mark.parametrize("case", ["cpu", "gpu"])
+def test_endpoints_match_oracle(bt, orc, cpu_case, case):
+    """Endpoint dedup (endpoints.wgsl): 2,100 / 9,800 edges between 235 /
+    1,715 nodes, so most endpoints repeat. Exact index list, before and
+    after iterations (endpoints are fixed samples)."""
+    g = cpu_case if case == "cpu" else workloads.plain_gpu_graph()
+    opts = (workloads.reference_cpu_options() if case == "cpu"
+            else bt.Options(spacing=0.002, radius=0.01, sigma=2.0, density=1024))
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(g)
+        pts, _ = gpu.points()
+        want = orc.endpoints(pts, gpu.paths())
+        assert 0 < len(want) <= len(np.unique(g.points, axis=0))
+        np.testing.assert_array_equal(gpu.endpoints(), want)
+        gpu.iterate()
+        gpu.iterate()
+        got = gpu.endpoints()
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(gpu.points()[0][got].view(np.uint32),
+                                      pts[want].view(np.uint32))
+        count = ctypes.c_uint32()
+        small = np.zeros(1, dtype=np.uint32)
+        rc = api.lib().bt_read_endpoints(gpu._h, small.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                         1, ctypes.byref(count))
+        assert rc == api.BT_E_CAPACITY and count.value == len(want) and small[0] == want[0]
 
 
 def test_error_paths(bt, cpu_case):

@@ -263,3 +263,44 @@ def test_distortion_identical_and_translation(orc):
     moved = (pts + v).astype(np.float32)
     d, r = orc.distortion(pts, moved, off, cnt)
     assert d == pytest.approx(0.05 / (2 * math.sqrt(2)), rel=1e-5)
+
+
+# ---------------------------------------------------------------- endpoints
+
+def test_endpoints_closed_form(orc):
+    # Layout: path 0 = samples 0..2, path 1 = 3..4, path 2 = one sample (5,
+    # skipped: count < 2), path 3 = 6..8. Sample 3 repeats sample 2, sample 6
+    # is (-0, 0) = sample 0, sample 8 repeats sample 4.
+    pts = np.array([[0, 0], [0.5, 0], [1, 0], [1, 0], [2, 2], [7, 7],
+                    [-0.0, 0], [1, 1], [2, 2]], dtype=np.float32)
+    paths = np.array([[0, 0, 3], [3, 2, 2], [5, 3, 1], [6, 3, 3]], dtype=np.uint32)
+    assert orc.endpoints(pts, paths).tolist() == [0, 2, 4]
+    assert orc.endpoints(pts, np.zeros((0, 3), np.uint32)).tolist() == []
+    # a closed path's two ends share a position: one endpoint
+    assert orc.endpoints(pts[[0, 1, 0]], [[0, 0, 3]]).tolist() == [0]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_endpoints_bruteforce(orc, seed):
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(0, 6, 300)
+    offsets = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    n = int(counts.sum())
+    # few distinct positions so that endpoints collide often; some -0 signs
+    pts = rng.integers(-3, 4, (n, 2)).astype(np.float32) * np.float32(0.25)
+    neg = rng.random((n, 2)) < 0.3
+    pts[neg & (pts == 0)] = np.float32(-0.0)
+    paths = np.stack([offsets, offsets, counts], 1).astype(np.uint32)
+    seen = []  # (x, y, smallest index) by pairwise == over every endpoint
+    for off, _, c in paths.tolist():
+        if c < 2:
+            continue
+        for i in (off, off + c - 1):
+            for k, (x, y, j) in enumerate(seen):
+                if x == pts[i, 0] and y == pts[i, 1]:
+                    seen[k] = (x, y, min(j, i))
+                    break
+            else:
+                seen.append((pts[i, 0], pts[i, 1], i))
+    want = sorted(j for _, _, j in seen)
+    assert orc.endpoints(pts, paths).tolist() == want
