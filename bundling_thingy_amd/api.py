@@ -22,6 +22,9 @@ BT_OK, BT_E_INVALID, BT_E_NO_GRAPH, BT_E_CAPACITY = 0, 1, 2, 3
 BT_E_DEVICE, BT_E_STATE, BT_E_MISMATCH, BT_E_BUSY = 4, 5, 6, 7
 FLAG_DENSITY, FLAG_DISTORTION = 1, 2
 DENSITY_COUNTS, DENSITY_ROWS, DENSITY_BLURRED, DENSITY_NORMALIZED = 0, 1, 2, 3
+PALETTE_CIVIDIS, PALETTE_VIRIDIS, PALETTE_PLASMA, PALETTE_BATLOW = 0, 1, 2, 3
+PALETTES = {"cividis": 0, "viridis": 1, "plasma": 2, "batlow": 3}
+COLOR_DENSITY, COLOR_EDGE_ID = 0, 1
 STAGES = ["clear", "grid_count", "grid_scan", "grid_tasks", "grid_scatter", "neighbor_mean",
           "blur_horizontal", "blur_vertical_and_maximum", "normalize"]
 
@@ -55,6 +58,17 @@ class bt_info(C.Structure):
                 ("iteration", C.c_uint32), ("capacity", C.c_uint64),
                 ("reserved_bytes", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("split_tasks", C.c_uint32), ("split_parts", C.c_uint32)]
+
+
+class bt_render_options(C.Structure):
+    _fields_ = [("palette", C.c_uint32), ("color_by", C.c_uint32), ("msaa", C.c_uint32),
+                ("endpoint_dots", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
+class bt_noisiness(C.Structure):
+    _fields_ = [("colored_pixels", C.c_uint64), ("background_pixels", C.c_uint64),
+                ("total_pixels", C.c_uint64), ("noisiness", C.c_double),
+                ("colored_to_background_ratio", C.c_double)]
 
 
 class bt_graph_desc(C.Structure):
@@ -92,6 +106,10 @@ SIGNATURES = {
     "bt_read_density": (C.c_int, [_P, _U32, _P, _SZ]),
     "bt_read_grid": (C.c_int, [_P, _PU32, _SZ, _PU32, _SZ]),
     "bt_read_endpoints": (C.c_int, [_P, _PU32, _SZ, _PU32]),
+    "bt_palette": (C.c_int, [_U32, _PF, _PF]),
+    "bt_render_options_default": (None, [C.POINTER(bt_render_options)]),
+    "bt_render_size": (C.c_int, [_P, _PU32, _PU32]),
+    "bt_render": (C.c_int, [_P, C.POINTER(bt_render_options), _P, _SZ]),
     "bt_get_info": (C.c_int, [_P, C.POINTER(bt_info)]),
     "bt_iter_params_for": (C.c_int, [C.POINTER(bt_options), _U32, C.POINTER(bt_iter_params)]),
     "bt_profile": (C.c_int, [_P, _U32, _PD, _U32, _PD]),
@@ -106,6 +124,10 @@ SIGNATURES = {
     "bt_trl_parse": (C.c_int, [C.c_char_p, _SZ, C.POINTER(_PF), _PU32, C.POINTER(_PU32), _PU32,
                                _PF, C.c_char_p, _SZ]),
     "bt_free": (None, [_P]),
+    "bt_noisiness_background": (C.c_int, [_P, _U32, _U32, _P, C.c_char_p, _SZ]),
+    "bt_noisiness_measure": (C.c_int, [_P, _U32, _U32, _P, C.POINTER(bt_noisiness),
+                                       C.c_char_p, _SZ]),
+    "bt_write_png": (C.c_int, [C.c_char_p, _P, _U32, _U32, C.c_char_p, _SZ]),
     "bt_normalization": (C.c_int, [_PF, _PF]),
     "bt_capacity": (C.c_int, [C.POINTER(bt_options), _U32, _U32, _PU64, _PU64]),
     "bt_blur_weights": (_U32, [_F, _PF, _U32]),
@@ -251,6 +273,62 @@ class Graph:
     @property
     def edge_count(self):
         return len(self.starts) - 1
+
+
+# ------------------------------------------------------------------- images
+
+def _image_args(rgba):
+    a = np.ascontiguousarray(rgba, dtype=np.uint8)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError("expected a (height, width, 4) RGBA array")
+    return a, a.shape[1], a.shape[0]
+
+
+def palette(name):
+    """(16 x 3 float32 stops, endpoint RGB) of a palette (palette.rs:26-130)."""
+    cols = np.zeros((16, 3), dtype=np.float32)
+    ep = np.zeros(3, dtype=np.float32)
+    idx = PALETTES[name] if isinstance(name, str) else int(name)
+    rc = lib().bt_palette(idx, _f32p(cols), _f32p(ep))
+    if rc:
+        raise BtError(rc, "unknown palette")
+    return cols, ep
+
+
+def noisiness_background(blank):
+    """noisiness::background (noisiness.rs:28-42): RGB of an opaque uniform frame."""
+    a, w, h = _image_args(blank)
+    rgb = (C.c_uint8 * 3)()
+    err = C.create_string_buffer(256)
+    rc = lib().bt_noisiness_background(a.ctypes.data_as(_P), w, h, C.cast(rgb, _P), err, 256)
+    if rc:
+        raise BtError(rc, err.value.decode())
+    return tuple(rgb)
+
+
+def noisiness(rgba, rgb):
+    """noisiness::measure (noisiness.rs:44-74) as a dict of the reference's fields."""
+    a, w, h = _image_args(rgba)
+    bg = (C.c_uint8 * 3)(*rgb)
+    m = bt_noisiness()
+    err = C.create_string_buffer(256)
+    rc = lib().bt_noisiness_measure(a.ctypes.data_as(_P), w, h, C.cast(bg, _P), C.byref(m), err,
+                                    256)
+    if rc:
+        raise BtError(rc, err.value.decode())
+    ratio = m.colored_to_background_ratio
+    return {"colored_pixels": m.colored_pixels, "background_pixels": m.background_pixels,
+            "total_pixels": m.total_pixels, "noisiness": m.noisiness,
+            "colored_to_background_ratio": None if ratio != ratio else ratio}
+
+
+def write_png(path, rgba):
+    """write_image (main.rs:193-221): 8-bit RGBA PNG."""
+    a, w, h = _image_args(rgba)
+    err = C.create_string_buffer(256)
+    rc = lib().bt_write_png(os.fsencode(path), a.ctypes.data_as(_P), w, h, err, 256)
+    if rc:
+        raise BtError(rc, err.value.decode())
 
 
 # ------------------------------------------------------------------- geometry
@@ -470,6 +548,25 @@ class Gpu:
         counts = np.zeros(p.gx * p.gy, dtype=np.uint32)
         self._check(lib().bt_read_grid(self._h, _u32p(cells), n, _u32p(counts), len(counts)))
         return cells, counts.reshape(p.gy, p.gx)
+
+    def render_size(self):
+        w, h = C.c_uint32(), C.c_uint32()
+        self._check(lib().bt_render_size(self._h, C.byref(w), C.byref(h)))
+        return w.value, h.value
+
+    def render(self, palette="viridis", color_by=COLOR_DENSITY, msaa=4, endpoint_dots=False):
+        """Gpu::render + Gpu::readback (gpu.rs:572-651): the export image as a
+        (height, width, 4) uint8 RGBA array, row 0 at the top. Without an
+        uploaded graph: the blank frame."""
+        ro = bt_render_options()
+        ro.palette = PALETTES[palette] if isinstance(palette, str) else int(palette)
+        ro.color_by = int(color_by)
+        ro.msaa = int(msaa)
+        ro.endpoint_dots = int(bool(endpoint_dots))
+        w, h = self.render_size()
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        self._check(lib().bt_render(self._h, C.byref(ro), out.ctypes.data_as(_P), out.nbytes))
+        return out
 
     def endpoints(self):
         """Sorted sample indices (Layout order) of the distinct path endpoint

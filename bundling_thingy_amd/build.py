@@ -16,7 +16,8 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bt_kernels.hip", "bt_common.cpp", "bt_engine.cpp", "bt_api.cpp"]
+SOURCES = ["bt_kernels.hip", "bt_render.hip", "bt_common.cpp", "bt_engine.cpp", "bt_api.cpp",
+           "bt_image.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -49,7 +50,7 @@ def build(verbose=False):
             if out.strip():
                 print(out)
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-lz", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)

@@ -11,6 +11,7 @@
 
 #include "bt_common.h"
 #include "bt_engine.h"
+#include "bt_palette.h"
 
 struct bt_ctx {
   std::unique_ptr<bt::Engine> engine;
@@ -171,6 +172,37 @@ int bt_read_endpoints(bt_ctx* ctx, uint32_t* indices, size_t capacity, uint32_t*
     std::copy(e.begin(), e.begin() + std::min(capacity, e.size()), indices);
     if (capacity < e.size()) bt::fail(BT_E_CAPACITY, "endpoint buffer too small");
   });
+}
+
+int bt_palette(uint32_t palette, float* colors48, float* rgb3) {
+  static const float pal[4][16][3] = BT_PALETTE_TABLE;
+  static const float ep[4][3] = BT_ENDPOINT_TABLE;
+  if (palette > BT_PALETTE_BATLOW) return BT_E_INVALID;
+  if (colors48) std::memcpy(colors48, pal[palette], sizeof pal[palette]);
+  if (rgb3) std::memcpy(rgb3, ep[palette], sizeof ep[palette]);
+  return BT_OK;
+}
+
+void bt_render_options_default(bt_render_options* ro) {
+  if (!ro) return;
+  std::memset(ro, 0, sizeof *ro);
+  ro->palette = BT_PALETTE_VIRIDIS;
+  ro->color_by = BT_COLOR_DENSITY;
+  ro->msaa = 4;
+}
+
+int bt_render_size(bt_ctx* ctx, uint32_t* width, uint32_t* height) {
+  CTX_CHECK();
+  if (!width || !height) return BT_E_INVALID;
+  return guarded(&ctx->err, [&] { ctx->engine->render_size(width, height); });
+}
+
+int bt_render(bt_ctx* ctx, const bt_render_options* ro, uint8_t* rgba, size_t n_bytes) {
+  CTX_CHECK();
+  if (!rgba) return BT_E_INVALID;
+  bt_render_options d;
+  bt_render_options_default(&d);
+  return guarded(&ctx->err, [&] { ctx->engine->render(ro ? *ro : d, rgba, n_bytes); });
 }
 
 int bt_job_samples(bt_ctx* ctx, uint32_t job, uint32_t* samples, uint32_t* iteration) {

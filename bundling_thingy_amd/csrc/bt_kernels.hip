@@ -145,7 +145,7 @@ extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restri
     if (clear_density && i < npix) dens[i] = 0u;
   }
   if (lb == 0 && threadIdx.x == 0) {
-    J.peak[0] = 0u;
+    if (J.peak) J.peak[0] = 0u;  // no peak buffer when density is off
     if (j == 0) {
       ctl->task_total = 0u;
       ctl->heavy_count = 0u;

@@ -1,0 +1,492 @@
+// Path rasterizer for the export image (render_paths.wgsl:1-97, gpu.rs:572-651)
+// and the endpoint dots, as compute kernels:
+//   count    one thread per sample: the quad of the segment (i, i+1), its
+//            MSAA coverage per pixel, and how many fragments it makes
+//   scan     exclusive sum of the counts (fragment offsets in draw order)
+//   emit     the same walk, writing {pixel | coverage << 24, segment}
+//   sort     stable radix sort on the pixel bits: each pixel's fragments
+//            end up together, still in draw (segment) order
+//   runs     first and last fragment of every pixel
+//   dots     endpoint outline / dot coverage bits per sample (atomicOr)
+//   resolve  one thread per pixel: blends its fragments into its samples in
+//            order (8-bit storage after every blend, as an RGBA8 MSAA target
+//            holds them), applies the dots, averages the samples
+//
+// Rasterization contract (the hardware's MSAA coverage is not pinned by any
+// reference output, so this is the definition the oracle shares):
+// corners are computed in f32 as the vertex shader does, mapped to pixels by
+// ((x * 0.5 + 0.5) * W, (0.5 - y * 0.5) * H) and snapped to 1/256 pixel; each
+// quad is the strip's two triangles (c0 c1 c2) and (c2 c1 c3) with no
+// culling; a sample is inside a triangle when all three integer edge
+// functions are positive, or zero on a top or left edge. Sample positions are
+// the standard 2x / 4x locations. Fragment shading happens once per pixel at
+// its center; endpoint dots are shaded per sample (interpolate(sample)).
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "bt_engine.h"
+#include "bt_launch.h"
+#include "bt_palette.h"
+
+namespace bt {
+namespace {
+
+// Standard sample locations, 1/256 pixel units: {x, y} per sample.
+__constant__ int kSample4[4][2] = {{96, 32}, {224, 96}, {32, 160}, {160, 224}};
+__constant__ int kSample2[2][2] = {{192, 192}, {64, 64}};
+
+__constant__ float kPalette[4][16][3] = BT_PALETTE_TABLE;
+__constant__ float kEndpoint[4][3] = BT_ENDPOINT_TABLE;
+
+constexpr float kOpacity = 0.1f;       // PATH_OPACITY
+constexpr float kHalfWidth = 0.001f;   // quad half-width, clip units
+constexpr float kDotRadius = 3.0f;     // ENDPOINT_RADIUS, pixels
+constexpr float kDotOutline = 1.0f;    // ENDPOINT_OUTLINE_WIDTH, pixels
+constexpr float kClear[3] = {0.025f, 0.025f, 0.035f};  // gpu.rs:586-591
+constexpr uint32_t kMaxPixelBits = 24;  // fragment keys hold the pixel in bits 0..23
+
+struct Frame {
+  const float* pts;  // current samples, Layout order, {x, y, edge id}
+  uint32_t n;        // samples
+  uint32_t w, h;     // render target
+  uint32_t msaa;     // 2 or 4
+  uint32_t n_edges;  // paths (the reference's edges buffer length)
+  const float* dens; // normalized density, dw x dh (density colouring)
+  uint32_t dw, dh;
+  uint32_t palette, color_by;
+};
+
+__device__ __forceinline__ void sample_pos(uint32_t msaa, uint32_t k, int& sx, int& sy) {
+  if (msaa == 4u) {
+    sx = kSample4[k][0];
+    sy = kSample4[k][1];
+  } else {
+    sx = kSample2[k][0];
+    sy = kSample2[k][1];
+  }
+}
+
+// Unorm8 storage of a blend result.
+__host__ __device__ __forceinline__ uint32_t unorm8(float x) {
+  return (uint32_t)rintf(fminf(fmaxf(x, 0.0f), 1.0f) * 255.0f);
+}
+
+// BlendState::ALPHA_BLENDING with a constant source alpha: one channel.
+__device__ __forceinline__ uint32_t blend8(uint32_t dst, float src) {
+  return unorm8(src * kOpacity + ((float)dst * (1.0f / 255.0f)) * (1.0f - kOpacity));
+}
+
+// Fixed-point (1/256 px) corners of the quad of segment (i, i + 1): the
+// vertex shader's corners (render_paths.wgsl:36-50) through the viewport
+// transform. False when i is the last sample of its path, or a corner is not
+// finite / out of the fixed-point range.
+struct Quad {
+  int x[4], y[4];
+};
+
+__device__ bool seg_quad(const Frame& f, uint32_t i, Quad& q) {
+  if (i + 1u >= f.n) return false;
+  const float* a = f.pts + 3ull * i;
+  if (__float_as_uint(a[2]) != __float_as_uint(a[5])) return false;
+  const float x0 = a[0], y0 = a[1], x1 = a[3], y1 = a[4];
+  const float dx = x1 - x0, dy = y1 - y0;
+  const float len = sqrtf(dx * dx + dy * dy);
+  const float k = kHalfWidth / fmaxf(len, 1e-10f);
+  const float nx = -dy * k, ny = dx * k;
+  const float cx[4] = {x0 + nx, x0 - nx, x1 + nx, x1 - nx};
+  const float cy[4] = {y0 + ny, y0 - ny, y1 + ny, y1 - ny};
+  const float W = (float)f.w, H = (float)f.h;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float fx = (cx[c] * 0.5f + 0.5f) * W;
+    const float fy = (0.5f - cy[c] * 0.5f) * H;
+    if (!(fabsf(fx) < 1.0e6f && fabsf(fy) < 1.0e6f)) return false;
+    q.x[c] = (int)rintf(fx * 256.0f);
+    q.y[c] = (int)rintf(fy * 256.0f);
+  }
+  return true;
+}
+
+// One triangle's edge functions E = dx * (sy - ay) - dy * (sx - ax), oriented
+// so the inside is E > 0; bias 0 on top / left edges (ties inside), else 1.
+struct Tri {
+  int dx[3], dy[3], ax[3], ay[3], bias[3];
+  bool live;
+};
+
+__device__ void tri_setup(int ax, int ay, int bx, int by, int cx, int cy, Tri& t) {
+  const long long area = (long long)(bx - ax) * (cy - ay) - (long long)(by - ay) * (cx - ax);
+  t.live = area != 0;
+  if (area < 0) {
+    int tx = bx, ty = by;
+    bx = cx;
+    by = cy;
+    cx = tx;
+    cy = ty;
+  }
+  const int px[3] = {ax, bx, cx}, py[3] = {ay, by, cy};
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int n = e == 2 ? 0 : e + 1;
+    t.ax[e] = px[e];
+    t.ay[e] = py[e];
+    t.dx[e] = px[n] - px[e];
+    t.dy[e] = py[n] - py[e];
+    const bool top_left = t.dy[e] < 0 || (t.dy[e] == 0 && t.dx[e] > 0);
+    t.bias[e] = top_left ? 0 : 1;
+  }
+}
+
+__device__ __forceinline__ bool tri_in(const Tri& t, int sx, int sy) {
+  bool in = t.live;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const long long E = (long long)t.dx[e] * (sy - t.ay[e]) - (long long)t.dy[e] * (sx - t.ax[e]);
+    in = in && E >= t.bias[e];
+  }
+  return in;
+}
+
+struct Span {
+  int x0, x1, y0, y1;  // inclusive pixel range (empty when x0 > x1 or y0 > y1)
+};
+
+__device__ Span quad_span(const Frame& f, const Quad& q) {
+  int mnx = q.x[0], mxx = q.x[0], mny = q.y[0], mxy = q.y[0];
+#pragma unroll
+  for (int c = 1; c < 4; ++c) {
+    mnx = min(mnx, q.x[c]);
+    mxx = max(mxx, q.x[c]);
+    mny = min(mny, q.y[c]);
+    mxy = max(mxy, q.y[c]);
+  }
+  // samples sit at 32..224 / 256 inside a pixel
+  Span s;
+  s.x0 = max(0, -((224 - mnx) >> 8));
+  s.x1 = min((int)f.w - 1, (mxx - 32) >> 8);
+  s.y0 = max(0, -((224 - mny) >> 8));
+  s.y1 = min((int)f.h - 1, (mxy - 32) >> 8);
+  return s;
+}
+
+__device__ __forceinline__ void pixel_masks(const Frame& f, const Tri& t0, const Tri& t1, int px,
+                                            int py, uint32_t& m0, uint32_t& m1) {
+  m0 = 0u;
+  m1 = 0u;
+  for (uint32_t k = 0; k < f.msaa; ++k) {
+    int sx, sy;
+    sample_pos(f.msaa, k, sx, sy);
+    sx += px * 256;
+    sy += py * 256;
+    if (tri_in(t0, sx, sy)) m0 |= 1u << k;
+    if (tri_in(t1, sx, sy)) m1 |= 1u << k;
+  }
+}
+
+// Walks the fragments of segment i in draw order: the pixels of its quad in
+// row-major order; a pixel that both triangles cover on a common sample gets
+// two fragments (first triangle first), otherwise one with both masks.
+template <class F>
+__device__ void walk_segment(const Frame& f, uint32_t i, F&& emit) {
+  Quad q;
+  if (!seg_quad(f, i, q)) return;
+  Tri t0, t1;
+  tri_setup(q.x[0], q.y[0], q.x[1], q.y[1], q.x[2], q.y[2], t0);
+  tri_setup(q.x[2], q.y[2], q.x[1], q.y[1], q.x[3], q.y[3], t1);
+  const Span s = quad_span(f, q);
+  for (int py = s.y0; py <= s.y1; ++py)
+    for (int px = s.x0; px <= s.x1; ++px) {
+      uint32_t m0, m1;
+      pixel_masks(f, t0, t1, px, py, m0, m1);
+      const uint32_t pix = (uint32_t)py * f.w + (uint32_t)px;
+      if (m0 & m1) {
+        emit(pix, m0);
+        emit(pix, m1);
+      } else if (m0 | m1) {
+        emit(pix, m0 | m1);
+      }
+    }
+}
+
+__global__ __launch_bounds__(256) void rs_count(Frame f, uint32_t* cnt) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i == 0) cnt[f.n] = 0;
+  if (i >= f.n) return;
+  uint32_t c = 0;
+  walk_segment(f, i, [&](uint32_t, uint32_t) { ++c; });
+  cnt[i] = c;
+}
+
+__global__ __launch_bounds__(256) void rs_emit(Frame f, const uint32_t* off, uint32_t* keys,
+                                               uint32_t* vals) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= f.n) return;
+  uint32_t o = off[i];
+  walk_segment(f, i, [&](uint32_t pix, uint32_t mask) {
+    keys[o] = pix | (mask << kMaxPixelBits);
+    vals[o] = i;
+    ++o;
+  });
+}
+
+__global__ __launch_bounds__(256) void rs_runs(const uint32_t* keys, uint32_t m, uint32_t* first,
+                                               uint32_t* last) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j >= m) return;
+  const uint32_t pm = (1u << kMaxPixelBits) - 1u;
+  const uint32_t p = keys[j] & pm;
+  if (j == 0 || (keys[j - 1] & pm) != p) first[p] = j;
+  if (j + 1 == m || (keys[j + 1] & pm) != p) last[p] = j + 1;
+}
+
+// Endpoint outline (bit k) and dot (bit 4 + k) coverage of sample k:
+// vs_endpoint's square of +-3 px around the endpoint, discarded outside the
+// unit circle (outline) or the (1 - 1/3) circle (dot), render_paths.wgsl:73-97.
+__global__ __launch_bounds__(256) void rs_dots(Frame f, const uint32_t* paths, uint32_t n_paths,
+                                               uint32_t* dots) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t path = t >> 1;
+  if (path >= n_paths) return;
+  const uint32_t count = paths[3u * path + 2u];
+  if (count < 2u) return;
+  const uint32_t idx = paths[3u * path] + ((t & 1u) ? count - 1u : 0u);
+  const float cx = (f.pts[3ull * idx] * 0.5f + 0.5f) * (float)f.w;
+  const float cy = (0.5f - f.pts[3ull * idx + 1] * 0.5f) * (float)f.h;
+  if (!(fabsf(cx) < 1.0e6f && fabsf(cy) < 1.0e6f)) return;
+  const float inner = 1.0f - kDotOutline / kDotRadius;
+  const float inner2 = inner * inner;
+  const int x0 = max(0, (int)floorf(cx - kDotRadius) - 1);
+  const int x1 = min((int)f.w - 1, (int)floorf(cx + kDotRadius) + 1);
+  const int y0 = max(0, (int)floorf(cy - kDotRadius) - 1);
+  const int y1 = min((int)f.h - 1, (int)floorf(cy + kDotRadius) + 1);
+  for (int py = y0; py <= y1; ++py)
+    for (int px = x0; px <= x1; ++px) {
+      uint32_t bits = 0;
+      for (uint32_t k = 0; k < f.msaa; ++k) {
+        int sx, sy;
+        sample_pos(f.msaa, k, sx, sy);
+        const float ox = ((float)px + (float)sx * (1.0f / 256.0f) - cx) / kDotRadius;
+        const float oy = ((float)py + (float)sy * (1.0f / 256.0f) - cy) / kDotRadius;
+        if (!(ox >= -1.0f && ox < 1.0f && oy >= -1.0f && oy < 1.0f)) continue;
+        const float d2 = ox * ox + oy * oy;
+        if (d2 <= 1.0f) bits |= 1u << k;
+        if (d2 <= inner2) bits |= 1u << (4u + k);
+      }
+      if (bits) atomicOr(&dots[(uint32_t)py * f.w + (uint32_t)px], bits);
+    }
+}
+
+// fs_main's color (render_paths.wgsl:53-66): value -> 15 palette intervals.
+__device__ __forceinline__ void palette_rgb(uint32_t pal, float value, float rgb[3]) {
+  const float stop = value * 15.0f;
+  const uint32_t idx = min((uint32_t)fmaxf(stop, 0.0f), 14u);
+  const float t = stop - (float)idx;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    rgb[c] = kPalette[pal][idx][c] * (1.0f - t) + kPalette[pal][idx + 1u][c] * t;
+}
+
+// The density texture (rgba16f, R channel) sampled bilinearly at a pixel's
+// center with clamp-to-edge; weights quantized to 1/256 as texture units do.
+__device__ float density_at(const Frame& f, uint32_t px, uint32_t py) {
+  const float tx = ((float)px + 0.5f) / (float)f.w * (float)(f.dw - 1u);
+  const float ty = ((float)py + 0.5f) / (float)f.h * (float)(f.dh - 1u);
+  const float fx = floorf(tx), fy = floorf(ty);
+  const float ax = rintf((tx - fx) * 256.0f) * (1.0f / 256.0f);
+  const float ay = rintf((ty - fy) * 256.0f) * (1.0f / 256.0f);
+  const uint32_t x0 = min((uint32_t)fx, f.dw - 1u), x1 = min(x0 + 1u, f.dw - 1u);
+  const uint32_t y0 = min((uint32_t)fy, f.dh - 1u), y1 = min(y0 + 1u, f.dh - 1u);
+  auto h = [&](uint32_t x, uint32_t y) {
+    return __half2float(__float2half_rn(f.dens[(size_t)y * f.dw + x]));
+  };
+  const float top = h(x0, y0) * (1.0f - ax) + h(x1, y0) * ax;
+  const float bot = h(x0, y1) * (1.0f - ax) + h(x1, y1) * ax;
+  const float v = top * (1.0f - ay) + bot * ay;
+  return fminf(fmaxf(v, 0.0f), 1.0f);
+}
+
+__global__ __launch_bounds__(256) void rs_resolve(Frame f, const uint32_t* keys,
+                                                  const uint32_t* vals, const uint32_t* first,
+                                                  const uint32_t* last, const uint32_t* dots,
+                                                  uint32_t* out) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= f.w * f.h) return;
+  const uint32_t px = p % f.w, py = p / f.w;
+  uint32_t v[4][3];
+  const uint32_t bg[3] = {unorm8(kClear[0]), unorm8(kClear[1]), unorm8(kClear[2])};
+  for (int k = 0; k < 4; ++k)
+    for (int c = 0; c < 3; ++c) v[k][c] = bg[c];
+  const uint32_t s0 = first ? first[p] : 0u, s1 = last ? last[p] : 0u;
+  if (s1 > s0) {
+    const bool by_edge = f.color_by == BT_COLOR_EDGE_ID;
+    float col[3] = {0.0f, 0.0f, 0.0f};
+    if (!by_edge) palette_rgb(f.palette, density_at(f, px, py), col);
+    const float denom = (float)(max(f.n_edges, 2u) - 1u);
+    uint32_t settled = 0;  // density colouring: samples a further blend leaves unchanged
+    for (uint32_t j = s0; j < s1; ++j) {
+      const uint32_t m = keys[j] >> kMaxPixelBits;
+      if (by_edge) {
+        const uint32_t id = __float_as_uint(f.pts[3ull * vals[j] + 2]);
+        palette_rgb(f.palette, (float)id / denom, col);
+      }
+      for (uint32_t k = 0; k < f.msaa; ++k) {
+        if (!((m >> k) & 1u)) continue;
+        uint32_t same = 1u;
+        for (int c = 0; c < 3; ++c) {
+          const uint32_t nv = blend8(v[k][c], col[c]);
+          same &= nv == v[k][c];
+          v[k][c] = nv;
+        }
+        if (same) settled |= 1u << k;
+      }
+      // with one colour per pixel, a settled sample stays settled
+      if (!by_edge && settled == (1u << f.msaa) - 1u) break;
+    }
+  }
+  const uint32_t d = dots ? dots[p] : 0u;
+  if (d) {
+    uint32_t ring[3], dot[3];
+    for (int c = 0; c < 3; ++c) {
+      ring[c] = unorm8(kPalette[f.palette][0][c] * 0.1f);
+      dot[c] = unorm8(kEndpoint[f.palette][c]);
+    }
+    for (uint32_t k = 0; k < f.msaa; ++k) {
+      const uint32_t* src = ((d >> (4u + k)) & 1u) ? dot : ((d >> k) & 1u) ? ring : nullptr;
+      if (src)
+        for (int c = 0; c < 3; ++c) v[k][c] = src[c];
+    }
+  }
+  uint32_t rgba = 0xff000000u;
+  for (int c = 0; c < 3; ++c) {
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < f.msaa; ++k) sum += v[k][c];
+    rgba |= ((sum + f.msaa / 2u) / f.msaa) << (8 * c);
+  }
+  out[p] = rgba;
+}
+
+uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255u) / 256u); }
+
+}  // namespace
+
+void Engine::render_size(uint32_t* w, uint32_t* h) {
+  if (!jobs_.empty()) {
+    *w = jobs_[0]->dw;
+    *h = jobs_[0]->dh;
+    return;
+  }
+  bt_iter_params p;
+  iter_params(opts_, 0, 1, &p);
+  *w = p.density_w;
+  *h = p.density_h;
+}
+
+void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes) {
+  if (ro.msaa != 2u && ro.msaa != 4u) fail(BT_E_INVALID, "MSAA must be 2 or 4 samples");
+  if (ro.palette > BT_PALETTE_BATLOW) fail(BT_E_INVALID, "unknown palette");
+  if (ro.color_by > BT_COLOR_EDGE_ID) fail(BT_E_INVALID, "unknown color_by");
+  Frame f{};
+  render_size(&f.w, &f.h);
+  if (f.w == 0 || f.h == 0) fail(BT_E_INVALID, "image dimensions must be nonzero");
+  if ((uint64_t)f.w * f.h > (1ull << kMaxPixelBits))
+    fail(BT_E_CAPACITY, "render target exceeds 2^24 pixels");
+  if (n_bytes != (size_t)f.w * f.h * 4) fail(BT_E_INVALID, "image buffer size mismatch");
+  f.msaa = ro.msaa;
+  f.palette = ro.palette;
+  f.color_by = ro.color_by;
+  const size_t npix = (size_t)f.w * f.h;
+  DevBuf image, plan, cnt, off, keys, vals, keys2, vals2, first, last, dots, temp;
+  for (DevBuf* b :
+       {&image, &plan, &cnt, &off, &keys, &vals, &keys2, &vals2, &first, &last, &dots, &temp})
+    b->pool = pool_.get();
+  image.alloc(npix * 4);
+  const uint32_t *kkeys = nullptr, *kvals = nullptr, *kfirst = nullptr, *klast = nullptr,
+                 *kdots = nullptr;
+  if (!jobs_.empty() && jobs_[0]->n > 0) {
+    Job& j = *jobs_[0];
+    if (ro.color_by == BT_COLOR_DENSITY && !j.dens_norm.p)
+      fail(BT_E_STATE, "density is disabled (color_by != density)");
+    f.n = j.n;
+    f.n_edges = j.n_paths;
+    f.dens = j.dens_norm.as<float>();
+    f.dw = j.dw;
+    f.dh = j.dh;
+    // the current samples in Layout order
+    staging_.reserve((size_t)j.n * 12);
+    std::vector<Job*> js = {&j};
+    Plan p = make_plan(js, {j.iteration});
+    plan.alloc(p.host.size());
+    put_plan(p, plan.p);
+    const unsigned char* dev = plan.as<unsigned char>();
+    btk::unpermute(stream_, p.grid[BT_K_BIN], reinterpret_cast<const BtJob*>(dev),
+                   reinterpret_cast<const uint32_t*>(dev + p.base_off) + BT_K_BIN * 2, 1,
+                   staging_.as<float>(), nullptr, 0u);
+    HIPCHK(hipGetLastError());
+    f.pts = staging_.as<float>();
+
+    // fragment counts -> offsets (draw order) -> total
+    cnt.alloc(((size_t)f.n + 1) * 4);
+    off.alloc(((size_t)f.n + 1) * 4);
+    hipLaunchKernelGGL(rs_count, dim3(blocks(f.n)), dim3(256), 0, stream_, f,
+                       cnt.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.as<uint32_t>(), off.as<uint32_t>(),
+                                            f.n + 1, stream_));
+    temp.alloc(std::max<size_t>(tb, 256));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(temp.p, tb, cnt.as<uint32_t>(), off.as<uint32_t>(),
+                                            f.n + 1, stream_));
+    uint32_t m = 0;
+    HIPCHK(hipMemcpyAsync(&m, off.as<uint32_t>() + f.n, 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+    if (m > 0) {
+      keys.alloc((size_t)m * 4);
+      vals.alloc((size_t)m * 4);
+      keys2.alloc((size_t)m * 4);
+      vals2.alloc((size_t)m * 4);
+      hipLaunchKernelGGL(rs_emit, dim3(blocks(f.n)), dim3(256), 0, stream_, f,
+                         off.as<uint32_t>(), keys.as<uint32_t>(), vals.as<uint32_t>());
+      HIPCHK(hipGetLastError());
+      // stable sort on the pixel bits: per pixel, fragments stay in draw order
+      int bits = 1;
+      while ((1ull << bits) < npix) ++bits;
+      hipcub::DoubleBuffer<uint32_t> dk(keys.as<uint32_t>(), keys2.as<uint32_t>());
+      hipcub::DoubleBuffer<uint32_t> dv(vals.as<uint32_t>(), vals2.as<uint32_t>());
+      size_t sb = 0;
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, dk, dv, (int)m, 0, bits, stream_));
+      temp.reserve(sb);
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(temp.p, sb, dk, dv, (int)m, 0, bits, stream_));
+      first.alloc(npix * 4);
+      last.alloc(npix * 4);
+      HIPCHK(hipMemsetAsync(first.p, 0, npix * 4, stream_));
+      HIPCHK(hipMemsetAsync(last.p, 0, npix * 4, stream_));
+      hipLaunchKernelGGL(rs_runs, dim3(blocks(m)), dim3(256), 0, stream_, dk.Current(), m,
+                         first.as<uint32_t>(), last.as<uint32_t>());
+      HIPCHK(hipGetLastError());
+      kkeys = dk.Current();
+      kvals = dv.Current();
+      kfirst = first.as<uint32_t>();
+      klast = last.as<uint32_t>();
+    }
+    if (ro.endpoint_dots && j.n_paths > 0) {
+      dots.alloc(npix * 4);
+      HIPCHK(hipMemsetAsync(dots.p, 0, npix * 4, stream_));
+      hipLaunchKernelGGL(rs_dots, dim3(blocks(2ull * j.n_paths)), dim3(256), 0, stream_, f,
+                         j.paths.as<uint32_t>(), j.n_paths, dots.as<uint32_t>());
+      HIPCHK(hipGetLastError());
+      kdots = dots.as<uint32_t>();
+    }
+  }
+  hipLaunchKernelGGL(rs_resolve, dim3(blocks(npix)), dim3(256), 0, stream_, f, kkeys, kvals,
+                     kfirst, klast, kdots, image.as<uint32_t>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rgba, image.p, npix * 4, hipMemcpyDeviceToHost, stream_));
+  sync();
+}
+
+}  // namespace bt

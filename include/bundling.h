@@ -142,6 +142,47 @@ int bt_read_grid(bt_ctx* ctx, uint32_t* cell_of, size_t n_samples, uint32_t* cou
  * count) of them and BT_E_CAPACITY is returned when capacity < count. */
 int bt_read_endpoints(bt_ctx* ctx, uint32_t* indices, size_t capacity, uint32_t* count);
 
+/* ---------------------------------------------------------------- render */
+
+/* Palette (palette.rs:13-24; VIRIDIS is the reference default). */
+#define BT_PALETTE_CIVIDIS 0u
+#define BT_PALETTE_VIRIDIS 1u
+#define BT_PALETTE_PLASMA 2u
+#define BT_PALETTE_BATLOW 3u
+/* ColorBy (palette.rs:3-10): the normalized density map, or the edge id. */
+#define BT_COLOR_DENSITY 0u
+#define BT_COLOR_EDGE_ID 1u
+
+/* Options of the path rasterizer (render_paths.wgsl, gpu.rs:234-335,572-613;
+ * main.rs:58-81 --palette, --color-by, --endpoint-dots, --msaa). */
+typedef struct bt_render_options {
+  uint32_t palette;        /* BT_PALETTE_* */
+  uint32_t color_by;       /* BT_COLOR_*; DENSITY needs BT_FLAG_DENSITY */
+  uint32_t msaa;           /* 2 or 4 samples per pixel */
+  uint32_t endpoint_dots;  /* nonzero: dots at path ends (endpoints.wgsl) */
+  uint32_t reserved[4];
+} bt_render_options;
+
+/* The 16 RGB stops of a palette (colors48 = 16 x {r, g, b}) and its endpoint
+ * dot color (rgb3), palette.rs:26-130. */
+int bt_palette(uint32_t palette, float* colors48, float* rgb3);
+
+/* viridis, density, 4x MSAA, no dots (the reference's CLI defaults). */
+void bt_render_options_default(bt_render_options* ro);
+
+/* Render target size: the map (gpu.rs:99-102,133-134), i.e. the density
+ * map's width and height. */
+int bt_render_size(bt_ctx* ctx, uint32_t* width, uint32_t* height);
+
+/* Gpu::render + Gpu::readback (gpu.rs:572-651): draws the current paths
+ * (one quad of half-width 0.001 clip per segment, opacity 0.1, alpha
+ * blending), then the endpoint outlines and dots, over the clear color
+ * (0.025, 0.025, 0.035), resolves the MSAA samples and copies the
+ * width x height RGBA8 image (row 0 = top) into rgba (n_bytes = w * h * 4).
+ * With no uploaded graph it returns the blank frame that calibrates the
+ * noisiness background (main.rs:117-123). */
+int bt_render(bt_ctx* ctx, const bt_render_options* ro, uint8_t* rgba, size_t n_bytes);
+
 int bt_get_info(bt_ctx* ctx, bt_info* info);
 
 /* Parameters iteration `iteration` of a job with these options runs with
@@ -202,6 +243,24 @@ int bt_trl_parse(const char* text, size_t len, float** xy, uint32_t* n_points,
                  uint32_t** starts, uint32_t* n_starts, float* bbox, char* err,
                  size_t errlen);
 void bt_free(void* p);
+
+/* noisiness::background (noisiness.rs:28-42): the uniform RGB of an opaque
+ * blank frame. err receives the reference's message on failure. */
+int bt_noisiness_background(const uint8_t* rgba, uint32_t width, uint32_t height, uint8_t* rgb,
+                            char* err, size_t errlen);
+
+/* noisiness::measure (noisiness.rs:44-74): pixels whose RGB differs exactly
+ * from rgb. ratio = colored / background, NaN when background == 0. */
+typedef struct bt_noisiness {
+  uint64_t colored_pixels, background_pixels, total_pixels;
+  double noisiness, colored_to_background_ratio;
+} bt_noisiness;
+int bt_noisiness_measure(const uint8_t* rgba, uint32_t width, uint32_t height,
+                         const uint8_t* rgb, bt_noisiness* out, char* err, size_t errlen);
+
+/* write_image (main.rs:193-221): an 8-bit RGBA PNG (zlib deflate, filter 0). */
+int bt_write_png(const char* path, const uint8_t* rgba, uint32_t width, uint32_t height,
+                 char* err, size_t errlen);
 
 /* Graph::normalization (graph.rs:79-91): out = {min_x, min_y, scale, tr_x, tr_y}. */
 int bt_normalization(const float* bbox, float* out);

@@ -779,3 +779,268 @@ int or_distortion(const float* original, const float* current, const uint64_t* o
     *distortion = r / (2.0 * 1.4142135623730951);
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Export image (render_paths.wgsl:1-97, gpu.rs:572-651, palette.rs).
+ * Restated sequentially: every segment of every path in Layout order, its
+ * first triangle then its second, blended sample by sample into 8-bit
+ * storage; then the endpoint outlines and dots; then the box resolve.
+ * Rasterization contract: corners in f32 as the vertex shader computes
+ * them, viewport ((x*0.5+0.5)*W, (0.5-y*0.5)*H), snapped to 1/256 px;
+ * integer edge functions, top-left fill rule, no culling; standard 2x / 4x
+ * sample locations; fragment colour evaluated at the pixel center; dots
+ * shaded per sample.
+ * ---------------------------------------------------------------------- */
+
+static const int OR_S4[4][2] = {{96, 32}, {224, 96}, {32, 160}, {160, 224}};
+static const int OR_S2[2][2] = {{192, 192}, {64, 64}};
+
+float or_half_round(float x) {
+    /* nearest binary16 value, ties to even, as an f32 */
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    const uint32_t sign = u & 0x80000000u, a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return x;                 /* inf / nan */
+    float r;
+    if (a < 0x38800000u) {                          /* below 2^-14: quantum 2^-24 */
+        float m;
+        memcpy(&m, &a, 4);
+        r = rintf(m * 16777216.0f) * (1.0f / 16777216.0f);
+    } else {
+        uint32_t b = (a + 0x0fffu + ((a >> 13) & 1u)) & ~0x1fffu;
+        if (b >= 0x477ff000u) b = 0x7f800000u;      /* >= 65520 rounds to inf */
+        memcpy(&r, &b, 4);
+    }
+    uint32_t ru;
+    memcpy(&ru, &r, 4);
+    ru |= sign;
+    memcpy(&r, &ru, 4);
+    return r;
+}
+
+static uint8_t or_unorm8(float x) {
+    if (x < 0.0f) x = 0.0f;
+    if (x > 1.0f) x = 1.0f;
+    return (uint8_t)rintf(x * 255.0f);
+}
+
+typedef struct {
+    long long dx[3], dy[3], ax[3], ay[3], bias[3];
+    int live;
+} or_tri;
+
+static void or_tri_make(const int* X, const int* Y, int a, int b, int c, or_tri* t) {
+    long long px[3] = {X[a], X[b], X[c]}, py[3] = {Y[a], Y[b], Y[c]};
+    long long area = (px[1] - px[0]) * (py[2] - py[0]) - (py[1] - py[0]) * (px[2] - px[0]);
+    t->live = area != 0;
+    if (area < 0) { /* make the inside the positive side of every edge */
+        long long sx = px[1], sy = py[1];
+        px[1] = px[2]; py[1] = py[2];
+        px[2] = sx; py[2] = sy;
+    }
+    for (int e = 0; e < 3; e++) {
+        int n = (e + 1) % 3;
+        t->ax[e] = px[e];
+        t->ay[e] = py[e];
+        t->dx[e] = px[n] - px[e];
+        t->dy[e] = py[n] - py[e];
+        /* y grows downwards: a left edge runs up (dy < 0), a top edge runs
+         * right (dy == 0, dx > 0) */
+        int top_left = t->dy[e] < 0 || (t->dy[e] == 0 && t->dx[e] > 0);
+        t->bias[e] = top_left ? 0 : 1;
+    }
+}
+
+static int or_tri_covers(const or_tri* t, long long sx, long long sy) {
+    if (!t->live) return 0;
+    for (int e = 0; e < 3; e++) {
+        long long E = t->dx[e] * (sy - t->ay[e]) - t->dy[e] * (sx - t->ax[e]);
+        if (E < t->bias[e]) return 0;
+    }
+    return 1;
+}
+
+/* Corners of segment (p, q): 0 = p + n, 1 = p - n, 2 = q + n, 3 = q - n
+ * (render_paths.wgsl:36-50, CORNERS), in 1/256 px. */
+static int or_corners(const float* p, const float* q, uint32_t w, uint32_t h, int* X, int* Y) {
+    float dx = q[0] - p[0], dy = q[1] - p[1];
+    float len = sqrtf(dx * dx + dy * dy);
+    float k = 0.001f / (len > 1e-10f ? len : 1e-10f);
+    float nx = -dy * k, ny = dx * k;
+    float cx[4] = {p[0] + nx, p[0] - nx, q[0] + nx, q[0] - nx};
+    float cy[4] = {p[1] + ny, p[1] - ny, q[1] + ny, q[1] - ny};
+    for (int c = 0; c < 4; c++) {
+        float fx = (cx[c] * 0.5f + 0.5f) * (float)w;
+        float fy = (0.5f - cy[c] * 0.5f) * (float)h;
+        if (!(fabsf(fx) < 1.0e6f && fabsf(fy) < 1.0e6f)) return 0;
+        X[c] = (int)rintf(fx * 256.0f);
+        Y[c] = (int)rintf(fy * 256.0f);
+    }
+    return 1;
+}
+
+static void or_palette(const float* pal, float value, float* rgb) {
+    float stop = value * 15.0f;
+    uint32_t i = stop <= 0.0f ? 0u : (uint32_t)stop;
+    if (i > 14u) i = 14u;
+    float t = stop - (float)i;
+    for (int c = 0; c < 3; c++) rgb[c] = pal[3 * i + c] * (1.0f - t) + pal[3 * (i + 1) + c] * t;
+}
+
+static float or_density_at(const or_render_in* in, uint32_t px, uint32_t py) {
+    float tx = ((float)px + 0.5f) / (float)in->w * (float)(in->dw - 1);
+    float ty = ((float)py + 0.5f) / (float)in->h * (float)(in->dh - 1);
+    float fx = floorf(tx), fy = floorf(ty);
+    float ax = rintf((tx - fx) * 256.0f) * (1.0f / 256.0f);
+    float ay = rintf((ty - fy) * 256.0f) * (1.0f / 256.0f);
+    uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
+    if (x0 > in->dw - 1) x0 = in->dw - 1;
+    if (y0 > in->dh - 1) y0 = in->dh - 1;
+    uint32_t x1 = x0 + 1 < in->dw ? x0 + 1 : in->dw - 1;
+    uint32_t y1 = y0 + 1 < in->dh ? y0 + 1 : in->dh - 1;
+    const float* d = in->density;
+    float a = or_half_round(d[(size_t)y0 * in->dw + x0]), b = or_half_round(d[(size_t)y0 * in->dw + x1]);
+    float c = or_half_round(d[(size_t)y1 * in->dw + x0]), e = or_half_round(d[(size_t)y1 * in->dw + x1]);
+    float top = a * (1.0f - ax) + b * ax;
+    float bot = c * (1.0f - ax) + e * ax;
+    float v = top * (1.0f - ay) + bot * ay;
+    return v < 0.0f ? 0.0f : v > 1.0f ? 1.0f : v;
+}
+
+/* Visits every (segment, triangle) fragment sample in draw order. */
+typedef void (*or_frag_fn)(void* ctx, uint32_t seg, uint32_t px, uint32_t py, uint32_t k);
+
+static void or_raster(const or_render_in* in, or_frag_fn fn, void* ctx) {
+    const int (*S)[2] = in->msaa == 4 ? OR_S4 : OR_S2;
+    for (uint32_t p = 0; p < in->n_paths; p++) {
+        uint32_t off = in->paths[3 * p], cnt = in->paths[3 * p + 2];
+        for (uint32_t s = off; cnt >= 2 && s + 1 < off + cnt; s++) {
+            int X[4], Y[4];
+            if (!or_corners(in->pts + 2 * (size_t)s, in->pts + 2 * (size_t)(s + 1), in->w, in->h, X, Y))
+                continue;
+            or_tri tris[2];
+            or_tri_make(X, Y, 0, 1, 2, &tris[0]); /* strip triangle 0 */
+            or_tri_make(X, Y, 2, 1, 3, &tris[1]); /* strip triangle 1 */
+            int mnx = X[0], mxx = X[0], mny = Y[0], mxy = Y[0];
+            for (int c = 1; c < 4; c++) {
+                if (X[c] < mnx) mnx = X[c];
+                if (X[c] > mxx) mxx = X[c];
+                if (Y[c] < mny) mny = Y[c];
+                if (Y[c] > mxy) mxy = Y[c];
+            }
+            long long x0 = (long long)floor((mnx - 256.0) / 256.0), x1 = (long long)floor(mxx / 256.0);
+            long long y0 = (long long)floor((mny - 256.0) / 256.0), y1 = (long long)floor(mxy / 256.0);
+            if (x0 < 0) x0 = 0;
+            if (y0 < 0) y0 = 0;
+            if (x1 > (long long)in->w - 1) x1 = (long long)in->w - 1;
+            if (y1 > (long long)in->h - 1) y1 = (long long)in->h - 1;
+            for (int t = 0; t < 2; t++)
+                for (long long py = y0; py <= y1; py++)
+                    for (long long px = x0; px <= x1; px++)
+                        for (uint32_t k = 0; k < in->msaa; k++)
+                            if (or_tri_covers(&tris[t], px * 256 + S[k][0], py * 256 + S[k][1]))
+                                fn(ctx, s, (uint32_t)px, (uint32_t)py, k);
+        }
+    }
+}
+
+static void or_count_frag(void* ctx, uint32_t seg, uint32_t px, uint32_t py, uint32_t k) {
+    const or_render_in* in = ((const or_render_in**)ctx)[0];
+    uint32_t* counts = ((uint32_t**)ctx)[1];
+    counts[((size_t)py * in->w + px) * in->msaa + k]++;
+}
+
+void or_coverage(const or_render_in* in, uint32_t* counts) {
+    memset(counts, 0, (size_t)in->w * in->h * in->msaa * 4);
+    void* ctx[2] = {(void*)in, counts};
+    or_raster(in, or_count_frag, ctx);
+}
+
+typedef struct {
+    const or_render_in* in;
+    uint8_t* s; /* w*h*msaa*3 */
+} or_blend_ctx;
+
+static void or_blend_frag(void* vctx, uint32_t seg, uint32_t px, uint32_t py, uint32_t k) {
+    or_blend_ctx* c = (or_blend_ctx*)vctx;
+    const or_render_in* in = c->in;
+    float rgb[3];
+    if (in->density) {
+        or_palette(in->palette, or_density_at(in, px, py), rgb);
+    } else {
+        uint32_t e = in->n_edges > 2 ? in->n_edges : 2;
+        or_palette(in->palette, (float)in->edge[seg] / (float)(e - 1), rgb);
+    }
+    uint8_t* v = c->s + (((size_t)py * in->w + px) * in->msaa + k) * 3;
+    for (int ch = 0; ch < 3; ch++)
+        v[ch] = or_unorm8(rgb[ch] * 0.1f + ((float)v[ch] * (1.0f / 255.0f)) * (1.0f - 0.1f));
+}
+
+int or_render(const or_render_in* in, uint8_t* rgba) {
+    if (in->msaa != 2 && in->msaa != 4) return 1;
+    const size_t npix = (size_t)in->w * in->h;
+    uint8_t* s = (uint8_t*)malloc(npix * in->msaa * 3);
+    uint8_t* dot = (uint8_t*)calloc(npix * in->msaa, 1); /* 1 outline, 2 dot */
+    if (!s || !dot) {
+        free(s);
+        free(dot);
+        return 1;
+    }
+    const float clear[3] = {0.025f, 0.025f, 0.035f};
+    for (size_t i = 0; i < npix * in->msaa; i++)
+        for (int ch = 0; ch < 3; ch++) s[3 * i + ch] = or_unorm8(clear[ch]);
+    or_blend_ctx bc = {in, s};
+    or_raster(in, or_blend_frag, &bc);
+    if (in->dots) {
+        const int (*S)[2] = in->msaa == 4 ? OR_S4 : OR_S2;
+        const float inner = 1.0f - 1.0f / 3.0f;
+        for (uint32_t p = 0; p < in->n_paths; p++) {
+            uint32_t off = in->paths[3 * p], cnt = in->paths[3 * p + 2];
+            if (cnt < 2) continue;
+            for (int end = 0; end < 2; end++) {
+                const float* q = in->pts + 2 * (size_t)(end ? off + cnt - 1 : off);
+                float cx = (q[0] * 0.5f + 0.5f) * (float)in->w;
+                float cy = (0.5f - q[1] * 0.5f) * (float)in->h;
+                if (!(fabsf(cx) < 1.0e6f && fabsf(cy) < 1.0e6f)) continue;
+                long long ya = (long long)floorf(cy) - 5, yb = (long long)floorf(cy) + 5;
+                long long xa = (long long)floorf(cx) - 5, xb = (long long)floorf(cx) + 5;
+                if (ya < 0) ya = 0;
+                if (xa < 0) xa = 0;
+                if (yb > (long long)in->h - 1) yb = (long long)in->h - 1;
+                if (xb > (long long)in->w - 1) xb = (long long)in->w - 1;
+                for (long long py = ya; py <= yb; py++) {
+                    for (long long px = xa; px <= xb; px++) {
+                        for (uint32_t k = 0; k < in->msaa; k++) {
+                            float ox = ((float)px + (float)S[k][0] * (1.0f / 256.0f) - cx) / 3.0f;
+                            float oy = ((float)py + (float)S[k][1] * (1.0f / 256.0f) - cy) / 3.0f;
+                            if (!(ox >= -1.0f && ox < 1.0f && oy >= -1.0f && oy < 1.0f)) continue;
+                            float d2 = ox * ox + oy * oy;
+                            uint8_t* f = &dot[((size_t)py * in->w + px) * in->msaa + k];
+                            if (d2 <= inner * inner) *f = 2;
+                            else if (d2 <= 1.0f && *f == 0) *f = 1;
+                        }
+                    }
+                }
+            }
+        }
+        uint8_t ring[3], fill[3];
+        for (int ch = 0; ch < 3; ch++) {
+            ring[ch] = or_unorm8(in->palette[ch] * 0.1f);
+            fill[ch] = or_unorm8(in->endpoint[ch]);
+        }
+        for (size_t i = 0; i < npix * in->msaa; i++)
+            if (dot[i])
+                for (int ch = 0; ch < 3; ch++) s[3 * i + ch] = dot[i] == 2 ? fill[ch] : ring[ch];
+    }
+    for (size_t p = 0; p < npix; p++) {
+        for (int ch = 0; ch < 3; ch++) {
+            uint32_t sum = 0;
+            for (uint32_t k = 0; k < in->msaa; k++) sum += s[(p * in->msaa + k) * 3 + ch];
+            rgba[4 * p + ch] = (uint8_t)((sum + in->msaa / 2) / in->msaa);
+        }
+        rgba[4 * p + 3] = 255;
+    }
+    free(s);
+    free(dot);
+    return 0;
+}

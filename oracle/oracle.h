@@ -142,6 +142,29 @@ float or_iterate(const float* in, const uint8_t* fixed, uint32_t n, const or_gri
                  uint32_t* counts, float* tmp, float* blurred, float* normalized,
                  int threads);
 
+/* Export image (render_paths.wgsl:1-97, gpu.rs:572-651). pts: n x {x, y}
+ * clip units in Layout order; edge: n edge ids; paths: n_paths x {offset,
+ * segment offset, count}; density: dw x dh normalized map for density
+ * colouring, NULL for edge-id colouring; palette 16 x 3; endpoint 3. */
+typedef struct {
+    const float* pts;
+    const uint32_t* edge;
+    uint32_t n, n_edges;
+    const uint32_t* paths;
+    uint32_t n_paths;
+    const float* density;
+    uint32_t dw, dh, w, h, msaa, dots;
+    const float* palette;
+    const float* endpoint;
+} or_render_in;
+
+/* w x h RGBA8, row 0 at the top. Returns nonzero on bad input. */
+int or_render(const or_render_in* in, uint8_t* rgba);
+/* Fragments per sample (w*h*msaa, sample k of pixel p at p*msaa+k). */
+void or_coverage(const or_render_in* in, uint32_t* counts);
+/* Nearest binary16 value (ties to even) of x, as an f32. */
+float or_half_round(float x);
+
 #ifdef __cplusplus
 }
 #endif

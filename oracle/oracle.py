@@ -65,6 +65,9 @@ def lib():
             "or_normalize": (None, [_PF, C.c_size_t, C.c_float, _PF]),
             "or_distortion": (C.c_int, [_PF, _PF, _PU64, _PU32, C.c_uint32,
                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "or_render": (C.c_int, [C.POINTER(OrRenderIn), _PU8]),
+            "or_coverage": (None, [C.POINTER(OrRenderIn), _PU32]),
+            "or_half_round": (C.c_float, [C.c_float]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -255,6 +258,69 @@ def endpoints(points, paths):
     key = (key[:, 0] << np.uint64(32)) | key[:, 1]
     _, first = np.unique(key, return_index=True)  # first occurrence = smallest index
     return np.sort(cand[first]).astype(np.uint32)
+
+
+class OrRenderIn(C.Structure):
+    _fields_ = [("pts", _PF), ("edge", _PU32), ("n", C.c_uint32), ("n_edges", C.c_uint32),
+                ("paths", _PU32), ("n_paths", C.c_uint32), ("density", _PF),
+                ("dw", C.c_uint32), ("dh", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
+                ("msaa", C.c_uint32), ("dots", C.c_uint32), ("palette", _PF),
+                ("endpoint", _PF)]
+
+
+def _render_in(points, edge_ids, paths, w, h, palette, endpoint, density=None, msaa=4,
+               dots=False, n_edges=None):
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    xy = arr(np.asarray(points, dtype=np.float32).reshape(-1, 2), np.float32)
+    ids = arr(np.asarray(edge_ids).reshape(-1), np.uint32)
+    pa = arr(np.asarray(paths).reshape(-1, 3), np.uint32)
+    pal = arr(np.asarray(palette).reshape(16, 3), np.float32)
+    ep = arr(np.asarray(endpoint).reshape(3), np.float32)
+    r = OrRenderIn()
+    r.pts, r.edge, r.n = _f(xy), _u(ids), len(xy)
+    r.n_edges = len(pa) if n_edges is None else n_edges
+    r.paths, r.n_paths = _u(pa), len(pa)
+    if density is not None:
+        d = arr(density, np.float32)
+        r.density, r.dh, r.dw = _f(d), d.shape[0], d.shape[1]
+    r.w, r.h, r.msaa, r.dots = w, h, msaa, int(bool(dots))
+    r.palette, r.endpoint = _f(pal), _f(ep)
+    return r, keep
+
+
+def render(points, edge_ids, paths, w, h, palette, endpoint, density=None, msaa=4, dots=False,
+           n_edges=None):
+    """The export image (render_paths.wgsl, gpu.rs:572-651) as (h, w, 4) uint8.
+    density: the normalized (dh, dw) map for density colouring, None for
+    edge-id colouring."""
+    r, keep = _render_in(points, edge_ids, paths, w, h, palette, endpoint, density, msaa, dots,
+                         n_edges)
+    out = np.zeros((h, w, 4), dtype=np.uint8)
+    if lib().or_render(C.byref(r), out.ctypes.data_as(_PU8)):
+        raise ValueError("invalid render input")
+    return out
+
+
+def coverage(points, paths, w, h, msaa=4):
+    """Fragments per sample, (h, w, msaa) uint32."""
+    pal = np.zeros((16, 3), np.float32)
+    r, keep = _render_in(points, np.zeros(len(points), np.uint32), paths, w, h, pal,
+                         np.zeros(3, np.float32), None, msaa)
+    out = np.zeros((h, w, msaa), dtype=np.uint32)
+    lib().or_coverage(C.byref(r), _u(out))
+    return out
+
+
+def half_round(x):
+    """Nearest binary16 values (ties to even) as float32."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    return np.array([lib().or_half_round(float(v)) for v in x], dtype=np.float32)
 
 
 class Pipeline:

@@ -294,6 +294,23 @@ def test_endpoints_match_oracle(bt, orc, cpu_case, case):
         assert rc == api.BT_E_CAPACITY and count.value == len(want) and small[0] == want[0]
 
 
+def test_density_disabled_matches_enabled(bt, cpu_case):
+    """color_by = edge id (no density buffers): the same positions, bit for
+    bit, as with the density map on."""
+    res = []
+    for dens in (True, False):
+        with bt.Gpu(bt.Options(map_size=201, neighbor_radius=10, color_by_density=dens)) as gpu:
+            gpu.upload(cpu_case)
+            gpu.iterate()
+            gpu.iterate()
+            res.append(gpu.points()[0])
+            if not dens:
+                with pytest.raises(bt.BtError) as e:
+                    gpu.density(3)
+                assert e.value.code == 5
+    np.testing.assert_array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+
+
 def test_error_paths(bt, cpu_case):
     with bt.Gpu(bt.Options(map_size=201)) as gpu:
         with pytest.raises(bt.BtError) as e:

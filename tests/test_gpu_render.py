@@ -1,0 +1,88 @@
+"""GPU parity of the export image (render_paths.wgsl, gpu.rs:572-651): the HIP
+rasterizer through the C ABI against the oracle's sequential restatement,
+byte for byte, and the noisiness of both (noisiness.rs)."""
+import numpy as np
+import pytest
+
+from bundling_thingy_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_image(orc, bt, pl, w, h, palette, color_by, msaa, dots):
+    pal, ep = bt.api.palette(palette)
+    paths = np.stack([pl.offsets.astype(np.uint32), np.zeros_like(pl.counts), pl.counts], 1)
+    dens = pl.normalized if color_by == bt.COLOR_DENSITY else None
+    return orc.render(pl.points, pl.edge_id, paths, w, h, pal, ep, density=dens, msaa=msaa,
+                      dots=dots)
+
+
+def _check(gpu, orc, bt, pl, palette, color_by, msaa, dots, label):
+    w, h = gpu.render_size()
+    got = gpu.render(palette, color_by, msaa, dots)
+    want = _oracle_image(orc, bt, pl, w, h, palette, color_by, msaa, dots)
+    diff = np.argwhere((got != want).any(2))
+    assert len(diff) == 0, f"{label}: {len(diff)} pixels differ, first {diff[:5].tolist()}"
+    return got
+
+
+def test_blank_frame(bt):
+    with bt.Gpu(workloads.reference_cpu_options()) as gpu:
+        img = gpu.render()
+        assert img.shape == (201, 201, 4)
+        assert bt.noisiness_background(img) == (6, 6, 9)
+        assert bt.noisiness(img, (6, 6, 9))["colored_pixels"] == 0
+
+
+@pytest.mark.parametrize("palette,color_by,msaa,dots", [
+    ("viridis", 0, 4, False),
+    ("viridis", 1, 4, False),
+    ("plasma", 0, 2, True),
+    ("batlow", 1, 2, False),
+    ("cividis", 1, 4, True),
+])
+def test_reference_mode_image_bitexact(bt, orc, palette, color_by, msaa, dots):
+    """Reference mode S = 201, r = 10 (the plain CPU case): after upload and
+    after two iterations."""
+    g = workloads.plain_cpu_graph()
+    opts = workloads.reference_cpu_options()
+    pl = orc.Pipeline(g.points, g.starts, g.bbox, opts.iter_params(0))
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(g)
+        _check(gpu, orc, bt, pl, palette, color_by, msaa, dots, "upload")
+        for it in range(2):
+            gpu.iterate()
+            pl.iterate(opts.iter_params(it))
+        img = _check(gpu, orc, bt, pl, palette, color_by, msaa, dots, "iteration 2")
+        m = bt.noisiness(img, (6, 6, 9))
+        assert 0 < m["noisiness"] < 1
+
+
+def test_gpu_case_image_and_noisiness(bt, orc):
+    """The plain GPU workload (9,800 edges, ~1.3M samples) on its 1024^2 map,
+    density colouring, 4x MSAA: byte-identical images after upload and after
+    one iteration at the schedule's smallest radius (0.01, sigma 2)."""
+    g = workloads.plain_gpu_graph()
+    opts = bt.Options(spacing=0.002, radius=0.01, sigma=2.0, density=1024)
+    pl = orc.Pipeline(g.points, g.starts, g.bbox, opts.iter_params(0))
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(g)
+        img0 = _check(gpu, orc, bt, pl, "viridis", 0, 4, False, "upload")
+        gpu.iterate()
+        pl.iterate(opts.iter_params(0), threads=0)
+        img1 = _check(gpu, orc, bt, pl, "viridis", 0, 4, False, "iteration 1")
+    n0 = bt.noisiness(img0, (6, 6, 9))["noisiness"]
+    n1 = bt.noisiness(img1, (6, 6, 9))["noisiness"]
+    assert 0 < n1 < n0 < 1  # bundling concentrates the drawing
+
+
+def test_render_errors(bt):
+    g = workloads.plain_cpu_graph()
+    with bt.Gpu(bt.Options(map_size=201, neighbor_radius=10, color_by_density=False)) as gpu:
+        gpu.upload(g)
+        with pytest.raises(bt.BtError) as e:
+            gpu.render(color_by=bt.COLOR_DENSITY)
+        assert e.value.code == 5
+        assert gpu.render(color_by=bt.COLOR_EDGE_ID).shape == (201, 201, 4)
+        with pytest.raises(bt.BtError):
+            gpu.render(color_by=bt.COLOR_EDGE_ID, msaa=8)
