@@ -179,6 +179,28 @@ def large_section(bt, workloads, steps=2, iters=50):
             "splits": chosen.tolist()}
 
 
+def render_section(bt, gpu, opts, reps=5):
+    """Export image of the bundled graph (render_paths.wgsl, gpu.rs:572-651):
+    density colouring, 4x MSAA, on the 1024^2 map; wall time of the whole
+    call (rasterize, sort, resolve, 4 MiB readback) and of the noisiness
+    measurement on the host."""
+    with bt.Gpu(opts) as blank:
+        rgb = bt.noisiness_background(blank.render())
+    gpu.render()
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        img = gpu.render()
+        times.append(time.perf_counter() - t)
+    t = time.perf_counter()
+    m = bt.noisiness(img, rgb)
+    noise_ms = (time.perf_counter() - t) * 1e3
+    ms = float(np.median(times) * 1e3)
+    log(f"render: {img.shape[1]}x{img.shape[0]} in {ms:.2f} ms, noisiness {m['noisiness']:.4f}")
+    return {"size": [img.shape[1], img.shape[0]], "msaa": 4, "color_by": "density",
+            "ms": ms, "noisiness": m["noisiness"], "noisiness_ms": noise_ms}
+
+
 def served_section(bt, workloads, n_jobs, rate_hz=200.0):
     """Poisson arrivals at rate_hz: the plain per-job form (FIFO, one job at a
     time, as the drop-in does) against the continuous-batching server."""
@@ -262,6 +284,8 @@ def main():
     ap.add_argument("--batched", type=int, default=256, help="graphs in the batched case (0: skip)")
     ap.add_argument("--served", type=int, default=100, help="jobs in the served case (0: skip)")
     ap.add_argument("--large", type=int, default=1, help="run the 100,000-edge case (0: skip)")
+    ap.add_argument("--render", type=int, default=1,
+                    help="time the export image of the bundled plain GPU case (0: skip)")
     ap.add_argument("--profile-only", action="store_true",
                     help="for rocprofv3 counter runs: a 256 MiB device copy (calibration), "
                          "then upload + autotune + one eager 30-iteration schedule")
@@ -366,6 +390,7 @@ def main():
     path = {"bytes_per_iteration": iter_bytes,
             "achieved_GBs": iter_bytes * ITERS / (total / args.steps) / 1e9}
 
+    render = render_section(bt, gpu, opts) if args.render else None
     gpu.close()
     batched = batched_section(bt, workloads, args.batched) if args.batched else None
     served = served_section(bt, workloads, args.served) if args.served else None
@@ -408,6 +433,7 @@ def main():
         "batched": batched,
         "served": served,
         "large": large,
+        "render": render,
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]

@@ -188,9 +188,33 @@ __device__ __forceinline__ void pixel_masks(const Frame& f, const Tri& t0, const
   }
 }
 
+// x-extent of edge (a, b) inside the horizontal band [yl, yh] (fixed point,
+// double precision), merged into [lo, hi].
+__device__ __forceinline__ void edge_band(const Quad& q, int a, int b, double yl, double yh,
+                                          double& lo, double& hi) {
+  const double x0 = q.x[a], y0 = q.y[a], x1 = q.x[b], y1 = q.y[b];
+  const double ya = fmin(y0, y1), yb = fmax(y0, y1);
+  if (yb < yl || ya > yh) return;
+  if (y0 == y1) {
+    lo = fmin(lo, fmin(x0, x1));
+    hi = fmax(hi, fmax(x0, x1));
+    return;
+  }
+  const double s = (x1 - x0) / (y1 - y0);
+  const double xa = x0 + (fmax(ya, yl) - y0) * s, xb = x0 + (fmin(yb, yh) - y0) * s;
+  lo = fmin(lo, fmin(xa, xb));
+  hi = fmax(hi, fmax(xa, xb));
+}
+
 // Walks the fragments of segment i in draw order: the pixels of its quad in
 // row-major order; a pixel that both triangles cover on a common sample gets
-// two fragments (first triangle first), otherwise one with both masks.
+// two fragments (first triangle first), otherwise one with both masks. Each
+// pixel row only visits the columns between the leftmost and rightmost
+// points of the triangles' edges inside the row's sample band (a closed
+// triangle meets a horizontal line between two of its edge points), widened
+// by 4/256 px against rounding; the exact per-sample test decides coverage,
+// so the bound changes work (O(length) instead of O(length^2) for long
+// segments), not results.
 template <class F>
 __device__ void walk_segment(const Frame& f, uint32_t i, F&& emit) {
   Quad q;
@@ -199,8 +223,18 @@ __device__ void walk_segment(const Frame& f, uint32_t i, F&& emit) {
   tri_setup(q.x[0], q.y[0], q.x[1], q.y[1], q.x[2], q.y[2], t0);
   tri_setup(q.x[2], q.y[2], q.x[1], q.y[1], q.x[3], q.y[3], t1);
   const Span s = quad_span(f, q);
-  for (int py = s.y0; py <= s.y1; ++py)
-    for (int px = s.x0; px <= s.x1; ++px) {
+  for (int py = s.y0; py <= s.y1; ++py) {
+    const double yl = (double)py * 256.0 + 32.0, yh = (double)py * 256.0 + 224.0;
+    double lo = 1.0e300, hi = -1.0e300;
+    edge_band(q, 0, 1, yl, yh, lo, hi);  // the edges of (c0 c1 c2) and (c2 c1 c3)
+    edge_band(q, 1, 2, yl, yh, lo, hi);
+    edge_band(q, 2, 0, yl, yh, lo, hi);
+    edge_band(q, 1, 3, yl, yh, lo, hi);
+    edge_band(q, 3, 2, yl, yh, lo, hi);
+    if (lo > hi) continue;
+    const int x0 = max(s.x0, (int)ceil((lo - 4.0 - 224.0) / 256.0));
+    const int x1 = min(s.x1, (int)floor((hi + 4.0 - 32.0) / 256.0));
+    for (int px = x0; px <= x1; ++px) {
       uint32_t m0, m1;
       pixel_masks(f, t0, t1, px, py, m0, m1);
       const uint32_t pix = (uint32_t)py * f.w + (uint32_t)px;
@@ -211,6 +245,7 @@ __device__ void walk_segment(const Frame& f, uint32_t i, F&& emit) {
         emit(pix, m0 | m1);
       }
     }
+  }
 }
 
 __global__ __launch_bounds__(256) void rs_count(Frame f, uint32_t* cnt) {

@@ -76,6 +76,22 @@ def test_gpu_case_image_and_noisiness(bt, orc):
     assert 0 < n1 < n0 < 1  # bundling concentrates the drawing
 
 
+@pytest.mark.parametrize("spacing,msaa", [(0.15, 4), (0.05, 2)])
+def test_long_segments_bitexact(bt, orc, spacing, msaa):
+    """Segments of tens to hundreds of pixels in every direction (coarse
+    spacing on a 1024^2 map): the row-bounded walk covers what the oracle's
+    full bounding-box walk covers."""
+    g = workloads.plain_cpu_graph()
+    opts = bt.Options(spacing=spacing, radius=3 * spacing, sigma=2.0, density=1024)
+    pl = orc.Pipeline(g.points, g.starts, g.bbox, opts.iter_params(0))
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(g)
+        _check(gpu, orc, bt, pl, "plasma", 1, msaa, True, "upload")
+        gpu.iterate()
+        pl.iterate(opts.iter_params(0))
+        _check(gpu, orc, bt, pl, "viridis", 0, msaa, False, "iteration 1")
+
+
 def test_render_errors(bt):
     g = workloads.plain_cpu_graph()
     with bt.Gpu(bt.Options(map_size=201, neighbor_radius=10, color_by_density=False)) as gpu:
@@ -86,3 +102,37 @@ def test_render_errors(bt):
         assert gpu.render(color_by=bt.COLOR_EDGE_ID).shape == (201, 201, 4)
         with pytest.raises(bt.BtError):
             gpu.render(color_by=bt.COLOR_EDGE_ID, msaa=8)
+
+
+def test_headless_export(bt, orc, tmp_path):
+    """main.rs headless flow: original.png + one PNG per iteration, with the
+    noisiness and distortion reports; PNGs hold exactly the rendered image and
+    the distortion agrees with the oracle's."""
+    import json
+
+    from bundling_thingy_amd.headless import headless
+    from tests.test_render_oracle import _read_png
+
+    g = workloads.plain_cpu_graph()
+    opts = workloads.reference_cpu_options()
+    res = headless(g, str(tmp_path), opts, iterations=3, evaluate_noisiness=True,
+                   evaluate_distortion=True, log=lambda *_: None)
+    assert res["images"] == ["original.png", "iteration-01.png", "iteration-02.png",
+                             "iteration-03.png"]
+    noise = json.loads((tmp_path / "noisiness.json").read_text())
+    dist = json.loads((tmp_path / "distortion.json").read_text())
+    assert noise["render"]["background_rgb"] == [6, 6, 9]
+    assert [i["iteration"] for i in noise["images"]] == [0, 1, 2, 3]
+    assert noise["images"][0]["change_from_original"] == 0.0
+    assert dist["geometry"]["edges_total"] == 2100
+    assert dist["images"][0]["distortion"] == 0.0
+    pl = orc.Pipeline(g.points, g.starts, g.bbox, opts.iter_params(0))
+    base = pl.points.copy()
+    for it in range(3):
+        pl.iterate(opts.iter_params(it))
+        d, _ = orc.distortion(base, pl.points, pl.offsets, pl.counts)
+        assert dist["images"][it + 1]["distortion"] == pytest.approx(d, rel=1e-12)
+    last = _read_png(tmp_path / "iteration-03.png")
+    want = _oracle_image(orc, bt, pl, 201, 201, "viridis", 0, 4, False)
+    np.testing.assert_array_equal(last, want)
+    assert noise["images"][3]["noisiness"] == bt.noisiness(want, (6, 6, 9))["noisiness"]

@@ -49,6 +49,22 @@ def test_parse_generated_trl_roundtrip(bt):
     np.testing.assert_array_equal(g.starts, h.starts)
 
 
+def test_parse_large_trl_matches_oracle(bt, orc, tmp_path):
+    """Loader at scale: the 100,000-edge graph (8 MB of TRL) through
+    Graph.load, equal to the generator's arrays and to the oracle's parse."""
+    from bundling_thingy_amd import workloads
+    g = workloads.large_graph()
+    path = tmp_path / "large.trl"
+    path.write_text(workloads.to_trl(g))
+    h = bt.Graph.load(path)
+    np.testing.assert_array_equal(g.points, h.points)
+    np.testing.assert_array_equal(g.starts, h.starts)
+    p, st, bbox = orc.parse_trl(path.read_text())
+    np.testing.assert_array_equal(p, h.points)
+    np.testing.assert_array_equal(st, h.starts)
+    np.testing.assert_array_equal(bbox, h.bbox)
+
+
 def test_normalization_matches_oracle(bt, orc):
     rng = np.random.default_rng(2)
     for _ in range(20):
