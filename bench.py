@@ -52,9 +52,10 @@ def cpu_baseline(graph, opts, target_s=15.0):
     pl = orc.Pipeline(graph.points, graph.starts, graph.bbox, p0)
     pts, fixed = pl.points, pl.fixed
     threads = os.cpu_count() or 1
+    threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS", threads))))
     grids = [(orc.grid(opts.iter_params(i)), opts.iter_params(i)) for i in range(ITERS)]
 
-    def schedule(stride):
+    def schedule(stride, threads=threads):
         """Returns the scaled schedule time, and the sample's own mean and
         density times."""
         total = tmean = tdens = 0.0
@@ -82,11 +83,17 @@ def cpu_baseline(graph, opts, target_s=15.0):
     stride = int(max(1, min(coarse, np.ceil(coarse * tm / max(target_s - td, 1.0)))))
     total, tm, td = schedule(stride)
     n = len(pts)
+    # the same on one thread, over a sparser sample (about a quarter of the time)
+    stride1 = int(max(stride, np.ceil(stride * threads / 4)))
+    total1, _, _ = schedule(stride1, threads=1)
     return {"value": ITERS * n / total, "unit": "sample-point updates/s", "cores": threads,
             "kind": "port",
             "sample": f"oracle neighbor means of every {stride}th sample for all {ITERS} "
                       f"radii at the initial positions (x{stride}), full splat/blur/normalize; "
-                      f"est. {total:.1f} s per schedule"}
+                      f"est. {total:.1f} s per schedule",
+            "value_1thread": ITERS * n / total1,
+            "sample_1thread": f"every {stride1}th sample, 1 thread; est. {total1:.1f} s per "
+                              f"schedule"}
 
 
 def pct(xs, q):

@@ -18,7 +18,7 @@
 #define BT_RANK_CHUNKS 8           // binning: 64-sample chunks per wave
 #define BT_RANK_BLOCK (256 * BT_RANK_CHUNKS)  // points per workgroup in binning
 #define BT_SCAN_ITEMS 4096         // cells per workgroup in the cell scan (256 x 16)
-#define BT_TASK_BLOCKS 1024        // task blocks per workgroup in task emission (256 x 4)
+#define BT_TASK_BLOCKS 256         // task blocks per workgroup in task emission (one per thread)
 #define BT_CLEAR_ITEMS 4096        // words per workgroup in the clear kernel
 #define BT_ROW_TILE_W 64           // row blur tile: 64 columns x 4 rows
 #define BT_ROW_TILE_H 4

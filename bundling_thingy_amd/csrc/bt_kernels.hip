@@ -343,13 +343,14 @@ extern "C" __global__ __launch_bounds__(256) void bt_tasks(const BtJob* __restri
   __shared__ uint32_t first[BT_TASK_BLOCKS];
   __shared__ uint32_t bstart[BT_TASK_BLOCKS];
   __shared__ uint32_t bend[BT_TASK_BLOCKS];
+  constexpr uint32_t PER = BT_TASK_BLOCKS / 256u;  // task blocks per thread
   const uint32_t nblocks = J.task_bx * J.gy;
-  const uint32_t b0 = lb * BT_TASK_BLOCKS + threadIdx.x * 4u;
+  const uint32_t b0 = lb * BT_TASK_BLOCKS + threadIdx.x * PER;
   const uint32_t* cells = J.cells;
-  uint32_t st[4], en[4], nt[4];
+  uint32_t st[PER], en[PER], nt[PER];
   uint32_t mine = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (uint32_t k = 0; k < PER; ++k) {
     const uint32_t b = b0 + (uint32_t)k;
     st[k] = en[k] = 0u;
     if (b < nblocks) {
@@ -364,8 +365,8 @@ extern "C" __global__ __launch_bounds__(256) void bt_tasks(const BtJob* __restri
   uint32_t total;
   uint32_t before = block_excl_scan(mine, part, &total);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t l = threadIdx.x * 4u + (uint32_t)k;
+  for (uint32_t k = 0; k < PER; ++k) {
+    const uint32_t l = threadIdx.x * PER + k;
     first[l] = before;
     bstart[l] = st[k];
     bend[l] = en[k];
@@ -481,35 +482,49 @@ extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_bsum(const BtJob* 
   }
 }
 
-extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_apply(
+// 1024 threads per block: each of the 16 waves scans 256 consecutive samples
+// as 4 chunks of 64 (lane l holds samples c * 64 + l), so loads and the int64
+// stores are coalesced and enough waves are resident to keep HBM busy; the
+// waves' totals and the block's offset from bsum are added last.
+constexpr uint32_t BT_PRE_APPLY_THREADS = 1024;
+extern "C" __global__ __launch_bounds__(1024) void bt_pre_scan_apply(
     const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
   JOB_PROLOGUE(BT_K_PTS);
-  __shared__ int2 lds[BT_SCAN_ITEMS];
-  __shared__ long long part[8];
-  const uint32_t first = lb * BT_SCAN_ITEMS;
-  for (uint32_t k = threadIdx.x; k < BT_SCAN_ITEMS; k += 256u) lds[k] = qf_of(J, first + k);
-  __syncthreads();
-  long long ex[16], ey[16];
-  long long sx = 0, sy = 0;
+  constexpr uint32_t WAVES = BT_PRE_APPLY_THREADS / 64u;
+  constexpr uint32_t PER_WAVE = BT_SCAN_ITEMS / WAVES, CHUNKS = PER_WAVE / 64u;
+  __shared__ long long part[2 * WAVES];
+  const uint32_t lane = LANE(), wave = threadIdx.x >> 6;
+  const uint32_t w0 = lb * BT_SCAN_ITEMS + wave * PER_WAVE;
+  long long ex[CHUNKS], ey[CHUNKS];
+  long long rx = 0, ry = 0;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    ex[k] = sx;
-    ey[k] = sy;
-    const int2 q = lds[threadIdx.x * 16 + k];
-    sx += q.x;
-    sy += q.y;
+  for (uint32_t c = 0; c < CHUNKS; ++c) {
+    const int2 q = qf_of(J, w0 + c * 64u + lane);
+    const long long ix = wave_incl_scan((long long)q.x);
+    const long long iy = wave_incl_scan((long long)q.y);
+    ex[c] = rx + ix - q.x;
+    ey[c] = ry + iy - q.y;
+    rx += __shfl(ix, 63, 64);
+    ry += __shfl(iy, 63, 64);
   }
-  long long totx, toty;
+  if (lane == 0) {
+    part[wave] = rx;
+    part[WAVES + wave] = ry;
+  }
+  __syncthreads();
   const unsigned long long* bs = reinterpret_cast<const unsigned long long*>(J.bsum);
-  const long long bx = block_excl_scan(sx, part, &totx) + (long long)bs[2u * lb];
-  const long long by = block_excl_scan(sy, part + 4, &toty) + (long long)bs[2u * lb + 1u];
+  long long ox = (long long)bs[2u * lb], oy = (long long)bs[2u * lb + 1u];
+  for (uint32_t w = 0; w < wave; ++w) {
+    ox += part[w];
+    oy += part[WAVES + w];
+  }
   const uint32_t n1 = J.n + 1u;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const uint32_t i = first + threadIdx.x * 16u + (uint32_t)k;
+  for (uint32_t c = 0; c < CHUNKS; ++c) {
+    const uint32_t i = w0 + c * 64u + lane;
     if (i < n1) {
-      J.pre_x[i] = (uint64_t)(bx + ex[k]);
-      J.pre_y[i] = (uint64_t)(by + ey[k]);
+      J.pre_x[i] = (uint64_t)(ox + ex[c]);
+      J.pre_y[i] = (uint64_t)(oy + ey[c]);
     }
   }
 }
@@ -1192,11 +1207,21 @@ extern "C" __global__ __launch_bounds__(256) void bt_blur_cols(const BtJob* __re
   const uint32_t x = x0 + tx;
   const uint32_t span = BT_COL_TILE_H + 2u * K;
   const float* src = J.dens_row;
-  for (uint32_t e = ty; e < span; e += 4u) {
-    const int gy = (int)y0 - (int)K + (int)e;
-    float v = 0.0f;
-    if (x < W && gy >= 0 && gy < (int)H) v = src[(size_t)gy * W + x];
-    tile[e][tx] = v;
+  // eight independent loads in flight per thread before the LDS stores
+  constexpr uint32_t B = 8;
+  for (uint32_t e0 = ty; e0 < span; e0 += 4u * B) {
+    float v[B];
+#pragma unroll
+    for (uint32_t b = 0; b < B; ++b) {
+      const uint32_t e = e0 + 4u * b;
+      const int gy = (int)y0 - (int)K + (int)e;
+      v[b] = (e < span && x < W && gy >= 0 && gy < (int)H) ? src[(size_t)gy * W + x] : 0.0f;
+    }
+#pragma unroll
+    for (uint32_t b = 0; b < B; ++b) {
+      const uint32_t e = e0 + 4u * b;
+      if (e < span) tile[e][tx] = v[b];
+    }
   }
   __syncthreads();
   const float* wt = J.wt;
@@ -1554,7 +1579,8 @@ void cell_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
 void pre_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_pre_scan_reduce, L256(grid), jobs, base, nj);
   hipLaunchKernelGGL(bt_pre_scan_bsum, L256((uint32_t)nj), jobs, nj);
-  hipLaunchKernelGGL(bt_pre_scan_apply, L256(grid), jobs, base, nj);
+  hipLaunchKernelGGL(bt_pre_scan_apply, dim3(grid), dim3(BT_PRE_APPLY_THREADS), 0, s, jobs, base,
+                     nj);
 }
 void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
            BtControl* ctl, BtTask* tasks_out) {
