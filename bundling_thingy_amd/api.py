@@ -110,6 +110,8 @@ SIGNATURES = {
     "bt_render_options_default": (None, [C.POINTER(bt_render_options)]),
     "bt_render_size": (C.c_int, [_P, _PU32, _PU32]),
     "bt_render": (C.c_int, [_P, C.POINTER(bt_render_options), _P, _SZ]),
+    "bt_render_job": (C.c_int, [_P, _U32, C.POINTER(bt_render_options), _P, _SZ]),
+    "bt_render_size_job": (C.c_int, [_P, _U32, _PU32, _PU32]),
     "bt_get_info": (C.c_int, [_P, C.POINTER(bt_info)]),
     "bt_iter_params_for": (C.c_int, [C.POINTER(bt_options), _U32, C.POINTER(bt_iter_params)]),
     "bt_profile": (C.c_int, [_P, _U32, _PD, _U32, _PD]),
@@ -549,12 +551,13 @@ class Gpu:
         self._check(lib().bt_read_grid(self._h, _u32p(cells), n, _u32p(counts), len(counts)))
         return cells, counts.reshape(p.gy, p.gx)
 
-    def render_size(self):
+    def render_size(self, job=0):
         w, h = C.c_uint32(), C.c_uint32()
-        self._check(lib().bt_render_size(self._h, C.byref(w), C.byref(h)))
+        self._check(lib().bt_render_size_job(self._h, job, C.byref(w), C.byref(h)))
         return w.value, h.value
 
-    def render(self, palette="viridis", color_by=COLOR_DENSITY, msaa=4, endpoint_dots=False):
+    def render(self, palette="viridis", color_by=COLOR_DENSITY, msaa=4, endpoint_dots=False,
+               job=0):
         """Gpu::render + Gpu::readback (gpu.rs:572-651): the export image as a
         (height, width, 4) uint8 RGBA array, row 0 at the top. Without an
         uploaded graph: the blank frame."""
@@ -563,9 +566,10 @@ class Gpu:
         ro.color_by = int(color_by)
         ro.msaa = int(msaa)
         ro.endpoint_dots = int(bool(endpoint_dots))
-        w, h = self.render_size()
+        w, h = self.render_size(job)
         out = np.zeros((h, w, 4), dtype=np.uint8)
-        self._check(lib().bt_render(self._h, C.byref(ro), out.ctypes.data_as(_P), out.nbytes))
+        self._check(lib().bt_render_job(self._h, job, C.byref(ro), out.ctypes.data_as(_P),
+                                        out.nbytes))
         return out
 
     def endpoints(self):

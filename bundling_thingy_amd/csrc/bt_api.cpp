@@ -198,11 +198,22 @@ int bt_render_size(bt_ctx* ctx, uint32_t* width, uint32_t* height) {
 }
 
 int bt_render(bt_ctx* ctx, const bt_render_options* ro, uint8_t* rgba, size_t n_bytes) {
+  return bt_render_job(ctx, 0, ro, rgba, n_bytes);
+}
+
+int bt_render_job(bt_ctx* ctx, uint32_t job, const bt_render_options* ro, uint8_t* rgba,
+                  size_t n_bytes) {
   CTX_CHECK();
   if (!rgba) return BT_E_INVALID;
   bt_render_options d;
   bt_render_options_default(&d);
-  return guarded(&ctx->err, [&] { ctx->engine->render(ro ? *ro : d, rgba, n_bytes); });
+  return guarded(&ctx->err, [&] { ctx->engine->render(ro ? *ro : d, rgba, n_bytes, job); });
+}
+
+int bt_render_size_job(bt_ctx* ctx, uint32_t job, uint32_t* width, uint32_t* height) {
+  CTX_CHECK();
+  if (!width || !height) return BT_E_INVALID;
+  return guarded(&ctx->err, [&] { ctx->engine->render_size(width, height, job); });
 }
 
 int bt_job_samples(bt_ctx* ctx, uint32_t job, uint32_t* samples, uint32_t* iteration) {

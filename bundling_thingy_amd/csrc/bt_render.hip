@@ -409,10 +409,11 @@ uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255u) / 256u); }
 
 }  // namespace
 
-void Engine::render_size(uint32_t* w, uint32_t* h) {
+void Engine::render_size(uint32_t* w, uint32_t* h, uint32_t job_i) {
   if (!jobs_.empty()) {
-    *w = jobs_[0]->dw;
-    *h = jobs_[0]->dh;
+    const Job& j = job(job_i);
+    *w = j.dw;
+    *h = j.dh;
     return;
   }
   bt_iter_params p;
@@ -421,12 +422,12 @@ void Engine::render_size(uint32_t* w, uint32_t* h) {
   *h = p.density_h;
 }
 
-void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes) {
+void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, uint32_t job_i) {
   if (ro.msaa != 2u && ro.msaa != 4u) fail(BT_E_INVALID, "MSAA must be 2 or 4 samples");
   if (ro.palette > BT_PALETTE_BATLOW) fail(BT_E_INVALID, "unknown palette");
   if (ro.color_by > BT_COLOR_EDGE_ID) fail(BT_E_INVALID, "unknown color_by");
   Frame f{};
-  render_size(&f.w, &f.h);
+  render_size(&f.w, &f.h, job_i);
   if (f.w == 0 || f.h == 0) fail(BT_E_INVALID, "image dimensions must be nonzero");
   if ((uint64_t)f.w * f.h > (1ull << kMaxPixelBits))
     fail(BT_E_CAPACITY, "render target exceeds 2^24 pixels");
@@ -442,8 +443,8 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes) 
   image.alloc(npix * 4);
   const uint32_t *kkeys = nullptr, *kvals = nullptr, *kfirst = nullptr, *klast = nullptr,
                  *kdots = nullptr;
-  if (!jobs_.empty() && jobs_[0]->n > 0) {
-    Job& j = *jobs_[0];
+  if (!jobs_.empty() && job(job_i).n > 0) {
+    Job& j = job(job_i);
     if (ro.color_by == BT_COLOR_DENSITY && !j.dens_norm.p)
       fail(BT_E_STATE, "density is disabled (color_by != density)");
     f.n = j.n;
@@ -476,9 +477,22 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes) 
     temp.alloc(std::max<size_t>(tb, 256));
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(temp.p, tb, cnt.as<uint32_t>(), off.as<uint32_t>(),
                                             f.n + 1, stream_));
-    uint32_t m = 0;
-    HIPCHK(hipMemcpyAsync(&m, off.as<uint32_t>() + f.n, 4, hipMemcpyDeviceToHost, stream_));
+    // the 32-bit offsets (and the sort's int count) hold up to 2^31 - 1
+    // fragments: check the total in 64 bits before trusting them
+    DevBuf total;
+    total.pool = pool_.get();
+    total.alloc(8);
+    size_t rb = 0;
+    HIPCHK(hipcub::DeviceReduce::Sum(nullptr, rb, cnt.as<uint32_t>(), total.as<uint64_t>(),
+                                     f.n + 1, stream_));
+    temp.reserve(rb);
+    HIPCHK(hipcub::DeviceReduce::Sum(temp.p, rb, cnt.as<uint32_t>(), total.as<uint64_t>(),
+                                     f.n + 1, stream_));
+    uint64_t m64 = 0;
+    HIPCHK(hipMemcpyAsync(&m64, total.p, 8, hipMemcpyDeviceToHost, stream_));
     sync();
+    if (m64 > 0x7fffffffull) fail(BT_E_CAPACITY, "frame exceeds 2^31 - 1 fragments");
+    const uint32_t m = (uint32_t)m64;
     if (m > 0) {
       keys.alloc((size_t)m * 4);
       vals.alloc((size_t)m * 4);

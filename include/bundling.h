@@ -183,6 +183,11 @@ int bt_render_size(bt_ctx* ctx, uint32_t* width, uint32_t* height);
  * noisiness background (main.rs:117-123). */
 int bt_render(bt_ctx* ctx, const bt_render_options* ro, uint8_t* rgba, size_t n_bytes);
 
+/* The same for one job of a batched context (bt_upload_batch). */
+int bt_render_job(bt_ctx* ctx, uint32_t job, const bt_render_options* ro, uint8_t* rgba,
+                  size_t n_bytes);
+int bt_render_size_job(bt_ctx* ctx, uint32_t job, uint32_t* width, uint32_t* height);
+
 int bt_get_info(bt_ctx* ctx, bt_info* info);
 
 /* Parameters iteration `iteration` of a job with these options runs with

@@ -92,6 +92,28 @@ def test_long_segments_bitexact(bt, orc, spacing, msaa):
         _check(gpu, orc, bt, pl, "viridis", 0, msaa, False, "iteration 1")
 
 
+def test_batched_job_images_equal_alone(bt):
+    """bt_render_job on a batched context: each job's image equals the image
+    of the same graph uploaded alone (non-square 320 x 200 map included)."""
+    graphs = workloads.batched_graphs(count=3, lo=50, hi=400)
+    opts = [bt.Options(map_size=301, neighbor_radius=8),
+            bt.Options(spacing=0.003, radius=0.03, sigma=3.0, density=(320, 200)),
+            workloads.plain_cpu_options()]
+    alone = []
+    for g, o in zip(graphs, opts):
+        with bt.Gpu(o) as gpu:
+            gpu.upload(g)
+            gpu.iterate(2)
+            alone.append(gpu.render("plasma", 0, 4, True))
+    with bt.Gpu(opts[0]) as gpu:
+        gpu.upload_batch(graphs, opts)
+        gpu.iterate(2)
+        for j in range(len(graphs)):
+            got = gpu.render("plasma", 0, 4, True, job=j)
+            assert got.shape == alone[j].shape
+            np.testing.assert_array_equal(got, alone[j])
+
+
 def test_render_errors(bt):
     g = workloads.plain_cpu_graph()
     with bt.Gpu(bt.Options(map_size=201, neighbor_radius=10, color_by_density=False)) as gpu:
