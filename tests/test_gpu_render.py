@@ -114,6 +114,34 @@ def test_batched_job_images_equal_alone(bt):
             np.testing.assert_array_equal(got, alone[j])
 
 
+def test_full_schedule_distortion_and_noisiness(bt, orc):
+    """The acceptance criterion after a whole schedule: the plain CPU case's
+    10 iterations (radius 0.05, 512^2, sigma 4). Distortion and noisiness
+    must agree with the oracle's within 1 % (they agree exactly: positions,
+    density and image are bit-identical)."""
+    g = workloads.plain_cpu_graph()
+    opts = workloads.plain_cpu_options()
+    pl = orc.Pipeline(g.points, g.starts, g.bbox, opts.iter_params(0))
+    base = pl.points.copy()
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(g)
+        layout = gpu.distortion_layout()
+        b = gpu.distortion_snapshot(layout)
+        gpu.iterate(10)
+        d_gpu = bt.distortion(b, gpu.distortion_snapshot(layout), 512)[0]
+        img = gpu.render()
+    for it in range(10):
+        pl.iterate(opts.iter_params(it))
+    d_ref, _ = orc.distortion(base, pl.points, pl.offsets, pl.counts)
+    assert d_ref > 0 and abs(d_gpu - d_ref) <= 0.01 * d_ref
+    assert d_gpu == d_ref
+    ref_img = _oracle_image(orc, bt, pl, 512, 512, "viridis", 0, 4, False)
+    n_gpu = bt.noisiness(img, (6, 6, 9))["noisiness"]
+    n_ref = bt.noisiness(ref_img, (6, 6, 9))["noisiness"]
+    assert abs(n_gpu - n_ref) <= 0.01 * n_ref
+    np.testing.assert_array_equal(img, ref_img)
+
+
 def test_render_errors(bt):
     g = workloads.plain_cpu_graph()
     with bt.Gpu(bt.Options(map_size=201, neighbor_radius=10, color_by_density=False)) as gpu:
