@@ -353,3 +353,47 @@ def test_server_results_match_alone(bt, policy, tick):
         p, _ = srv.points(t, res.samples)
         np.testing.assert_array_equal(p.view(np.uint32), ref.view(np.uint32))
     srv.close()
+
+
+def _edge_case_graph(bt):
+    """Polylines that stress the path: two identical co-located edges, a
+    closed loop (first = last point), a zero-length edge (one repeated point:
+    the closed-path special case), an edge shorter than one texel, a polyline
+    with many short segments, and a lone long edge."""
+    polys = [
+        [(0.1, 0.5), (0.9, 0.5)],
+        [(0.1, 0.5), (0.9, 0.5)],
+        [(0.3, 0.3), (0.6, 0.3), (0.6, 0.6), (0.3, 0.3)],
+        [(0.7, 0.8), (0.7, 0.8)],
+        [(0.2, 0.2), (0.2005, 0.2001)],
+        [(0.1 + 0.01 * k, 0.7 + 0.02 * ((-1) ** k)) for k in range(60)],
+        [(0.05, 0.05), (0.95, 0.95)],
+    ]
+    pts = np.array([p for poly in polys for p in poly], np.float32)
+    starts = np.cumsum([0] + [len(p) for p in polys]).astype(np.uint32)
+    return bt.Graph(pts, starts)
+
+
+@pytest.mark.parametrize("radius", [10, 400])
+def test_edge_case_graph_bitexact(bt, orc, radius):
+    """Reference mode S = 201: upload, three iterations and the image
+    bit-identical to the oracle; radius 400 texels puts the whole map in one
+    grid cell."""
+    g = _edge_case_graph(bt)
+    opts = bt.Options(map_size=201, neighbor_radius=radius)
+    pl = _oracle_pipeline(orc, g, opts)
+    with bt.Gpu(opts) as gpu:
+        n = gpu.upload(g)
+        assert n == len(pl.points)
+        np.testing.assert_array_equal(gpu.paths()[:, 2], pl.counts)
+        _assert_points(gpu.points()[0], pl.points, "upload")
+        for it in range(3):
+            gpu.iterate()
+            _assert_points(gpu.points()[0], pl.iterate(opts.iter_params(it)), f"iteration {it + 1}")
+            _assert_density(gpu, pl, f"iteration {it + 1}")
+        pal, ep = bt.api.palette("viridis")
+        paths = np.stack([pl.offsets.astype(np.uint32), np.zeros_like(pl.counts), pl.counts], 1)
+        want = orc.render(pl.points, pl.edge_id, paths, 201, 201, pal, ep,
+                          density=pl.normalized, msaa=4, dots=True)
+        np.testing.assert_array_equal(gpu.render("viridis", 0, 4, True), want)
+        np.testing.assert_array_equal(gpu.endpoints(), orc.endpoints(pl.points, paths))
