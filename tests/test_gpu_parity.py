@@ -397,3 +397,26 @@ def test_edge_case_graph_bitexact(bt, orc, radius):
                           density=pl.normalized, msaa=4, dots=True)
         np.testing.assert_array_equal(gpu.render("viridis", 0, 4, True), want)
         np.testing.assert_array_equal(gpu.endpoints(), orc.endpoints(pl.points, paths))
+
+
+def test_second_upload_replaces_the_graph(bt, cpu_case):
+    """gpu.rs:494-497: a new upload drops the previous graph (and its
+    iteration count); the results equal those of a fresh context."""
+    other = workloads.gmm_graph(7, 80, 3, 500)
+    opts = workloads.reference_cpu_options()
+    with bt.Gpu(opts) as fresh:
+        n_fresh = fresh.upload(other)
+        fresh.iterate()
+        fresh.iterate()
+        want = fresh.points()[0]
+        want_d = fresh.density(3)
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(cpu_case)
+        gpu.iterate()
+        n = gpu.upload(other)
+        assert n == n_fresh and gpu.job_samples(0) == (n, 0)
+        assert len(gpu.paths()) == other.edge_count
+        gpu.iterate()
+        gpu.iterate()
+        np.testing.assert_array_equal(gpu.points()[0].view(np.uint32), want.view(np.uint32))
+        np.testing.assert_array_equal(gpu.density(3).view(np.uint32), want_d.view(np.uint32))
