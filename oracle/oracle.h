@@ -5,7 +5,10 @@
  * TRL parsing, 95 % normalization, arc-length resampling, spatial-grid binning,
  * the fixed-radius neighbor mean with fixed path endpoints, the unit-weight
  * density splat, the separable Gaussian blur, the peak and normalize steps,
- * and the f64 distortion metric. Each function cites the reference lines it
+ * the f64 distortion metric, and the export image (segment quads, MSAA
+ * coverage, 8-bit alpha blending, endpoint dots, resolve; a written
+ * rasterization contract shared with the HIP rasterizer, since the
+ * reference's coverage comes from its driver). Each function cites the reference lines it
  * follows. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load this library, and only as the checker or the timed CPU
  * baseline; the product never links it.
