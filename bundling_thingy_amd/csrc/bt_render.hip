@@ -206,63 +206,105 @@ __device__ __forceinline__ void edge_band(const Quad& q, int a, int b, double yl
   hi = fmax(hi, fmax(xa, xb));
 }
 
-// Walks the fragments of segment i in draw order: the pixels of its quad in
-// row-major order; a pixel that both triangles cover on a common sample gets
-// two fragments (first triangle first), otherwise one with both masks. Each
-// pixel row only visits the columns between the leftmost and rightmost
-// points of the triangles' edges inside the row's sample band (a closed
-// triangle meets a horizontal line between two of its edge points), widened
-// by 4/256 px against rounding; the exact per-sample test decides coverage,
-// so the bound changes work (O(length) instead of O(length^2) for long
-// segments), not results.
-template <class F>
-__device__ void walk_segment(const Frame& f, uint32_t i, F&& emit) {
+// A segment's quad, triangles and pixel span, set up once per work item.
+struct SegSetup {
   Quad q;
-  if (!seg_quad(f, i, q)) return;
   Tri t0, t1;
-  tri_setup(q.x[0], q.y[0], q.x[1], q.y[1], q.x[2], q.y[2], t0);
-  tri_setup(q.x[2], q.y[2], q.x[1], q.y[1], q.x[3], q.y[3], t1);
-  const Span s = quad_span(f, q);
-  for (int py = s.y0; py <= s.y1; ++py) {
-    const double yl = (double)py * 256.0 + 32.0, yh = (double)py * 256.0 + 224.0;
-    double lo = 1.0e300, hi = -1.0e300;
-    edge_band(q, 0, 1, yl, yh, lo, hi);  // the edges of (c0 c1 c2) and (c2 c1 c3)
-    edge_band(q, 1, 2, yl, yh, lo, hi);
-    edge_band(q, 2, 0, yl, yh, lo, hi);
-    edge_band(q, 1, 3, yl, yh, lo, hi);
-    edge_band(q, 3, 2, yl, yh, lo, hi);
-    if (lo > hi) continue;
-    const int x0 = max(s.x0, (int)ceil((lo - 4.0 - 224.0) / 256.0));
-    const int x1 = min(s.x1, (int)floor((hi + 4.0 - 32.0) / 256.0));
-    for (int px = x0; px <= x1; ++px) {
-      uint32_t m0, m1;
-      pixel_masks(f, t0, t1, px, py, m0, m1);
-      const uint32_t pix = (uint32_t)py * f.w + (uint32_t)px;
-      if (m0 & m1) {
-        emit(pix, m0);
-        emit(pix, m1);
-      } else if (m0 | m1) {
-        emit(pix, m0 | m1);
-      }
+  Span s;
+  bool ok;
+};
+
+__device__ __forceinline__ SegSetup seg_setup(const Frame& f, uint32_t i) {
+  SegSetup g;
+  g.ok = seg_quad(f, i, g.q);
+  if (!g.ok) return g;
+  tri_setup(g.q.x[0], g.q.y[0], g.q.x[1], g.q.y[1], g.q.x[2], g.q.y[2], g.t0);
+  tri_setup(g.q.x[2], g.q.y[2], g.q.x[1], g.q.y[1], g.q.x[3], g.q.y[3], g.t1);
+  g.s = quad_span(f, g.q);
+  g.ok = g.s.x0 <= g.s.x1 && g.s.y0 <= g.s.y1;
+  return g;
+}
+
+// Pixel rows of segment i's quad inside the image (0 if none).
+__device__ __forceinline__ uint32_t seg_rows(const SegSetup& g) {
+  return g.ok ? (uint32_t)(g.s.y1 - g.s.y0 + 1) : 0u;
+}
+
+// Walks the fragments of one pixel row py of a segment in draw order
+// (left to right); a pixel that both triangles cover on a common sample gets
+// two fragments (first triangle first), otherwise one with both masks. The
+// row only visits the columns between the leftmost and rightmost points of
+// the triangles' edges inside its sample band (a closed triangle meets a
+// horizontal line between two of its edge points), widened by 4/256 px
+// against rounding; the exact per-sample test decides coverage, so the bound
+// changes work, not results.
+template <class F>
+__device__ void walk_row(const Frame& f, const SegSetup& g, int py, F&& emit) {
+  const double yl = (double)py * 256.0 + 32.0, yh = (double)py * 256.0 + 224.0;
+  double lo = 1.0e300, hi = -1.0e300;
+  edge_band(g.q, 0, 1, yl, yh, lo, hi);  // the edges of (c0 c1 c2) and (c2 c1 c3)
+  edge_band(g.q, 1, 2, yl, yh, lo, hi);
+  edge_band(g.q, 2, 0, yl, yh, lo, hi);
+  edge_band(g.q, 1, 3, yl, yh, lo, hi);
+  edge_band(g.q, 3, 2, yl, yh, lo, hi);
+  if (lo > hi) return;
+  const int x0 = max(g.s.x0, (int)ceil((lo - 4.0 - 224.0) / 256.0));
+  const int x1 = min(g.s.x1, (int)floor((hi + 4.0 - 32.0) / 256.0));
+  for (int px = x0; px <= x1; ++px) {
+    uint32_t m0, m1;
+    pixel_masks(f, g.t0, g.t1, px, py, m0, m1);
+    const uint32_t pix = (uint32_t)py * f.w + (uint32_t)px;
+    if (m0 & m1) {
+      emit(pix, m0);
+      emit(pix, m1);
+    } else if (m0 | m1) {
+      emit(pix, m0 | m1);
     }
   }
 }
 
-__global__ __launch_bounds__(256) void rs_count(Frame f, uint32_t* cnt) {
+// Work items are (segment, pixel row) pairs, so a segment hundreds of pixels
+// long (bundling leaves such segments next to fixed endpoints) is spread over
+// as many threads as it has rows instead of holding one lane of a wave.
+// rs_rows: rows per segment (entry n = 0, for the scan).
+__global__ __launch_bounds__(256) void rs_rows(Frame f, uint32_t* rows) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i == 0) cnt[f.n] = 0;
+  if (i == 0) rows[f.n] = 0;
   if (i >= f.n) return;
-  uint32_t c = 0;
-  walk_segment(f, i, [&](uint32_t, uint32_t) { ++c; });
-  cnt[i] = c;
+  rows[i] = seg_rows(seg_setup(f, i));
 }
 
-__global__ __launch_bounds__(256) void rs_emit(Frame f, const uint32_t* off, uint32_t* keys,
-                                               uint32_t* vals) {
+// rs_row_seg: the segment of every row item (row_off = exclusive scan of rows).
+__global__ __launch_bounds__(256) void rs_row_seg(Frame f, const uint32_t* rows,
+                                                  const uint32_t* row_off, uint32_t* row_seg) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= f.n) return;
-  uint32_t o = off[i];
-  walk_segment(f, i, [&](uint32_t pix, uint32_t mask) {
+  const uint32_t o = row_off[i], k = rows[i];
+  for (uint32_t r = 0; r < k; ++r) row_seg[o + r] = i;
+}
+
+// rs_count: fragments per row item (entry R = 0, for the scan).
+__global__ __launch_bounds__(256) void rs_count(Frame f, uint32_t n_items, const uint32_t* row_seg,
+                                                const uint32_t* row_off, uint32_t* cnt) {
+  const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+  if (r == 0) cnt[n_items] = 0;
+  if (r >= n_items) return;
+  const uint32_t i = row_seg[r];
+  const SegSetup g = seg_setup(f, i);
+  uint32_t c = 0;
+  walk_row(f, g, g.s.y0 + (int)(r - row_off[i]), [&](uint32_t, uint32_t) { ++c; });
+  cnt[r] = c;
+}
+
+__global__ __launch_bounds__(256) void rs_emit(Frame f, uint32_t n_items, const uint32_t* row_seg,
+                                               const uint32_t* row_off, const uint32_t* off,
+                                               uint32_t* keys, uint32_t* vals) {
+  const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+  if (r >= n_items) return;
+  const uint32_t i = row_seg[r];
+  const SegSetup g = seg_setup(f, i);
+  uint32_t o = off[r];
+  walk_row(f, g, g.s.y0 + (int)(r - row_off[i]), [&](uint32_t pix, uint32_t mask) {
     keys[o] = pix | (mask << kMaxPixelBits);
     vals[o] = i;
     ++o;
@@ -465,32 +507,48 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
     HIPCHK(hipGetLastError());
     f.pts = staging_.as<float>();
 
-    // fragment counts -> offsets (draw order) -> total
-    cnt.alloc(((size_t)f.n + 1) * 4);
-    off.alloc(((size_t)f.n + 1) * 4);
-    hipLaunchKernelGGL(rs_count, dim3(blocks(f.n)), dim3(256), 0, stream_, f,
-                       cnt.as<uint32_t>());
-    HIPCHK(hipGetLastError());
-    size_t tb = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.as<uint32_t>(), off.as<uint32_t>(),
-                                            f.n + 1, stream_));
-    temp.alloc(std::max<size_t>(tb, 256));
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(temp.p, tb, cnt.as<uint32_t>(), off.as<uint32_t>(),
-                                            f.n + 1, stream_));
-    // the 32-bit offsets (and the sort's int count) hold up to 2^31 - 1
-    // fragments: check the total in 64 bits before trusting them
+    // Exclusive scan of `count` + 1 u32 entries (the last one 0) into out;
+    // returns the total, summed in 64 bits: the 32-bit offsets (and the
+    // sort's int count) hold at most 2^31 - 1 items.
     DevBuf total;
     total.pool = pool_.get();
     total.alloc(8);
-    size_t rb = 0;
-    HIPCHK(hipcub::DeviceReduce::Sum(nullptr, rb, cnt.as<uint32_t>(), total.as<uint64_t>(),
-                                     f.n + 1, stream_));
-    temp.reserve(rb);
-    HIPCHK(hipcub::DeviceReduce::Sum(temp.p, rb, cnt.as<uint32_t>(), total.as<uint64_t>(),
-                                     f.n + 1, stream_));
-    uint64_t m64 = 0;
-    HIPCHK(hipMemcpyAsync(&m64, total.p, 8, hipMemcpyDeviceToHost, stream_));
-    sync();
+    auto scan = [&](const uint32_t* in, uint32_t* out, uint64_t count) -> uint64_t {
+      size_t tb = 0, rb = 0;
+      HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, count + 1, stream_));
+      HIPCHK(hipcub::DeviceReduce::Sum(nullptr, rb, in, total.as<uint64_t>(), count + 1,
+                                       stream_));
+      temp.reserve(std::max<size_t>(std::max(tb, rb), 256));
+      HIPCHK(hipcub::DeviceScan::ExclusiveSum(temp.p, tb, in, out, count + 1, stream_));
+      HIPCHK(hipcub::DeviceReduce::Sum(temp.p, rb, in, total.as<uint64_t>(), count + 1,
+                                       stream_));
+      uint64_t t = 0;
+      HIPCHK(hipMemcpyAsync(&t, total.p, 8, hipMemcpyDeviceToHost, stream_));
+      sync();
+      return t;
+    };
+    // rows per segment -> row items (segment order, then top to bottom)
+    cnt.alloc(((size_t)f.n + 1) * 4);
+    off.alloc(((size_t)f.n + 1) * 4);
+    hipLaunchKernelGGL(rs_rows, dim3(blocks(f.n)), dim3(256), 0, stream_, f,
+                       cnt.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+    const uint64_t n_rows = scan(cnt.as<uint32_t>(), off.as<uint32_t>(), f.n);
+    if (n_rows > 0x7fffffffull) fail(BT_E_CAPACITY, "frame exceeds 2^31 - 1 segment rows");
+    const uint32_t R = (uint32_t)n_rows;
+    DevBuf row_seg, fcnt, foff;
+    for (DevBuf* b : {&row_seg, &fcnt, &foff}) b->pool = pool_.get();
+    row_seg.alloc(std::max<size_t>(R, 1) * 4);
+    fcnt.alloc(((size_t)R + 1) * 4);
+    foff.alloc(((size_t)R + 1) * 4);
+    hipLaunchKernelGGL(rs_row_seg, dim3(blocks(f.n)), dim3(256), 0, stream_, f,
+                       cnt.as<uint32_t>(), off.as<uint32_t>(), row_seg.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+    // fragments per row item -> offsets (draw order) -> total
+    hipLaunchKernelGGL(rs_count, dim3(blocks(R + 1ull)), dim3(256), 0, stream_, f, R,
+                       row_seg.as<uint32_t>(), off.as<uint32_t>(), fcnt.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+    const uint64_t m64 = scan(fcnt.as<uint32_t>(), foff.as<uint32_t>(), R);
     if (m64 > 0x7fffffffull) fail(BT_E_CAPACITY, "frame exceeds 2^31 - 1 fragments");
     const uint32_t m = (uint32_t)m64;
     if (m > 0) {
@@ -498,8 +556,9 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
       vals.alloc((size_t)m * 4);
       keys2.alloc((size_t)m * 4);
       vals2.alloc((size_t)m * 4);
-      hipLaunchKernelGGL(rs_emit, dim3(blocks(f.n)), dim3(256), 0, stream_, f,
-                         off.as<uint32_t>(), keys.as<uint32_t>(), vals.as<uint32_t>());
+      hipLaunchKernelGGL(rs_emit, dim3(blocks(R)), dim3(256), 0, stream_, f, R,
+                         row_seg.as<uint32_t>(), off.as<uint32_t>(), foff.as<uint32_t>(),
+                         keys.as<uint32_t>(), vals.as<uint32_t>());
       HIPCHK(hipGetLastError());
       // stable sort on the pixel bits: per pixel, fragments stay in draw order
       int bits = 1;
