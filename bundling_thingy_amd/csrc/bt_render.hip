@@ -6,11 +6,14 @@
 //   emit     the same walk, writing {pixel | coverage << 24, segment}
 //   sort     stable radix sort on the pixel bits: each pixel's fragments
 //            end up together, still in draw (segment) order
-//   runs     first and last fragment of every pixel
+//   counts   density colouring (one colour per pixel): fragments per sample,
+//            one atomic per run of a pixel's fragments in a wave
+//   runs     edge colouring: first and last fragment of every pixel
 //   dots     endpoint outline / dot coverage bits per sample (atomicOr)
 //   resolve  one thread per pixel: blends its fragments into its samples in
 //            order (8-bit storage after every blend, as an RGBA8 MSAA target
-//            holds them), applies the dots, averages the samples
+//            holds them; with density colouring, its count of blends of the
+//            pixel's colour), applies the dots, averages the samples
 //
 // Rasterization contract (the hardware's MSAA coverage is not pinned by any
 // reference output, so this is the definition the oracle shares):
@@ -387,10 +390,41 @@ __device__ float density_at(const Frame& f, uint32_t px, uint32_t py) {
   return fminf(fmaxf(v, 0.0f), 1.0f);
 }
 
+// Density colouring gives every fragment of a pixel the pixel's colour, so a
+// sample's stored value after c blends is the blend applied c times to the
+// clear colour, in whatever order: the resolve then needs only per-sample
+// fragment counts. The fragments are sorted by pixel, so a wave's fragments
+// of one pixel are adjacent lanes: the first lane of each such run adds the
+// run's per-sample counts (one atomic per run and sample).
+__global__ __launch_bounds__(256) void rs_sample_counts(const uint32_t* keys, uint32_t m,
+                                                        uint32_t msaa, uint32_t* scount) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool valid = j < m;
+  const uint32_t key = valid ? keys[j] : 0u;
+  const uint32_t pm = (1u << kMaxPixelBits) - 1u;
+  const uint32_t p = key & pm, mask = key >> kMaxPixelBits;
+  const uint32_t prev = __shfl_up(p, 1, 64);
+  const bool head = valid && (lane == 0u || prev != p);
+  const unsigned long long heads = __ballot(head);
+  unsigned long long bits[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4u; ++k) bits[k] = __ballot(valid && ((mask >> k) & 1u));
+  if (head) {
+    const unsigned long long above = lane == 63u ? 0ull : heads & (~0ull << (lane + 1u));
+    const unsigned long long upto = above ? (1ull << __builtin_ctzll(above)) - 1ull : ~0ull;
+    const unsigned long long run = upto & (~0ull << lane);
+    for (uint32_t k = 0; k < msaa; ++k) {
+      const uint32_t c = (uint32_t)__popcll(bits[k] & run);
+      if (c) atomicAdd(&scount[4u * p + k], c);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void rs_resolve(Frame f, const uint32_t* keys,
                                                   const uint32_t* vals, const uint32_t* first,
-                                                  const uint32_t* last, const uint32_t* dots,
-                                                  uint32_t* out) {
+                                                  const uint32_t* last, const uint32_t* scount,
+                                                  const uint32_t* dots, uint32_t* out) {
   const uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= f.w * f.h) return;
   const uint32_t px = p % f.w, py = p / f.w;
@@ -398,8 +432,29 @@ __global__ __launch_bounds__(256) void rs_resolve(Frame f, const uint32_t* keys,
   const uint32_t bg[3] = {unorm8(kClear[0]), unorm8(kClear[1]), unorm8(kClear[2])};
   for (int k = 0; k < 4; ++k)
     for (int c = 0; c < 3; ++c) v[k][c] = bg[c];
+  if (scount) {
+    // density colouring: c blends of the pixel's colour per sample; the
+    // blend is monotone in the stored value, so it stops changing after at
+    // most 255 applications and the loop ends there
+    const uint4 cnt = reinterpret_cast<const uint4*>(scount)[p];
+    const uint32_t ck[4] = {cnt.x, cnt.y, cnt.z, cnt.w};
+    if (cnt.x | cnt.y | cnt.z | cnt.w) {
+      float col[3];
+      palette_rgb(f.palette, density_at(f, px, py), col);
+      for (uint32_t k = 0; k < f.msaa; ++k)
+        for (uint32_t t = 0; t < ck[k]; ++t) {
+          uint32_t same = 1u;
+          for (int c = 0; c < 3; ++c) {
+            const uint32_t nv = blend8(v[k][c], col[c]);
+            same &= nv == v[k][c];
+            v[k][c] = nv;
+          }
+          if (same) break;
+        }
+    }
+  }
   const uint32_t s0 = first ? first[p] : 0u, s1 = last ? last[p] : 0u;
-  if (s1 > s0) {
+  if (!scount && s1 > s0) {
     const bool by_edge = f.color_by == BT_COLOR_EDGE_ID;
     float col[3] = {0.0f, 0.0f, 0.0f};
     if (!by_edge) palette_rgb(f.palette, density_at(f, px, py), col);
@@ -478,13 +533,13 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
   f.palette = ro.palette;
   f.color_by = ro.color_by;
   const size_t npix = (size_t)f.w * f.h;
-  DevBuf image, plan, cnt, off, keys, vals, keys2, vals2, first, last, dots, temp;
-  for (DevBuf* b :
-       {&image, &plan, &cnt, &off, &keys, &vals, &keys2, &vals2, &first, &last, &dots, &temp})
+  DevBuf image, plan, cnt, off, keys, vals, keys2, vals2, first, last, scount, dots, temp;
+  for (DevBuf* b : {&image, &plan, &cnt, &off, &keys, &vals, &keys2, &vals2, &first, &last,
+                    &scount, &dots, &temp})
     b->pool = pool_.get();
   image.alloc(npix * 4);
   const uint32_t *kkeys = nullptr, *kvals = nullptr, *kfirst = nullptr, *klast = nullptr,
-                 *kdots = nullptr;
+                 *kscount = nullptr, *kdots = nullptr;
   if (!jobs_.empty() && job(job_i).n > 0) {
     Job& j = job(job_i);
     if (ro.color_by == BT_COLOR_DENSITY && !j.dens_norm.p)
@@ -569,17 +624,26 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
       HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, dk, dv, (int)m, 0, bits, stream_));
       temp.reserve(sb);
       HIPCHK(hipcub::DeviceRadixSort::SortPairs(temp.p, sb, dk, dv, (int)m, 0, bits, stream_));
-      first.alloc(npix * 4);
-      last.alloc(npix * 4);
-      HIPCHK(hipMemsetAsync(first.p, 0, npix * 4, stream_));
-      HIPCHK(hipMemsetAsync(last.p, 0, npix * 4, stream_));
-      hipLaunchKernelGGL(rs_runs, dim3(blocks(m)), dim3(256), 0, stream_, dk.Current(), m,
-                         first.as<uint32_t>(), last.as<uint32_t>());
-      HIPCHK(hipGetLastError());
       kkeys = dk.Current();
       kvals = dv.Current();
-      kfirst = first.as<uint32_t>();
-      klast = last.as<uint32_t>();
+      if (ro.color_by == BT_COLOR_DENSITY) {
+        scount.alloc(npix * 16);
+        HIPCHK(hipMemsetAsync(scount.p, 0, npix * 16, stream_));
+        hipLaunchKernelGGL(rs_sample_counts, dim3(blocks(m)), dim3(256), 0, stream_, kkeys, m,
+                           f.msaa, scount.as<uint32_t>());
+        HIPCHK(hipGetLastError());
+        kscount = scount.as<uint32_t>();
+      } else {
+        first.alloc(npix * 4);
+        last.alloc(npix * 4);
+        HIPCHK(hipMemsetAsync(first.p, 0, npix * 4, stream_));
+        HIPCHK(hipMemsetAsync(last.p, 0, npix * 4, stream_));
+        hipLaunchKernelGGL(rs_runs, dim3(blocks(m)), dim3(256), 0, stream_, kkeys, m,
+                           first.as<uint32_t>(), last.as<uint32_t>());
+        HIPCHK(hipGetLastError());
+        kfirst = first.as<uint32_t>();
+        klast = last.as<uint32_t>();
+      }
     }
     if (ro.endpoint_dots && j.n_paths > 0) {
       dots.alloc(npix * 4);
@@ -591,7 +655,7 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
     }
   }
   hipLaunchKernelGGL(rs_resolve, dim3(blocks(npix)), dim3(256), 0, stream_, f, kkeys, kvals,
-                     kfirst, klast, kdots, image.as<uint32_t>());
+                     kfirst, klast, kscount, kdots, image.as<uint32_t>());
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(rgba, image.p, npix * 4, hipMemcpyDeviceToHost, stream_));
   sync();
