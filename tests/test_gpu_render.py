@@ -1,10 +1,12 @@
 """GPU parity of the export image (render_paths.wgsl, gpu.rs:572-651): the HIP
 rasterizer through the C ABI against the oracle's sequential restatement,
 byte for byte, and the noisiness of both (noisiness.rs)."""
+import ctypes
+
 import numpy as np
 import pytest
 
-from bundling_thingy_amd import workloads
+from bundling_thingy_amd import api, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -150,8 +152,24 @@ def test_render_errors(bt):
             gpu.render(color_by=bt.COLOR_DENSITY)
         assert e.value.code == 5
         assert gpu.render(color_by=bt.COLOR_EDGE_ID).shape == (201, 201, 4)
-        with pytest.raises(bt.BtError):
-            gpu.render(color_by=bt.COLOR_EDGE_ID, msaa=8)
+        for kw in ({"msaa": 8}, {"palette": 7}, {"msaa": 1}):
+            with pytest.raises(bt.BtError) as e:
+                gpu.render(color_by=bt.COLOR_EDGE_ID, **kw)
+            assert e.value.code == 1
+        # a buffer of the wrong size is refused, not overrun
+        ro = api.bt_render_options()
+        api.lib().bt_render_options_default(ctypes.byref(ro))
+        ro.color_by = bt.COLOR_EDGE_ID
+        small = np.zeros(201 * 201 * 4 - 4, np.uint8)
+        assert api.lib().bt_render(gpu._h, ctypes.byref(ro), small.ctypes.data_as(api._P),
+                                   small.nbytes) == 1
+        with pytest.raises(bt.BtError) as e:
+            gpu.render(color_by=bt.COLOR_EDGE_ID, job=3)
+        assert e.value.code == 1  # job index out of range
+    with bt.Gpu(bt.Options(map_size=201, neighbor_radius=10)) as gpu:
+        with pytest.raises(bt.BtError) as e:
+            gpu.endpoints()
+        assert e.value.code == 2  # no uploaded graph
 
 
 def test_headless_export(bt, orc, tmp_path):
