@@ -538,6 +538,17 @@ int bt_server_submit(bt_server* srv, const float* xy, uint32_t n_points, const u
                      uint32_t n_starts, const bt_options* opts, uint32_t iterations,
                      uint64_t* ticket) {
   if (!srv || !xy || !starts || n_starts < 2 || iterations == 0 || !ticket) return BT_E_INVALID;
+  // the footprint estimate walks the polylines: check them first (the same
+  // rules as an upload)
+  for (uint32_t i = 0; i + 1 < n_starts; ++i)
+    if (starts[i] > starts[i + 1]) {
+      srv->err = "path starts are not sorted";
+      return BT_E_INVALID;
+    }
+  if (starts[n_starts - 1] > n_points || n_points < 2) {
+    srv->err = "path starts exceed the points";
+    return BT_E_INVALID;
+  }
   auto s = std::make_unique<ServedJob>();
   s->ticket = srv->next_ticket++;
   s->xy.assign(xy, xy + 2ull * n_points);

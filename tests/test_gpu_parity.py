@@ -420,3 +420,17 @@ def test_second_upload_replaces_the_graph(bt, cpu_case):
         gpu.iterate()
         np.testing.assert_array_equal(gpu.points()[0].view(np.uint32), want.view(np.uint32))
         np.testing.assert_array_equal(gpu.density(3).view(np.uint32), want_d.view(np.uint32))
+
+
+def test_server_rejects_malformed_starts(bt):
+    """A submit validates the polylines before estimating their footprint."""
+    srv = bt.Server(bt.Options(), memory_budget=1 << 30, max_inflight=4)
+    try:
+        pts = np.array([[0, 0], [1, 1], [2, 0]], np.float32)
+        for starts in ([0, 3, 2], [0, 2, 9]):
+            with pytest.raises(bt.BtError) as e:
+                srv.submit(bt.Graph(pts, np.array(starts, np.uint32), [0, 0, 2, 1]),
+                           bt.Options(map_size=201, neighbor_radius=10), 2)
+            assert e.value.code == 1
+    finally:
+        srv.close()
