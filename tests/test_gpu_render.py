@@ -204,3 +204,24 @@ def test_headless_export(bt, orc, tmp_path):
     want = _oracle_image(orc, bt, pl, 201, 201, "viridis", 0, 4, False)
     np.testing.assert_array_equal(last, want)
     assert noise["images"][3]["noisiness"] == bt.noisiness(want, (6, 6, 9))["noisiness"]
+
+
+def test_non_square_map_bitexact(bt, orc):
+    """A 301 x 201 map in reference mode (texel scales differ per axis; the
+    window's size after a resize, gpu.rs:653-697): positions, density and
+    the image (quads, dots, bilinear density lookup) against the oracle."""
+    g = workloads.plain_cpu_graph()
+    opts = bt.Options(map_size=301, map_h=201, neighbor_radius=10)
+    pl = orc.Pipeline(g.points, g.starts, g.bbox, opts.iter_params(0))
+    with bt.Gpu(opts) as gpu:
+        assert gpu.upload(g) == len(pl.points)
+        for it in range(2):
+            gpu.iterate()
+            pl.iterate(opts.iter_params(it))
+        got = gpu.points()[0]
+        np.testing.assert_array_equal(got.view(np.uint32), pl.points.view(np.uint32))
+        np.testing.assert_array_equal(gpu.density(3).view(np.uint32),
+                                      pl.normalized.view(np.uint32))
+        assert gpu.render_size() == (301, 201)
+        for color_by in (0, 1):
+            _check(gpu, orc, bt, pl, "viridis", color_by, 4, True, f"color_by {color_by}")
