@@ -18,12 +18,13 @@ static inline size_t align_up(size_t a, size_t b) { return (a + b - 1) / b * b; 
 
 static const uint32_t kWt = BT_MAX_TAPS + 1;  // weights per iteration row
 
-// Layout of the heavy-task split scratch (one per engine).
-// Pinned host slots for plan uploads, and device slots for one-job readback plans.
+// Pinned host slots for plan uploads, and device slots for one-job readback
+// plans.
 static const uint32_t kPinnedSlots = 4;
 static const uint32_t kRetireSlots = 64;
 static const size_t kRetireSlotBytes = 2048;
 
+// Layout of the heavy-task split scratch (one per engine).
 static const size_t kSplitItemsOff = (size_t)BT_MAX_HEAVY * sizeof(BtHeavy);
 static const size_t kSplitAccOff = kSplitItemsOff + (size_t)BT_MAX_ITEMS * sizeof(uint2);
 static const size_t kSplitCandOff = kSplitAccOff + (size_t)BT_MAX_HEAVY * 192 * sizeof(long long);

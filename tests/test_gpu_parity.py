@@ -326,6 +326,38 @@ def test_error_paths(bt, cpu_case):
         assert e.value.code == 3 and "sample buffer capacity" in e.value.msg
 
 
+@pytest.mark.parametrize("points,starts,msg", [
+    ([[0.2, 0.3]], [0, 1], "no drawable edges"),
+    ([[0, 0], [1, 1], [2, 0], [3, 1]], [0, 3, 2, 4], "not sorted"),
+    ([[0, 0], [1, 1], [2, 0]], [0, 5], "exceed the points"),
+    ([[0, 0], [np.nan, 1], [2, 0]], [0, 3], "non-finite"),
+    ([[0.5, 0.5], [0.5, 0.5], [0.5, 0.5]], [0, 3], "invalid extent"),
+])
+def test_rejected_upload_drops_the_graph(bt, cpu_case, points, starts, msg):
+    """gpu.rs:494: an upload drops the previous graph before anything can
+    fail, so a rejected one leaves none (iterate: code 2). Malformed input is
+    refused with the reference's messages (graph.rs:49-90); a valid upload
+    afterwards runs as in a fresh context."""
+    opts = workloads.reference_cpu_options()
+    with bt.Gpu(opts) as fresh:
+        fresh.upload(cpu_case)
+        fresh.iterate()
+        want = fresh.points()[0]
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(cpu_case)
+        gpu.iterate()
+        with pytest.raises(bt.BtError) as e:
+            gpu.upload(bt.Graph(np.array(points, np.float32), np.array(starts, np.uint32),
+                                bbox=np.zeros(4, np.float32)))
+        assert e.value.code == 1 and msg in e.value.msg, e.value.msg
+        with pytest.raises(bt.BtError) as e:
+            gpu.iterate()
+        assert e.value.code == 2
+        gpu.upload(cpu_case)
+        gpu.iterate()
+        np.testing.assert_array_equal(gpu.points()[0].view(np.uint32), want.view(np.uint32))
+
+
 @pytest.mark.parametrize("policy,tick", [(0, 0), (1, 0), (1, 1)])
 def test_server_results_match_alone(bt, policy, tick):
     """Co-scheduled jobs give the bits of each job alone, under both policies
