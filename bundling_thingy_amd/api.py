@@ -121,6 +121,7 @@ SIGNATURES = {
     "bt_upload_batch": (C.c_int, [_P, C.POINTER(bt_graph_desc), _U32, _PU32]),
     "bt_read_paths_job": (C.c_int, [_P, _U32, _PU32, _SZ]),
     "bt_read_points_job": (C.c_int, [_P, _U32, _P, _SZ]),
+    "bt_read_positions_job": (C.c_int, [_P, _U32, _PF, _PU32, _SZ]),
     "bt_read_density_job": (C.c_int, [_P, _U32, _U32, _P, _SZ]),
     "bt_job_samples": (C.c_int, [_P, _U32, _PU32, _PU32]),
     "bt_trl_parse": (C.c_int, [C.c_char_p, _SZ, C.POINTER(_PF), _PU32, C.POINTER(_PU32), _PU32,
@@ -498,14 +499,18 @@ class Gpu:
 
     def points_raw(self, job=0):
         n, _ = self.job_samples(job)
-        buf = np.zeros(n * 12, dtype=np.uint8)
+        buf = np.empty(n * 12, dtype=np.uint8)
         self._check(lib().bt_read_points_job(self._h, job, buf.ctypes.data_as(_P), n))
         return buf
 
     def points(self, job=0):
-        """(positions (N, 2) float32, edge ids (N,) uint32) in Layout order."""
-        r = self.points_raw(job).view(np.dtype([("x", "<f4"), ("y", "<f4"), ("id", "<u4")]))
-        return np.stack([r["x"], r["y"]], 1), r["id"].copy()
+        """(positions (N, 2) float32, edge ids (N,) uint32) in Layout order:
+        the fields of points_raw, split on the device."""
+        n, _ = self.job_samples(job)
+        xy = np.empty((n, 2), dtype=np.float32)
+        ids = np.empty(n, dtype=np.uint32)
+        self._check(lib().bt_read_positions_job(self._h, job, _f32p(xy), _u32p(ids), n))
+        return xy, ids
 
     def distortion_layout(self, job=0):
         n, _ = self.job_samples(job)

@@ -144,6 +144,14 @@ int bt_read_points_job(bt_ctx* ctx, uint32_t job, void* rec12, size_t n_samples)
   return guarded(&ctx->err, [&] { ctx->engine->read_points(job, rec12, n_samples); });
 }
 
+int bt_read_positions_job(bt_ctx* ctx, uint32_t job, float* xy, uint32_t* edge_ids,
+                          size_t n_samples) {
+  CTX_CHECK();
+  if (!xy) return BT_E_INVALID;
+  return guarded(&ctx->err,
+                 [&] { ctx->engine->read_positions(job, xy, edge_ids, n_samples); });
+}
+
 int bt_read_density(bt_ctx* ctx, uint32_t kind, void* out, size_t n_pixels) {
   return bt_read_density_job(ctx, 0, kind, out, n_pixels);
 }

@@ -20,7 +20,59 @@ void hip_check(hipError_t e, const char* what) {
 }
 
 DevPool::~DevPool() {
-  for (auto& kv : free_) (void)hipFree(kv.second);
+  for (auto& kv : free_)
+    if (own_.count(kv.second)) (void)hipFree(kv.second);
+  for (const Slab& s : slabs_) (void)hipFree(s.base);
+}
+
+void DevPool::release_cached() {
+  for (auto it = free_.begin(); it != free_.end();) {
+    if (own_.erase(it->second)) {
+      (void)hipFree(it->second);
+      cached_ -= it->first;
+      it = free_.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  // a slab whose carved bytes are all cached holds no live block
+  std::vector<Slab> keep;
+  for (size_t k = 0; k < slabs_.size(); ++k) {
+    const Slab& s = slabs_[k];
+    size_t in_cache = 0;
+    for (const auto& kv : free_) {
+      const char* q = static_cast<const char*>(kv.second);
+      if (q >= s.base && q < s.base + kSlabBytes) in_cache += kv.first;
+    }
+    if (in_cache != s.carved || k + 1 == slabs_.size()) {
+      keep.push_back(s);
+      continue;
+    }
+    for (auto it = free_.begin(); it != free_.end();) {
+      const char* q = static_cast<const char*>(it->second);
+      if (q >= s.base && q < s.base + kSlabBytes) {
+        cached_ -= it->first;
+        it = free_.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    (void)hipFree(s.base);
+  }
+  slabs_.swap(keep);
+}
+
+void* DevPool::device_alloc(size_t bytes) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    // out of memory: return what the cache can give back and try once more
+    (void)hipGetLastError();
+    release_cached();
+    e = hipMalloc(&q, bytes);
+  }
+  HIPCHK(e);
+  return q;
 }
 
 void* DevPool::get(size_t bytes, size_t* cap) {
@@ -39,17 +91,33 @@ void* DevPool::get(size_t bytes, size_t* cap) {
     free_.erase(it);
     return q;
   }
-  void* q = nullptr;
-  hipError_t e = hipMalloc(&q, c);
-  if (e != hipSuccess && !free_.empty()) {
-    // out of memory: return the cache to HIP and try once more
-    (void)hipGetLastError();
-    for (auto& kv : free_) (void)hipFree(kv.second);
-    free_.clear();
-    cached_ = 0;
-    e = hipMalloc(&q, c);
+  if (c > kSlabBlockMax) {
+    void* q = device_alloc(c);
+    own_.emplace(q, c);
+    *cap = c;
+    return q;
   }
-  HIPCHK(e);
+  if (slabs_.empty() || kSlabBytes - slabs_.back().carved < c) {
+    if (!slabs_.empty()) {
+      // the rest of the full slab goes to the cache as blocks of the size classes
+      Slab& s = slabs_.back();
+      while (kSlabBytes - s.carved >= 256) {
+        const size_t left = kSlabBytes - s.carved;
+        size_t piece = 256;
+        if (left >= (1u << 20)) {
+          piece = std::min(left >> 20 << 20, kSlabBlockMax);
+        } else {
+          while (piece * 2 <= left) piece <<= 1;
+        }
+        put(s.base + s.carved, piece);
+        s.carved += piece;
+      }
+    }
+    slabs_.push_back({static_cast<char*>(device_alloc(kSlabBytes)), 0});
+  }
+  Slab& s = slabs_.back();
+  void* q = s.base + s.carved;
+  s.carved += c;
   *cap = c;
   return q;
 }

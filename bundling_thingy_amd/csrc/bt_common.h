@@ -23,10 +23,16 @@ void hip_check(hipError_t e, const char* what);
 
 // Per-engine cache of device blocks: a freed block is kept and handed out
 // again (same stream, so reuse is ordered after every earlier use) instead of
-// going back through hipFree, which waits for the device. Blocks return to
-// HIP when the pool is destroyed.
+// going back through hipFree, which waits for the device. Blocks of at most
+// kSlabBlockMax bytes are carved from kSlabBytes slabs, so an upload of many
+// small jobs makes a few hipMalloc calls rather than one per buffer (each
+// costs about 10 us). When hipMalloc fails, the cached blocks of their own
+// and the slabs whose blocks are all cached return to HIP and the call is
+// tried once more; everything else returns when the pool is destroyed.
 class DevPool {
  public:
+  static constexpr size_t kSlabBytes = 64u << 20;
+  static constexpr size_t kSlabBlockMax = 4u << 20;
   DevPool() = default;
   DevPool(const DevPool&) = delete;
   DevPool& operator=(const DevPool&) = delete;
@@ -34,9 +40,18 @@ class DevPool {
   void* get(size_t bytes, size_t* cap);
   void put(void* p, size_t cap);
   size_t cached() const { return cached_; }
+  size_t slabs() const { return slabs_.size(); }
 
  private:
+  struct Slab {
+    char* base;
+    size_t carved;  // bytes handed out from the slab so far
+  };
+  void* device_alloc(size_t bytes);  // hipMalloc, retried once after release_cached()
+  void release_cached();
   std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> own_;      // blocks with a hipMalloc of their own
+  std::vector<Slab> slabs_;          // the last one is being carved
   size_t cached_ = 0;
 };
 

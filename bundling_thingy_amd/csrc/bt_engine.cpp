@@ -795,8 +795,7 @@ void Engine::read_paths(uint32_t job_i, uint32_t* rec3, size_t n) {
   sync();
 }
 
-void Engine::read_points_of(Engine& e, Job& j, void* rec12, size_t n) {
-  if (n != j.n) fail(BT_E_INVALID, "sample count mismatch");
+void Engine::unpermute_to_staging(Engine& e, Job& j, bool split) {
   e.staging_.reserve((size_t)j.n * 12);
   std::vector<Job*> js = {&j};
   Plan p = e.make_plan(js, {j.iteration});
@@ -807,14 +806,30 @@ void Engine::read_points_of(Engine& e, Job& j, void* rec12, size_t n) {
   const unsigned char* dev = table.as<unsigned char>();
   btk::unpermute(e.stream_, p.grid[BT_K_BIN], reinterpret_cast<const BtJob*>(dev),
                  reinterpret_cast<const uint32_t*>(dev + p.base_off) + BT_K_BIN * 2, 1,
-                 e.staging_.as<float>(), nullptr, 0u);
+                 e.staging_.as<float>(), nullptr, 0u, split ? 1u : 0u);
   HIPCHK(hipGetLastError());
+}
+
+void Engine::read_points_of(Engine& e, Job& j, void* rec12, size_t n) {
+  if (n != j.n) fail(BT_E_INVALID, "sample count mismatch");
+  unpermute_to_staging(e, j, false);
   HIPCHK(hipMemcpyAsync(rec12, e.staging_.p, (size_t)j.n * 12, hipMemcpyDeviceToHost, e.stream_));
   e.sync();
 }
 
 void Engine::read_points(uint32_t job_i, void* rec12, size_t n) {
   read_points_of(*this, job(job_i), rec12, n);
+}
+
+void Engine::read_positions(uint32_t job_i, float* xy, uint32_t* ids, size_t n) {
+  Job& j = job(job_i);
+  if (n != j.n) fail(BT_E_INVALID, "sample count mismatch");
+  unpermute_to_staging(*this, j, true);
+  HIPCHK(hipMemcpyAsync(xy, staging_.p, (size_t)j.n * 8, hipMemcpyDeviceToHost, stream_));
+  if (ids)
+    HIPCHK(hipMemcpyAsync(ids, staging_.as<unsigned char>() + (size_t)j.n * 8, (size_t)j.n * 4,
+                          hipMemcpyDeviceToHost, stream_));
+  sync();
 }
 
 void Engine::read_density_of(Engine& e, Job& j, uint32_t kind, void* out, size_t n) {

@@ -85,6 +85,8 @@ class Engine {
 
   void read_paths(uint32_t job, uint32_t* rec3, size_t n);
   void read_points(uint32_t job, void* rec12, size_t n);
+  // The same samples as n positions (x, y) and n edge ids (ids may be null).
+  void read_positions(uint32_t job, float* xy, uint32_t* ids, size_t n);
   void read_density(uint32_t job, uint32_t kind, void* out, size_t n);
   void read_grid(uint32_t job, uint32_t* cell_of, size_t n, uint32_t* counts, size_t ncells);
   // Sorted sample indices of the distinct path endpoint positions.
@@ -93,6 +95,8 @@ class Engine {
   void render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, uint32_t job = 0);
   void render_size(uint32_t* w, uint32_t* h, uint32_t job = 0);
   static void read_points_of(Engine& e, Job& j, void* rec12, size_t n);
+  // Unpermutes a job's current samples into staging_ (records, or split).
+  static void unpermute_to_staging(Engine& e, Job& j, bool split);
   static void read_density_of(Engine& e, Job& j, uint32_t kind, void* out, size_t n);
 
   // Heavy tasks split by the last iteration run, and their parts.

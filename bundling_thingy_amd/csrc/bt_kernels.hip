@@ -1488,16 +1488,23 @@ extern "C" __global__ __launch_bounds__(256) void bt_unpermute(const BtJob* __re
                                                                const uint32_t* __restrict__ base,
                                                                int njobs, float* __restrict__ out,
                                                                uint32_t* __restrict__ cell_out,
-                                                               uint32_t want_cells) {
+                                                               uint32_t want_cells,
+                                                               uint32_t split) {
   JOB_PROLOGUE(BT_K_BIN);
   const uint32_t i = lb * BT_BIN_BLOCK + threadIdx.x;
   if (i >= J.n) return;
   const float* rec = J.rec[J.parity];
   const uint32_t id = __float_as_uint(rec[3u * i + 2u]);
   const uint32_t orig = id & ~BT_FIXED_BIT;
-  out[3u * orig] = rec[3u * i];
-  out[3u * orig + 1u] = rec[3u * i + 1u];
-  out[3u * orig + 2u] = __uint_as_float(J.edge_of[orig]);
+  if (split) {
+    // n positions (x, y), then n edge ids
+    reinterpret_cast<float2*>(out)[orig] = make_float2(rec[3u * i], rec[3u * i + 1u]);
+    reinterpret_cast<uint32_t*>(out)[2u * J.n + orig] = J.edge_of[orig];
+  } else {
+    out[3u * orig] = rec[3u * i];
+    out[3u * orig + 1u] = rec[3u * i + 1u];
+    out[3u * orig + 2u] = __uint_as_float(J.edge_of[orig]);
+  }
   if (want_cells) {
     // cell of the sample as binned in the last iteration: sid/sorted are in
     // the order this iteration's input was binned from (rec of the other parity)
@@ -1624,8 +1631,8 @@ void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32
   hipLaunchKernelGGL(bt_resample_emit, L256(grid), jobs, base, nj);
 }
 void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
-               float* out, uint32_t* cell_out, uint32_t want_cells) {
-  hipLaunchKernelGGL(bt_unpermute, L256(grid), jobs, base, nj, out, cell_out, want_cells);
+               float* out, uint32_t* cell_out, uint32_t want_cells, uint32_t split) {
+  hipLaunchKernelGGL(bt_unpermute, L256(grid), jobs, base, nj, out, cell_out, want_cells, split);
 }
 void endpoints(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                const float* pts, uint32_t* table, uint32_t mask) {

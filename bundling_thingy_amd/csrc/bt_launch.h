@@ -23,8 +23,10 @@ void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
 void resample_count(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void path_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+// Current positions in original sample order: 12-byte records {x, y, edge id}
+// (split = 0) or n positions (x, y) followed by n edge ids (split = 1).
 void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
-               float* out, uint32_t* cell_out, uint32_t want_cells);
+               float* out, uint32_t* cell_out, uint32_t want_cells, uint32_t split = 0);
 void endpoints(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                const float* pts, uint32_t* table, uint32_t mask);
 }  // namespace btk

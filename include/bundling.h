@@ -237,6 +237,12 @@ int bt_upload_batch(bt_ctx* ctx, const bt_graph_desc* graphs, uint32_t n_graphs,
 
 int bt_read_paths_job(bt_ctx* ctx, uint32_t job, uint32_t* rec3, size_t n_paths);
 int bt_read_points_job(bt_ctx* ctx, uint32_t job, void* rec12, size_t n_samples);
+/* The samples of bt_read_points_job split into two arrays: n_samples
+ * positions {f32 x, f32 y} and n_samples edge ids (edge_ids may be NULL).
+ * No counterpart in the reference, whose Decoder reads the 12-byte records;
+ * for hosts that keep positions and ids apart (no host-side decode). */
+int bt_read_positions_job(bt_ctx* ctx, uint32_t job, float* xy, uint32_t* edge_ids,
+                          size_t n_samples);
 int bt_read_density_job(bt_ctx* ctx, uint32_t job, uint32_t kind, void* out, size_t n_pixels);
 int bt_job_samples(bt_ctx* ctx, uint32_t job, uint32_t* samples, uint32_t* iteration);
 

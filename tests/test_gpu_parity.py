@@ -326,6 +326,28 @@ def test_error_paths(bt, cpu_case):
         assert e.value.code == 3 and "sample buffer capacity" in e.value.msg
 
 
+def test_split_positions_equal_records(bt):
+    """bt_read_positions_job (positions and edge ids as two arrays) carries
+    the bits of bt_read_points_job's 12-byte records, job by job."""
+    from bundling_thingy_amd.api import lib, _f32p
+    graphs = workloads.batched_graphs(count=3, lo=60, hi=400)
+    opts = [workloads.batched_options(i, 4) for i in range(3)]
+    rec = np.dtype([("x", "<f4"), ("y", "<f4"), ("id", "<u4")])
+    with bt.Gpu(opts[0]) as gpu:
+        gpu.upload_batch(graphs, opts)
+        gpu.iterate(2)
+        for j in range(3):
+            r = gpu.points_raw(j).view(rec)
+            xy, ids = gpu.points(j)
+            np.testing.assert_array_equal(xy[:, 0].view(np.uint32), r["x"].view(np.uint32))
+            np.testing.assert_array_equal(xy[:, 1].view(np.uint32), r["y"].view(np.uint32))
+            np.testing.assert_array_equal(ids, r["id"])
+            xy2 = np.empty_like(xy)
+            assert lib().bt_read_positions_job(gpu._h, j, _f32p(xy2), None, len(xy2)) == 0
+            np.testing.assert_array_equal(xy2.view(np.uint32), xy.view(np.uint32))
+            assert lib().bt_read_positions_job(gpu._h, j, _f32p(xy2), None, len(xy2) - 1) == 1
+
+
 @pytest.mark.parametrize("points,starts,msg", [
     ([[0.2, 0.3]], [0, 1], "no drawable edges"),
     ([[0, 0], [1, 1], [2, 0], [3, 1]], [0, 3, 2, 4], "not sorted"),
