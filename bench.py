@@ -208,6 +208,69 @@ def render_section(bt, gpu, opts, reps=5):
             "ms": ms, "noisiness": m["noisiness"], "noisiness_ms": noise_ms}
 
 
+# FIFO for throughput; shortest-first with 1M sample-updates per tick for
+# latency (shorter ticks let small jobs through sooner, at some throughput)
+SERVER_POLICIES = (("server", "FIFO", 0), ("server_shortest", "SHORTEST", 1 << 20))
+
+
+def serve(bt, jobs, policy, tick):
+    """Feeds the arrival stream to a server in real time; returns (makespan s,
+    [(job index, latency ms)], ticks, (start, end) on the monotonic clock)."""
+    n_jobs = len(jobs)
+    srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy,
+                    tick_samples=tick)
+    clk0 = srv.clock_ms()
+    t0 = time.perf_counter()
+    pending, slat, i, done = {}, [], 0, 0
+    ticks = 0
+    while done < n_jobs:
+        now = time.perf_counter() - t0
+        while i < n_jobs and jobs[i][0] <= now:
+            ta, g, o, it = jobs[i]
+            pending[srv.submit(g, o, it)] = (ta, i)
+            i += 1
+        running, queued = srv.step()
+        ticks += 1
+        for tk in list(pending):
+            r = srv.poll(tk)
+            if r.state >= 2:
+                ta, k = pending.pop(tk)
+                slat.append((k, r.done_ms - (clk0 + ta * 1e3)))
+                srv.release(tk)
+                done += 1
+        if running == 0 and queued == 0 and i < n_jobs:
+            time.sleep(max(0.0, jobs[i][0] - (time.perf_counter() - t0)))
+    t1 = time.perf_counter()
+    srv.close()
+    return t1 - t0, slat, ticks, (t0, t1)
+
+
+def profile_batched(bt, workloads, n_graphs=256, iters=30):
+    """For rocprofv3 runs: upload the batched graphs, capture the schedule as
+    a hipGraph, rewind, replay it; prints the replay's window."""
+    graphs = workloads.batched_graphs(n_graphs)
+    opts = [workloads.batched_options(i, iters) for i in range(n_graphs)]
+    with bt.Gpu(opts[0]) as gpu:
+        s = gpu.upload_batch(graphs, opts)
+        gpu.iterate(iters, graph=True)
+        gpu.rewind()
+        t0 = time.perf_counter()
+        gpu.iterate(iters, graph=True)
+        t1 = time.perf_counter()
+    log(f"batched replay: {int(s.sum())} samples, {(t1 - t0) * 1e3:.1f} ms, "
+        f"window {t0:.4f} {t1:.4f}")
+
+
+def profile_served(bt, workloads, n_jobs=100, rate_hz=200.0):
+    """For rocprofv3 runs: the served stream under each server policy; prints
+    each run's window."""
+    jobs = workloads.served_jobs(n_jobs, rate_hz)
+    serve(bt, jobs[:3], bt.Server.FIFO, 0)  # warm the code paths
+    for name, policy, tick in SERVER_POLICIES:
+        srv_s, _, ticks, (t0, t1) = serve(bt, jobs, getattr(bt.Server, policy), tick)
+        log(f"served {name}: makespan {srv_s:.3f} s, {ticks} ticks, window {t0:.4f} {t1:.4f}")
+
+
 def served_section(bt, workloads, n_jobs, rate_hz=200.0):
     """Poisson arrivals at rate_hz: the plain per-job form (FIFO, one job at a
     time, as the drop-in does) against the continuous-batching server."""
@@ -235,36 +298,10 @@ def served_section(bt, workloads, n_jobs, rate_hz=200.0):
     log(f"served: plain FIFO makespan {plain_s:.2f} s")
     # continuous-batching server, both policies
     runs = {}
-    # FIFO for throughput; shortest-first with 1M sample-updates per tick for
-    # latency (shorter ticks let small jobs through sooner, at some throughput)
-    for name, policy, tick in (("server", bt.Server.FIFO, 0),
-                               ("server_shortest", bt.Server.SHORTEST, 1 << 20)):
-        srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy,
-                        tick_samples=tick)
-        clk0 = srv.clock_ms()
-        t0 = time.perf_counter()
-        pending, slat, i, done = {}, [], 0, 0
-        ticks = 0
-        while done < n_jobs:
-            now = time.perf_counter() - t0
-            while i < n_jobs and jobs[i][0] <= now:
-                ta, g, o, it = jobs[i]
-                pending[srv.submit(g, o, it)] = (ta, i)
-                i += 1
-            running, queued = srv.step()
-            ticks += 1
-            for tk in list(pending):
-                r = srv.poll(tk)
-                if r.state >= 2:
-                    ta, k = pending.pop(tk)
-                    slat.append((k, r.done_ms - (clk0 + ta * 1e3)))
-                    srv.release(tk)
-                    done += 1
-            if running == 0 and queued == 0 and i < n_jobs:
-                time.sleep(max(0.0, jobs[i][0] - (time.perf_counter() - t0)))
-        runs[name] = (time.perf_counter() - t0, slat, ticks)
-        srv.close()
-        log(f"served: {name} makespan {runs[name][0]:.2f} s in {ticks} ticks")
+    for name, policy, tick in SERVER_POLICIES:
+        srv_s, slat, ticks, _ = serve(bt, jobs, getattr(bt.Server, policy), tick)
+        runs[name] = (srv_s, slat, ticks)
+        log(f"served: {name} makespan {srv_s:.2f} s in {ticks} ticks")
     small = set(k for k, (_, g, _, _) in enumerate(jobs) if g.edge_count <= 20000)
     out = {"jobs": n_jobs, "rate_hz": rate_hz, "updates": total_updates,
            "plain_fifo": {"updates_per_s": total_updates / plain_s, "makespan_s": plain_s,
@@ -296,6 +333,12 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="for rocprofv3 counter runs: a 256 MiB device copy (calibration), "
                          "then upload + autotune + one eager 30-iteration schedule")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="for rocprofv3 kernel traces: the batched schedule (capture, rewind, "
+                         "replay), printing the replay's window on the monotonic clock")
+    ap.add_argument("--profile-served", action="store_true",
+                    help="for rocprofv3 kernel traces: the served stream under each server "
+                         "policy, printing each run's window on the monotonic clock")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -305,6 +348,13 @@ def main():
 
     import bundling_thingy_amd as bt
     from bundling_thingy_amd import workloads
+
+    if args.profile_batched or args.profile_served:
+        if args.profile_batched:
+            profile_batched(bt, workloads)
+        if args.profile_served:
+            profile_served(bt, workloads)
+        return
 
     graph = workloads.plain_gpu_graph()
     opts = workloads.plain_gpu_options(ITERS, cell_split=args.cell_split)

@@ -6,6 +6,13 @@
       average duration of the last n dispatches; <kernels> is one name or a
       comma-separated list whose averages are added (one stage made of
       several launches, each dispatched once per stage)
+  python -m bundling_thingy_amd.profstats series <kernel_trace.csv> <kernel> <n>
+      durations (ms) of the last n dispatches of one kernel, in order
+  python -m bundling_thingy_amd.profstats window <kernel_trace.csv> <t0> <t1>
+      the dispatches that start between t0 and t1 (seconds of the monotonic
+      clock, as bench.py prints them): span from the first start to the
+      last end, busy time (union of the dispatch intervals), idle gaps, and
+      per-kernel totals
   python -m bundling_thingy_amd.profstats traffic <fetch.csv> <write.csv> <kernels> <n> <copy_bytes>
       HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
       each scaled by what the counter reads for HIP's device copy of
@@ -53,6 +60,43 @@ def last(path, kernels, n):
     return total
 
 
+def window(path, t0, t1):
+    iv = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            if t0 * 1e9 <= a <= t1 * 1e9:
+                iv.append((a, b, r["Kernel_Name"]))
+    if not iv:
+        return "no dispatches in the window"
+    iv.sort()
+    busy, gaps = 0, []
+    cs, ce = iv[0][0], iv[0][1]
+    for a, b, _ in iv[1:]:
+        if a > ce:
+            busy += ce - cs
+            gaps.append(a - ce)
+            cs, ce = a, b
+        else:
+            ce = max(ce, b)
+    busy += ce - cs
+    span = max(b for _, b, _ in iv) - iv[0][0]
+    gaps.sort()
+    big = [g for g in gaps if g > 500_000]
+    med = gaps[len(gaps) // 2] if gaps else 0
+    agg = {}
+    for a, b, name in iv:
+        agg[name] = agg.get(name, 0) + (b - a)
+    lines = [f"dispatches {len(iv)}, span {span / 1e6:.1f} ms, busy {busy / 1e6:.1f} ms "
+             f"({100.0 * busy / span:.0f} %)",
+             f"idle gaps: {len(gaps)}, {sum(gaps) / 1e6:.1f} ms in all, median {med / 1e3:.1f} us; "
+             f"{len(big)} over 0.5 ms, {sum(big) / 1e6:.1f} ms",
+             "| kernel | total ms | share of busy |", "|---|---|---|"]
+    for name, t in sorted(agg.items(), key=lambda kv: -kv[1]):
+        lines.append(f"| {name[:48]} | {t / 1e6:.2f} | {100.0 * t / busy:.1f} % |")
+    return "\n".join(lines)
+
+
 def _counter(path, name):
     with open(path) as f:
         return [r for r in csv.DictReader(f) if r["Counter_Name"] == name]
@@ -83,6 +127,11 @@ if __name__ == "__main__":
         print(stats(sys.argv[2]))
     elif cmd == "last":
         print(f"{last(sys.argv[2], sys.argv[3], int(sys.argv[4])):.4f} ms")
+    elif cmd == "series":
+        d = [t for name, t in _trace_rows(sys.argv[2]) if name == sys.argv[3]][-int(sys.argv[4]):]
+        print(" ".join(f"{t / 1e6:.2f}" for t in d))
+    elif cmd == "window":
+        print(window(sys.argv[2], float(sys.argv[3]), float(sys.argv[4])))
     elif cmd == "traffic":
         print(json.dumps(traffic(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]),
                                  int(sys.argv[6])), indent=1))
