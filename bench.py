@@ -83,16 +83,17 @@ def cpu_baseline(graph, opts, target_s=15.0):
     stride = int(max(1, min(coarse, np.ceil(coarse * tm / max(target_s - td, 1.0)))))
     total, tm, td = schedule(stride)
     n = len(pts)
-    # the same on one thread, over a sparser sample (about a quarter of the time)
-    stride1 = int(max(stride, np.ceil(stride * threads / 4)))
+    # the same on one thread over a sample `threads` times sparser, so it takes
+    # about as long as the timed pass
+    stride1 = int(stride * threads)
     total1, _, _ = schedule(stride1, threads=1)
     return {"value": ITERS * n / total, "unit": "sample-point updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle neighbor means of every {stride}th sample for all {ITERS} "
+            "sample": f"oracle neighbor means of one sample in {stride} for all {ITERS} "
                       f"radii at the initial positions (x{stride}), full splat/blur/normalize; "
                       f"est. {total:.1f} s per schedule",
             "value_1thread": ITERS * n / total1,
-            "sample_1thread": f"every {stride1}th sample, 1 thread; est. {total1:.1f} s per "
+            "sample_1thread": f"one sample in {stride1}, 1 thread; est. {total1:.1f} s per "
                               f"schedule"}
 
 
