@@ -73,6 +73,9 @@ struct __attribute__((aligned(16))) BtJob {
   uint32_t parity;        // which rec[] holds the current positions
   uint32_t capacity;      // sample capacity (reference budget model)
   const float* wt;        // blur weights [taps + 1]
+  // ---- saved state (rewind source; the served form stages results in rec_saved)
+  float* rec_saved;       // samples after upload, {x, y, id} records
+  float* dens_saved;      // density planes after upload (counts, row, col, norm), or null
 };
 
 // Per-launch work description: for each kernel kind, the first workgroup of

@@ -50,8 +50,8 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   d.dens_norm = dens_norm.as<float>();
   d.peak = peak.as<uint32_t>();
   d.status = status.as<uint32_t>();
-  d.ctrl = ctrl.as<float>();
-  d.starts = starts.as<uint32_t>();
+  d.ctrl = in_ctrl;
+  d.starts = in_starts;
   d.path_scan = path_scan.as<uint64_t>();
   d.paths = paths.as<uint32_t>();
   d.n = n;
@@ -88,6 +88,8 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   d.parity = parity_at;
   d.capacity = (uint32_t)std::min<uint64_t>(capacity, 0xfffffffeull);
   d.wt = wt.as<float>() ? wt.as<float>() + (size_t)std::min(it, sched - 1) * kWt : nullptr;
+  d.rec_saved = rec_saved.as<float>();
+  d.dens_saved = dens_saved.as<float>();
   return d;
 }
 
@@ -109,7 +111,7 @@ bool Job::set_split(uint32_t e, uint32_t k) {
 }
 
 void Job::set_pool(DevPool* pool) {
-  DevBuf* all[] = {&ctrl, &starts, &path_scan, &paths, &bsum, &status, &rec0,
+  DevBuf* all[] = {&path_scan, &paths, &bsum, &status, &rec0,
                    &rec1, &rec_saved, &sorted, &sid, &cellrank, &cells, &edge_of,
                    &pre_x, &pre_y, &dens_cnt, &dens_row, &dens_col, &dens_norm, &dens_saved,
                    &peak, &wt};
@@ -117,7 +119,7 @@ void Job::set_pool(DevPool* pool) {
 }
 
 size_t Job::device_bytes() const {
-  const DevBuf* all[] = {&ctrl, &starts, &path_scan, &paths, &bsum, &status, &rec0,
+  const DevBuf* all[] = {&path_scan, &paths, &bsum, &status, &rec0,
                          &rec1, &rec_saved, &sorted, &sid, &cellrank, &cells, &edge_of,
                          &pre_x, &pre_y,
                          &dens_cnt, &dens_row, &dens_col, &dens_norm, &dens_saved, &peak, &wt};
@@ -160,6 +162,7 @@ Engine::~Engine() {
     if (s.ev) (void)hipEventDestroy(s.ev);
     if (s.p) (void)hipHostFree(s.p);
   }
+  if (up_host_) (void)hipHostFree(up_host_);
 }
 
 void Engine::sync() { HIPCHK(hipStreamSynchronize(stream_)); }
@@ -365,6 +368,7 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
   BtControl* const ctl = side ? ctl_side_.as<BtControl>() : ctl_.as<BtControl>();
   std::vector<std::unique_ptr<Job>> nj;
   std::vector<Job*> js;
+  std::vector<std::vector<float>> wts;
   for (const GraphIn& g : gs) {
     auto j = std::make_unique<Job>();
     j->set_pool(pool_.get());
@@ -408,8 +412,6 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
       fail(BT_E_CAPACITY, "neighbor grid exceeds GPU dispatch limits");
     if (j->n_paths + 1ull > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS)
       fail(BT_E_CAPACITY, "paths exceed GPU dispatch limits");
-    j->ctrl.alloc((size_t)g.n_points * 8);
-    j->starts.alloc((size_t)g.n_starts * 4);
     j->path_scan.alloc((size_t)g.n_starts * 8);
     j->paths.alloc((size_t)std::max(j->n_paths, 1u) * 12);
     j->status.alloc(16);
@@ -417,15 +419,56 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
                                  cdiv(j->max_cells + 1ull, BT_SCAN_ITEMS));
     j->bsum.alloc((size_t)nb * 8);
     j->wt.alloc(wt.size() * 4);
-    HIPCHK(hipMemcpyAsync(j->ctrl.p, g.xy, (size_t)g.n_points * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(j->starts.p, g.starts, (size_t)g.n_starts * 4, hipMemcpyHostToDevice,
-                          st));
-    HIPCHK(hipMemcpyAsync(j->wt.p, wt.data(), wt.size() * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemsetAsync(j->status.p, 0, 16, st));
+    wts.push_back(std::move(wt));
     js.push_back(j.get());
     nj.push_back(std::move(j));
   }
   if (js.empty()) fail(BT_E_INVALID, "no graphs");
+  // Every job's inputs in one copy through pinned memory. The upload area
+  // holds a (weights offset, weights length) word pair per job, then each
+  // job's control points, path starts and blur weights, 64-word aligned; the
+  // set-up kernel moves the weights to the jobs' own buffers and clears
+  // their status words.
+  const size_t njs = js.size();
+  std::vector<size_t> off(3 * njs);
+  size_t words = align_up(2 * njs, 64);
+  for (size_t i = 0; i < njs; ++i) {
+    off[3 * i] = words;
+    words += align_up(2ull * gs[i].n_points, 64);
+    off[3 * i + 1] = words;
+    words += align_up(gs[i].n_starts, 64);
+    off[3 * i + 2] = words;
+    words += align_up(wts[i].size(), 64);
+  }
+  if (words > 0xffffffffull) fail(BT_E_CAPACITY, "upload exceeds 2^32 words");
+  if (up_host_bytes_ < words * 4) {
+    if (up_host_) HIPCHK(hipHostFree(up_host_));
+    up_host_ = nullptr;
+    up_host_bytes_ = 0;
+    const size_t b = align_up(std::max<size_t>(words * 4, 1 << 20), 1 << 20);
+    HIPCHK(hipHostMalloc(&up_host_, b, hipHostMallocDefault));
+    up_host_bytes_ = b;
+  }
+  uint32_t* host = static_cast<uint32_t*>(up_host_);
+  for (size_t i = 0; i < njs; ++i) {
+    host[2 * i] = (uint32_t)off[3 * i + 2];
+    host[2 * i + 1] = (uint32_t)wts[i].size();
+    std::memcpy(host + off[3 * i], gs[i].xy, (size_t)gs[i].n_points * 8);
+    std::memcpy(host + off[3 * i + 1], gs[i].starts, (size_t)gs[i].n_starts * 4);
+    std::memcpy(host + off[3 * i + 2], wts[i].data(), wts[i].size() * 4);
+  }
+  DevBuf area;
+  area.pool = pool_.get();
+  area.alloc(words * 4);
+  HIPCHK(hipMemcpyAsync(area.p, host, words * 4, hipMemcpyHostToDevice, st));
+  for (size_t i = 0; i < njs; ++i) {
+    js[i]->in_ctrl = reinterpret_cast<const float*>(area.as<uint32_t>() + off[3 * i]);
+    js[i]->in_starts = area.as<uint32_t>() + off[3 * i + 1];
+  }
+  // per-job words read back by the host (totals, then status)
+  DevBuf jw;
+  jw.pool = pool_.get();
+  jw.alloc(njs * 8);
   // Pass 1: per-path sample counts and their exclusive scan (count_paths,
   // resampling.wgsl:138-159, with offsets from a scan instead of atomics).
   std::vector<uint32_t> zeros(js.size(), 0);
@@ -440,14 +483,14 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
     auto B = [&](int k) {
       return reinterpret_cast<const uint32_t*>(dev + p1.base_off) + (size_t)k * (p1.njobs + 1);
     };
+    btk::upload_setup(st, J, (int)p1.njobs, area.as<uint32_t>());
     btk::resample_count(st, p1.grid[BT_K_PATH], J, B(BT_K_PATH), (int)p1.njobs);
     btk::path_scan(st, p1.grid[BT_K_PSCAN], J, B(BT_K_PSCAN), (int)p1.njobs);
+    btk::job_words(st, J, (int)p1.njobs, 0u, jw.as<unsigned long long>());
     HIPCHK(hipGetLastError());
   }
-  std::vector<uint64_t> totals(js.size());
-  for (size_t i = 0; i < js.size(); ++i)
-    HIPCHK(hipMemcpyAsync(&totals[i], js[i]->path_scan.as<uint64_t>() + js[i]->n_paths, 8,
-                          hipMemcpyDeviceToHost, st));
+  std::vector<uint64_t> totals(njs);
+  HIPCHK(hipMemcpyAsync(totals.data(), jw.p, njs * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   for (size_t i = 0; i < js.size(); ++i) {
     Job& j = *js[i];
@@ -508,26 +551,23 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
       btk::blur_cols(st, p2.grid[BT_K_COL], J, B(BT_K_COL), (int)n2);
       btk::normalize(st, p2.grid[BT_K_NORM], J, B(BT_K_NORM), (int)n2);
     }
+    btk::job_words(st, J, (int)n2, 1u, jw.as<unsigned long long>());
+    // the state after upload: rewinds return to it (the served form stages
+    // its results in rec_saved instead)
+    btk::save_points(st, p2.grid[BT_K_BIN], J, B(BT_K_BIN), (int)n2, 0);
+    if (rewindable && p2.grid[BT_K_NORM])
+      btk::save_density(st, p2.grid[BT_K_NORM], J, B(BT_K_NORM), (int)n2, 0);
     HIPCHK(hipGetLastError());
   }
-  std::vector<uint32_t> status(js.size() * 4);
-  for (size_t i = 0; i < js.size(); ++i)
-    HIPCHK(hipMemcpyAsync(&status[i * 4], js[i]->status.p, 16, hipMemcpyDeviceToHost, st));
-  for (Job* j : js) {
-    HIPCHK(hipMemcpyAsync(j->rec_saved.p, j->rec0.p, (size_t)j->n * 12, hipMemcpyDeviceToDevice,
-                          st));
-    if (j->dens_cnt.p && rewindable) {
-      const size_t pb = (size_t)j->dw * j->dh * 4;
-      char* s = j->dens_saved.as<char>();
-      HIPCHK(hipMemcpyAsync(s, j->dens_cnt.p, pb, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpyAsync(s + pb, j->dens_row.p, pb, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpyAsync(s + 2 * pb, j->dens_col.p, pb, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpyAsync(s + 3 * pb, j->dens_norm.p, pb, hipMemcpyDeviceToDevice, st));
-    }
-  }
+  std::vector<uint64_t> status(njs);
+  HIPCHK(hipMemcpyAsync(status.data(), jw.p, njs * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  for (size_t i = 0; i < js.size(); ++i)
-    if (status[i * 4] & 2u) fail(BT_E_MISMATCH, "GPU resampling count and write passes disagree");
+  for (Job* j : js) {
+    j->in_ctrl = nullptr;  // the upload area is released on return
+    j->in_starts = nullptr;
+  }
+  for (size_t i = 0; i < njs; ++i)
+    if (status[i] & 2u) fail(BT_E_MISMATCH, "GPU resampling count and write passes disagree");
   for (auto& j : nj) {
     j->iteration = 0;
     j->parity = 0;
@@ -537,17 +577,22 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
 
 void Engine::rewind() {
   if (jobs_.empty()) fail(BT_E_NO_GRAPH, "no uploaded graph");
-  for (auto& j : jobs_) {
-    HIPCHK(hipMemcpyAsync(j->rec0.p, j->rec_saved.p, (size_t)j->n * 12, hipMemcpyDeviceToDevice,
-                          stream_));
-    if (j->dens_cnt.p && j->dens_saved.p) {
-      const size_t pb = (size_t)j->dw * j->dh * 4;
-      const char* s = j->dens_saved.as<char>();
-      HIPCHK(hipMemcpyAsync(j->dens_cnt.p, s, pb, hipMemcpyDeviceToDevice, stream_));
-      HIPCHK(hipMemcpyAsync(j->dens_row.p, s + pb, pb, hipMemcpyDeviceToDevice, stream_));
-      HIPCHK(hipMemcpyAsync(j->dens_col.p, s + 2 * pb, pb, hipMemcpyDeviceToDevice, stream_));
-      HIPCHK(hipMemcpyAsync(j->dens_norm.p, s + 3 * pb, pb, hipMemcpyDeviceToDevice, stream_));
-    }
+  std::vector<Job*> js = all_jobs();
+  Plan p = make_plan(js, std::vector<uint32_t>(js.size(), 0u));
+  DevBuf table;
+  table.pool = pool_.get();
+  table.alloc(p.host.size());
+  put_plan(p, table.p);
+  const unsigned char* dev = table.as<unsigned char>();
+  const BtJob* J = reinterpret_cast<const BtJob*>(dev);
+  auto B = [&](int k) {
+    return reinterpret_cast<const uint32_t*>(dev + p.base_off) + (size_t)k * (p.njobs + 1);
+  };
+  btk::save_points(stream_, p.grid[BT_K_BIN], J, B(BT_K_BIN), (int)p.njobs, 1);
+  if (p.grid[BT_K_NORM])
+    btk::save_density(stream_, p.grid[BT_K_NORM], J, B(BT_K_NORM), (int)p.njobs, 1);
+  HIPCHK(hipGetLastError());
+  for (Job* j : js) {
     j->iteration = 0;
     j->parity = 0;
   }

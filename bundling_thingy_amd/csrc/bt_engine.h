@@ -23,7 +23,10 @@ struct Job {
   uint32_t sched = 1;           // distinct parameter sets
   std::vector<bt_iter_params> params;
   std::vector<uint32_t> splits; // grid cells per radius, per schedule entry
-  DevBuf ctrl, starts, path_scan, paths, bsum, status;
+  // control points and path starts: in the upload area, during the upload only
+  const float* in_ctrl = nullptr;
+  const uint32_t* in_starts = nullptr;
+  DevBuf path_scan, paths, bsum, status;
   DevBuf rec0, rec1, rec_saved, sorted, sid, cellrank, cells, edge_of, pre_x, pre_y;
   DevBuf dens_cnt, dens_row, dens_col, dens_norm, dens_saved, peak, wt;
 
@@ -140,6 +143,8 @@ class Engine {
     bool pending = false;
   };
   std::vector<Pinned> pinned_;
+  void* up_host_ = nullptr;  // pinned staging of the upload inputs
+  size_t up_host_bytes_ = 0;
   uint32_t pinned_next_ = 0;
   uint32_t retire_next_ = 0;
   size_t slot_bytes_ = 0;

@@ -1514,6 +1514,80 @@ extern "C" __global__ __launch_bounds__(256) void bt_unpermute(const BtJob* __re
   }
 }
 
+// --------------------------------------------------- upload / rewind set-up
+// Per-job set-up of an upload from the packed upload area (one H2D copy for
+// all jobs): each job's blur weights go to its own buffer and its status
+// words are cleared. area[2j], area[2j + 1]: word offset and length of job
+// j's weights. One workgroup per job.
+extern "C" __global__ __launch_bounds__(256) void bt_upload_setup(const BtJob* __restrict__ jobs,
+                                                                  int njobs,
+                                                                  const uint32_t* __restrict__ area) {
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  const BtJob& J = jobs[j];
+  const uint32_t off = area[2 * j], len = area[2 * j + 1];
+  float* wt = const_cast<float*>(J.wt);
+  for (uint32_t i = threadIdx.x; i < len; i += 256u) wt[i] = __uint_as_float(area[off + i]);
+  if (threadIdx.x < 4u) J.status[threadIdx.x] = 0u;
+}
+
+// One 64-bit word per job for the host: the path scan's total (which = 0:
+// samples << 32 | drawable paths) or the first status word (which = 1).
+extern "C" __global__ __launch_bounds__(256) void bt_job_words(const BtJob* __restrict__ jobs,
+                                                               int njobs, uint32_t which,
+                                                               unsigned long long* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= njobs) return;
+  const BtJob& J = jobs[j];
+  out[j] = which == 0u ? (unsigned long long)J.path_scan[J.n_paths]
+                       : (unsigned long long)J.status[0];
+}
+
+// The samples after upload: rec[0] -> rec_saved (restore = 0) or back.
+extern "C" __global__ __launch_bounds__(256) void bt_save_points(const BtJob* __restrict__ jobs,
+                                                                 const uint32_t* __restrict__ base,
+                                                                 int njobs, int restore) {
+  JOB_PROLOGUE(BT_K_BIN);
+  const uint32_t i = lb * BT_BIN_BLOCK + threadIdx.x;
+  if (i >= J.n || !J.rec_saved) return;
+  float* live = J.rec[0] + 3u * i;
+  float* saved = J.rec_saved + 3u * i;
+  const float* src = restore ? saved : live;
+  float* dst = restore ? live : saved;
+  dst[0] = src[0];
+  dst[1] = src[1];
+  dst[2] = src[2];
+}
+
+// The four density planes after upload: counts, row, column and normalized
+// planes -> dens_saved (restore = 0) or back.
+extern "C" __global__ __launch_bounds__(256) void bt_save_density(const BtJob* __restrict__ jobs,
+                                                                  const uint32_t* __restrict__ base,
+                                                                  int njobs, int restore) {
+  JOB_PROLOGUE(BT_K_NORM);
+  if (!J.dens_saved) return;
+  const uint32_t npix = J.dw * J.dh;
+  uint32_t* planes[4] = {J.dens_cnt, reinterpret_cast<uint32_t*>(J.dens_row),
+                         reinterpret_cast<uint32_t*>(J.dens_col),
+                         reinterpret_cast<uint32_t*>(J.dens_norm)};
+  uint32_t* saved = reinterpret_cast<uint32_t*>(J.dens_saved);
+  const uint32_t i0 = lb * BT_NORM_ITEMS + threadIdx.x;
+#pragma unroll
+  for (uint32_t k = 0; k < BT_NORM_ITEMS / 256u; ++k) {
+    const uint32_t i = i0 + k * 256u;
+    if (i >= npix) continue;
+#pragma unroll
+    for (uint32_t q = 0; q < 4u; ++q) {
+      uint32_t* live = planes[q] + i;
+      uint32_t* copy = saved + (size_t)q * npix + i;
+      if (restore)
+        *live = *copy;
+      else
+        *copy = *live;
+    }
+  }
+}
+
 // ------------------------------------------------------------- endpoints
 // endpoints.wgsl:9-49 (deduplicate, for --endpoint-dots): the distinct
 // positions among the first and last samples of the paths with two or more
@@ -1633,6 +1707,22 @@ void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32
 void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                float* out, uint32_t* cell_out, uint32_t want_cells, uint32_t split) {
   hipLaunchKernelGGL(bt_unpermute, L256(grid), jobs, base, nj, out, cell_out, want_cells, split);
+}
+void upload_setup(hipStream_t s, const BtJob* jobs, int nj, const uint32_t* area) {
+  hipLaunchKernelGGL(bt_upload_setup, dim3((unsigned)nj), dim3(256), 0, s, jobs, nj, area);
+}
+void job_words(hipStream_t s, const BtJob* jobs, int nj, uint32_t which,
+               unsigned long long* out) {
+  hipLaunchKernelGGL(bt_job_words, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, s, jobs, nj,
+                     which, out);
+}
+void save_points(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+                 int restore) {
+  hipLaunchKernelGGL(bt_save_points, L256(grid), jobs, base, nj, restore);
+}
+void save_density(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+                  int restore) {
+  hipLaunchKernelGGL(bt_save_density, L256(grid), jobs, base, nj, restore);
 }
 void endpoints(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                const float* pts, uint32_t* table, uint32_t mask) {

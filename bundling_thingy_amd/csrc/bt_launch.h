@@ -27,6 +27,15 @@ void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32
 // (split = 0) or n positions (x, y) followed by n edge ids (split = 1).
 void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                float* out, uint32_t* cell_out, uint32_t want_cells, uint32_t split = 0);
+// Upload set-up (weights from the packed area, status cleared), one workgroup per job.
+void upload_setup(hipStream_t s, const BtJob* jobs, int nj, const uint32_t* area);
+// One 64-bit word per job: path-scan total (which = 0) or status word 0 (which = 1).
+void job_words(hipStream_t s, const BtJob* jobs, int nj, uint32_t which, unsigned long long* out);
+// Saved state for rewinds (restore = 1 copies it back).
+void save_points(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+                 int restore);
+void save_density(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+                  int restore);
 void endpoints(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                const float* pts, uint32_t* table, uint32_t mask);
 }  // namespace btk
