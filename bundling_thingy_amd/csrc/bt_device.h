@@ -72,7 +72,7 @@ struct __attribute__((aligned(16))) BtJob {
   float norm_min_x, norm_min_y, norm_scale, norm_tr_x, norm_tr_y;  // Graph::normalization
   uint32_t parity;        // which rec[] holds the current positions
   uint32_t capacity;      // sample capacity (reference budget model)
-  const float* wt;        // blur weights [taps + 1]
+  float* wt;              // blur weights [taps + 1] (written once, by the upload set-up)
   // ---- saved state (rewind source; the served form stages results in rec_saved)
   float* rec_saved;       // samples after upload, {x, y, id} records
   float* dens_saved;      // density planes after upload (counts, row, col, norm), or null

@@ -1526,8 +1526,7 @@ extern "C" __global__ __launch_bounds__(256) void bt_upload_setup(const BtJob* _
   if (j >= njobs) return;
   const BtJob& J = jobs[j];
   const uint32_t off = area[2 * j], len = area[2 * j + 1];
-  float* wt = const_cast<float*>(J.wt);
-  for (uint32_t i = threadIdx.x; i < len; i += 256u) wt[i] = __uint_as_float(area[off + i]);
+  for (uint32_t i = threadIdx.x; i < len; i += 256u) J.wt[i] = __uint_as_float(area[off + i]);
   if (threadIdx.x < 4u) J.status[threadIdx.x] = 0u;
 }
 
