@@ -642,6 +642,9 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
         if (!jobs[i]) continue;
         ServedJob* s = admit[i];
         s->job = std::move(jobs[i]);
+        // the density the server returns is that of the last iteration; the
+        // earlier ones are never read, so they are not computed
+        s->job->density_from = s->iterations - 1;
         s->state = 1;
         s->admit_ms = t;
         s->last_ms = t;

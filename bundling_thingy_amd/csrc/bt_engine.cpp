@@ -44,11 +44,13 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   d.pre_x = pre_x.as<uint64_t>();
   d.pre_y = pre_y.as<uint64_t>();
   d.edge_of = edge_of.as<uint32_t>();
-  d.dens_cnt = dens_cnt.as<uint32_t>();
+  // density stages run for this iteration (kernels test dens_cnt, dw x dh, peak)
+  const bool density = dens_cnt.p != nullptr && it >= density_from;
+  d.dens_cnt = density ? dens_cnt.as<uint32_t>() : nullptr;
   d.dens_row = dens_row.as<float>();
   d.dens_col = dens_col.as<float>();
   d.dens_norm = dens_norm.as<float>();
-  d.peak = peak.as<uint32_t>();
+  d.peak = density ? peak.as<uint32_t>() : nullptr;
   d.status = status.as<uint32_t>();
   d.ctrl = in_ctrl;
   d.starts = in_starts;
@@ -56,7 +58,6 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   d.paths = paths.as<uint32_t>();
   d.n = n;
   d.n_paths = n_paths;
-  const bool density = dens_cnt.p != nullptr;
   d.dw = density ? dw : 0;
   d.dh = density ? dh : 0;
   const bt_iter_params& p = at(it);

@@ -20,6 +20,10 @@ struct Job {
   uint32_t dw = 0, dh = 0;
   uint32_t max_cells = 1;
   uint32_t iteration = 0, parity = 0;
+  // first iteration that splats, blurs and normalizes the density (0: every
+  // one, as Gpu::iterate does; the served form computes only the density it
+  // returns, that of a job's last iteration)
+  uint32_t density_from = 0;
   uint32_t sched = 1;           // distinct parameter sets
   std::vector<bt_iter_params> params;
   std::vector<uint32_t> splits; // grid cells per radius, per schedule entry

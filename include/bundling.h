@@ -334,6 +334,9 @@ int bt_server_submit(bt_server* srv, const float* xy, uint32_t n_points, const u
 int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued);
 
 int bt_server_poll(bt_server* srv, uint64_t ticket, bt_job_result* out);
+/* A finished job's results: its samples after the last iteration, and the
+ * normalized density of that iteration (the only density a job computes on
+ * the server; bits equal to running the job alone through bt_iterate). */
 int bt_server_read_points(bt_server* srv, uint64_t ticket, void* rec12, size_t n_samples);
 int bt_server_read_density(bt_server* srv, uint64_t ticket, uint32_t kind, void* out,
                            size_t n_pixels);

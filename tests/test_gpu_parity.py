@@ -382,18 +382,20 @@ def test_rejected_upload_drops_the_graph(bt, cpu_case, points, starts, msg):
 
 @pytest.mark.parametrize("policy,tick", [(0, 0), (1, 0), (1, 1)])
 def test_server_results_match_alone(bt, policy, tick):
-    """Co-scheduled jobs give the bits of each job alone, under both policies
-    (tick 1: the shortest-first policy advances one job per tick)."""
+    """Co-scheduled jobs give the bits of each job alone, points and final
+    density, under both policies (tick 1: the shortest-first policy advances
+    one job per tick)."""
     graphs = workloads.batched_graphs(count=4, lo=60, hi=400)
     opts = bt.Options(spacing=0.004, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=256,
                       schedule_len=6)
     iters = [6, 3, 5, 4]
-    alone = []
+    alone, alone_d = [], []
     for g, it in zip(graphs, iters):
         with bt.Gpu(opts) as gpu:
             gpu.upload(g)
             gpu.iterate(it)
             alone.append(gpu.points()[0])
+            alone_d.append(gpu.density())
     srv = bt.Server(opts, memory_budget=1 << 30, max_inflight=3, policy=policy,
                     tick_samples=tick)
     tickets = [srv.submit(g, opts, it) for g, it in zip(graphs, iters)]
@@ -401,11 +403,15 @@ def test_server_results_match_alone(bt, policy, tick):
         r, q = srv.step()
         if r == 0 and q == 0:
             break
-    for t, ref in zip(tickets, alone):
+    for t, ref, ref_d in zip(tickets, alone, alone_d):
         res = srv.poll(t)
-        assert res.state == 2
+        assert res.state == 2 and res.iterations_done == res.iterations_total
         p, _ = srv.points(t, res.samples)
         np.testing.assert_array_equal(p.view(np.uint32), ref.view(np.uint32))
+        # the server computes only the last iteration's density
+        d = srv.density(t, ref_d.shape[1], ref_d.shape[0])
+        assert d.max() == 1.0
+        np.testing.assert_array_equal(d.view(np.uint32), ref_d.view(np.uint32))
     srv.close()
 
 
