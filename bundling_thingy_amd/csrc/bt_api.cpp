@@ -387,7 +387,7 @@ struct ServedJob {
 }  // namespace
 
 static const double kStarveMs = 1000.0;       // SHORTEST: a job waiting longer goes first
-static const size_t kTicksAhead = 2;          // ticks queued on the GPU at most
+static const size_t kTicksAhead = 4;          // ticks queued on the GPU at most
 static const uint64_t kTickSamples = 4ull << 20;  // SHORTEST: default sample-updates per tick
 
 struct bt_server {

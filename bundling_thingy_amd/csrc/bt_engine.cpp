@@ -18,9 +18,10 @@ static inline size_t align_up(size_t a, size_t b) { return (a + b - 1) / b * b; 
 
 static const uint32_t kWt = BT_MAX_TAPS + 1;  // weights per iteration row
 
-// Pinned host slots for plan uploads, and device slots for one-job readback
-// plans.
-static const uint32_t kPinnedSlots = 4;
+// Pinned host slots for plan uploads (the server queues up to 4 ticks ahead,
+// each with its plan, besides admissions' plans), and device slots for
+// one-job readback plans.
+static const uint32_t kPinnedSlots = 16;
 static const uint32_t kRetireSlots = 64;
 static const size_t kRetireSlotBytes = 2048;
 
