@@ -311,6 +311,37 @@ def test_density_disabled_matches_enabled(bt, cpu_case):
     np.testing.assert_array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
 
 
+def test_rewind_mixed_density_batch(bt):
+    """A batch mixing jobs with and without a density map: rewinding
+    restores every job's samples (and density where it has one), so a
+    second eager schedule and a graph replay give the first run's bits."""
+    graphs = workloads.batched_graphs(count=4, lo=60, hi=400)
+    opts = [bt.Options(spacing=0.004, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=256,
+                       schedule_len=4, color_by_density=(i % 2 == 0)) for i in range(4)]
+    with bt.Gpu(opts[0]) as gpu:
+        gpu.upload_batch(graphs, opts)
+        up = [gpu.points(j)[0] for j in range(4)]
+        up_d = [gpu.density(3, j) for j in (0, 2)]
+        gpu.iterate(4)
+        first = [gpu.points(j)[0] for j in range(4)]
+        first_d = [gpu.density(3, j) for j in (0, 2)]
+        for graph in (False, True):
+            gpu.rewind()
+            for j in range(4):
+                np.testing.assert_array_equal(gpu.points(j)[0].view(np.uint32),
+                                              up[j].view(np.uint32))
+            for k, j in enumerate((0, 2)):
+                np.testing.assert_array_equal(gpu.density(3, j).view(np.uint32),
+                                              up_d[k].view(np.uint32))
+            gpu.iterate(4, graph=graph)
+            for j in range(4):
+                np.testing.assert_array_equal(gpu.points(j)[0].view(np.uint32),
+                                              first[j].view(np.uint32))
+            for k, j in enumerate((0, 2)):
+                np.testing.assert_array_equal(gpu.density(3, j).view(np.uint32),
+                                              first_d[k].view(np.uint32))
+
+
 def test_error_paths(bt, cpu_case):
     with bt.Gpu(bt.Options(map_size=201)) as gpu:
         with pytest.raises(bt.BtError) as e:
