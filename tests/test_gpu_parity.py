@@ -446,6 +446,39 @@ def test_server_results_match_alone(bt, policy, tick):
     srv.close()
 
 
+def test_server_isolates_a_failed_upload(bt):
+    """A job whose resampled samples exceed its sample budget (2 MiB of
+    graph memory: room for 15,932 samples against its 30,217) fails alone,
+    state 3, when its admission batch's counts come back; the jobs admitted
+    with it finish with the bits of each job alone."""
+    graphs = workloads.batched_graphs(count=4, lo=60, hi=400)
+    kw = dict(spacing=0.004, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=256, schedule_len=3)
+    opts, tight = bt.Options(**kw), bt.Options(max_graph_memory_mib=2, **kw)
+    alone = {}
+    for k in (1, 2, 3):
+        with bt.Gpu(opts) as gpu:
+            gpu.upload(graphs[k])
+            gpu.iterate(3)
+            alone[k] = gpu.points()[0]
+    with bt.Gpu(tight) as gpu:
+        with pytest.raises(bt.BtError) as e:
+            gpu.upload(graphs[0])
+        assert e.value.code == 3 and "sample buffer capacity" in e.value.msg
+    srv = bt.Server(opts, memory_budget=1 << 30, max_inflight=8, policy=0)
+    tickets = [srv.submit(g, tight if k == 0 else opts, 3) for k, g in enumerate(graphs)]
+    for _ in range(256):
+        r, q = srv.step()
+        if r == 0 and q == 0:
+            break
+    assert srv.poll(tickets[0]).state == 3
+    for k in (1, 2, 3):
+        res = srv.poll(tickets[k])
+        assert res.state == 2
+        p, _ = srv.points(tickets[k], res.samples)
+        np.testing.assert_array_equal(p.view(np.uint32), alone[k].view(np.uint32))
+    srv.close()
+
+
 def _edge_case_graph(bt):
     """Polylines that stress the path: two identical co-located edges, a
     closed loop (first = last point), a zero-length edge (one repeated point:
