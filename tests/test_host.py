@@ -158,6 +158,70 @@ def test_capacity_model(bt):
         bt.capacity(bt.Options(neighbor_radius=100, max_graph_memory_mib=1), 200000, 3)
 
 
+# Allocation::new (resampling.rs:321-397) term by term, for a graph of
+# 19,600 control points and 9,801 starts (9,800 paths):
+#   points 8 B each, starts 4 B each, Path 12 B, Edge 16 B and endpoint 8 B
+#   per path, Parameters 32 B, map uniform 16 B (shader.rs:4), state 48 B,
+#   statistics 16 B (resampling.rs:20-21), the neighbor grid
+#   (neighbor_mean.rs:176-183: cells 8 C + 8, summaries 24 C, 52 B of
+#   uniform and indirect dispatches) and the distortion staging 4,194,300 B
+#   (resampling.rs:16); per sample 12 + 4 B, plus 32 B with a radius
+#   (neighbor_mean.rs:15); capacity = min(available / sample_bytes,
+#   limit / 12).
+_PER_GRAPH = 8 * 19600 + 4 * 9801 + (12 + 16 + 8) * 9800 + 32 + 16 + 48 + 16  # 548,916
+_GRID_1600_R4 = (8 * 400 * 400 + 8) + 24 * 400 * 400 + 52                    # 5,120,060
+_MIB = 1 << 20
+
+
+@pytest.mark.parametrize("kw,capacity,reserved", [
+    # radius 4 on the 1600 map: grid of 400 x 400 cells, 48-byte samples
+    (dict(neighbor_radius=4), 22251517, 548916 + 5120060 + 22251517 * 48),
+    # the same with distortion staging
+    (dict(neighbor_radius=4, evaluate_distortion=True), 22164136,
+     548916 + 5120060 + 4194300 + 22164136 * 48),
+    # no radius: no grid, 16-byte samples
+    (dict(), 67074556, 548916 + 67074556 * 16),
+    # a budget large enough for the 4 GiB binding limit to bind: 2^32 - 1 / 12
+    (dict(max_graph_memory_mib=64 << 10), 357913941, 548916 + 357913941 * 16),
+])
+def test_capacity_model_known_answers(bt, kw, capacity, reserved):
+    assert _PER_GRAPH == 548916 and _GRID_1600_R4 == 5120060
+    o = bt.Options(**{"max_graph_memory_mib": 1024, **kw})
+    cap, res = bt.capacity(o, 19600, 9801)
+    assert (cap, res) == (capacity, reserved)
+    budget = o.c.max_graph_memory_bytes
+    fixed = res - cap * (48 if kw.get("neighbor_radius") else 16)
+    if cap < 357913941:
+        assert budget - res < (48 if kw.get("neighbor_radius") else 16)  # maximal
+    assert fixed == _PER_GRAPH + (_GRID_1600_R4 if kw.get("neighbor_radius") else 0) + (
+        4194300 if kw.get("evaluate_distortion") else 0)
+
+
+def test_capacity_model_limits(bt):
+    """The budget checks of Resampler::new and Allocation::new: grid plus
+    staging must be below the budget; fixed bytes must fit; capacity >= 3."""
+    # radius 1: grid of 1600^2 cells = 32 * 2,560,000 + 60 B > 64 MiB
+    with pytest.raises(bt.BtError) as e:
+        bt.capacity(bt.Options(neighbor_radius=1, max_graph_memory_mib=64), 19600, 9801)
+    assert e.value.code == 3
+    # exactly the grid + fixed bytes + 2 samples of budget: no room (capacity < 3)
+    budget = 548916 + 5120060 + 2 * 48
+    o = bt.Options(neighbor_radius=4, max_graph_memory_mib=budget / _MIB)
+    assert o.c.max_graph_memory_bytes == budget
+    with pytest.raises(bt.BtError) as e:
+        bt.capacity(o, 19600, 9801)
+    assert e.value.code == 3
+    o = bt.Options(neighbor_radius=4, max_graph_memory_mib=(budget + 48) / _MIB)
+    assert bt.capacity(o, 19600, 9801) == (3, budget + 48)
+    # fixed bytes above the budget
+    o = bt.Options(neighbor_radius=4, max_graph_memory_mib=(548916 + 5120060 - 1) / _MIB)
+    with pytest.raises(bt.BtError) as e:
+        bt.capacity(o, 19600, 9801)
+    assert e.value.code == 3
+    o = bt.Options(neighbor_radius=4, max_graph_memory_mib=(548916 + 5120060 + 3 * 48) / _MIB)
+    assert bt.capacity(o, 19600, 9801)[0] == 3
+
+
 def test_distortion_matches_oracle(bt, orc):
     rng = np.random.default_rng(4)
     n = 400

@@ -116,6 +116,73 @@ def test_resample_closed_path_special_case(orc):
     assert np.all(a[0] == a[2]) and np.any(a[1] != a[0])
 
 
+def _arc_positions(orc, pts, bbox, res, out):
+    """Arc length (texels, float64) of each sample along the polyline `pts`
+    after the reference's normalization; each sample is placed on the
+    segment nearest to it."""
+    nm = orc.normalization(bbox)
+    c = ((pts.astype(np.float64) - [nm.min_x, nm.min_y]) * nm.scale + [nm.tr_x, nm.tr_y]
+         - 0.5) * 2.0 * (res / 2.0)                     # control points in texels
+    seg = np.diff(c, axis=0)
+    lens = np.linalg.norm(seg, axis=1)
+    cum = np.concatenate([[0.0], np.cumsum(lens)])
+    q = out.astype(np.float64) * (res / 2.0)
+    arc = np.empty(len(q))
+    for i, p in enumerate(q):
+        t = np.clip(np.einsum("ij,ij->i", p - c[:-1], seg) / lens ** 2, 0.0, 1.0)
+        d = np.linalg.norm(c[:-1] + t[:, None] * seg - p, axis=1)
+        k = int(np.argmin(d))
+        assert d[k] < 1e-3, "sample off the polyline"
+        arc[i] = cum[k] + t[k] * lens[k]
+    return arc, cum
+
+
+@pytest.mark.parametrize("shape", ["L", "zigzag"])
+def test_resample_polyline_carries_remainder_across_joints(orc, shape):
+    """resampling.wgsl:91-116: `remaining` carries over each joint, so the
+    samples of a polyline lie exactly one texel of arc length apart across
+    joints, not one texel from each segment start; the first sample is the
+    first control point and the last the last one. Segment lengths are not
+    whole texels, so resetting the remainder per segment breaks the spacing
+    at every joint."""
+    if shape == "L":
+        pts = np.array([[0, 0], [0.52, 0], [0.52, 0.37]], dtype=np.float32)
+    else:
+        x = np.linspace(0.0, 1.0, 9)
+        y = np.where(np.arange(9) % 2 == 0, 0.1, 0.43) + 0.021 * np.arange(9)
+        pts = np.stack([x, y], 1).astype(np.float32)
+    st = np.array([0, len(pts)], dtype=np.uint32)
+    bbox = np.concatenate([pts.min(0), pts.max(0)]).astype(np.float32)
+    res = 117.0
+    counts, off, out, eid, status = orc.resample(pts, st, bbox, res, res)
+    assert status == 0 and counts[0] == len(out)
+    arc, cum = _arc_positions(orc, pts, bbox, res, out)
+    frac = cum[1:-1] - np.floor(cum[1:-1])
+    assert np.all((frac > 0.05) & (frac < 0.95)), "joints must fall between whole texels"
+    gaps = np.diff(arc)
+    np.testing.assert_allclose(gaps[:-1], 1.0, atol=2e-3)
+    assert 0.0 < gaps[-1] <= 1.0 + 2e-3
+    # one sample per whole texel of arc, then the endpoint
+    assert counts[0] == 1 + int(np.floor(cum[-1])) + 1
+    np.testing.assert_allclose(arc[:-1], np.arange(len(arc) - 1), atol=2e-3)
+    assert arc[-1] == pytest.approx(cum[-1], abs=1e-3)
+
+
+def test_polyline_arc_check_detects_reset_remainder(orc):
+    """The check above fails for samples placed one texel from each
+    segment start (the remainder reset per segment), built here from the
+    same L-shaped path."""
+    pts = np.array([[0, 0], [0.52, 0], [0.52, 0.37]], dtype=np.float32)
+    bbox = np.concatenate([pts.min(0), pts.max(0)]).astype(np.float32)
+    res = 117.0
+    _, _, out, _, _ = orc.resample(pts, np.array([0, 3], np.uint32), bbox, res, res)
+    arc, cum = _arc_positions(orc, pts, bbox, res, out)
+    wrong = np.concatenate([np.arange(0.0, cum[1], 1.0), cum[1] + np.arange(0.0, cum[2] - cum[1], 1.0),
+                            [cum[2]]])
+    gaps = np.diff(wrong)
+    assert not np.allclose(gaps[:-1], 1.0, atol=2e-3)
+
+
 # --------------------------------------------------------------------- grid
 
 def test_grid_bin_histogram_bruteforce(orc):
