@@ -22,6 +22,7 @@ BT_OK, BT_E_INVALID, BT_E_NO_GRAPH, BT_E_CAPACITY = 0, 1, 2, 3
 BT_E_DEVICE, BT_E_STATE, BT_E_MISMATCH, BT_E_BUSY = 4, 5, 6, 7
 FLAG_DENSITY, FLAG_DISTORTION = 1, 2
 DENSITY_COUNTS, DENSITY_ROWS, DENSITY_BLURRED, DENSITY_NORMALIZED = 0, 1, 2, 3
+DENSITY_HALF = 4  # the normalized R channel as binary16 bits (the rgba16f texture)
 PALETTE_CIVIDIS, PALETTE_VIRIDIS, PALETTE_PLASMA, PALETTE_BATLOW = 0, 1, 2, 3
 PALETTES = {"cividis": 0, "viridis": 1, "plasma": 2, "batlow": 3}
 COLOR_DENSITY, COLOR_EDGE_ID = 0, 1
@@ -522,7 +523,8 @@ class Gpu:
     def density(self, kind=DENSITY_NORMALIZED, job=0):
         p = self._opts[job].iter_params(0)
         w, h = p.density_w, p.density_h
-        out = np.zeros(w * h, dtype=np.uint32 if kind == DENSITY_COUNTS else np.float32)
+        dt = {DENSITY_COUNTS: np.uint32, DENSITY_HALF: np.uint16}.get(kind, np.float32)
+        out = np.zeros(w * h, dtype=dt)
         self._check(lib().bt_read_density_job(self._h, job, kind, out.ctypes.data_as(_P), w * h))
         return out.reshape(h, w)
 

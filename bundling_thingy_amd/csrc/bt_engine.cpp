@@ -882,6 +882,14 @@ void Engine::read_positions(uint32_t job_i, float* xy, uint32_t* ids, size_t n) 
 void Engine::read_density_of(Engine& e, Job& j, uint32_t kind, void* out, size_t n) {
   if (!j.dens_cnt.p) fail(BT_E_STATE, "density is disabled (color_by != density)");
   if (n != (size_t)j.dw * j.dh) fail(BT_E_INVALID, "pixel count mismatch");
+  if (kind == 4) {  // binary16 R channel of the normalized density (the texture's bits)
+    e.staging_.reserve(n * 2 + 4);
+    btk::half_plane(e.stream_, j.dens_norm.as<float>(), e.staging_.as<uint16_t>(), (uint32_t)n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, e.staging_.p, n * 2, hipMemcpyDeviceToHost, e.stream_));
+    e.sync();
+    return;
+  }
   const DevBuf* src = kind == 0 ? &j.dens_cnt : kind == 1 ? &j.dens_row
                     : kind == 2 ? &j.dens_col : kind == 3 ? &j.dens_norm : nullptr;
   if (!src) fail(BT_E_INVALID, "unknown density kind");

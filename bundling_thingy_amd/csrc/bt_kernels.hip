@@ -1514,6 +1514,24 @@ extern "C" __global__ __launch_bounds__(256) void bt_unpermute(const BtJob* __re
   }
 }
 
+// The normalized density as the reference's Rgba16Float texture holds its R
+// channel (density.rs:96, written by textureStore in normalization.wgsl:16):
+// binary16 bits, rounded to nearest even (v_cvt_f16_f32 under the default
+// round-to-nearest mode, f16 denormals kept). Two pixels per thread.
+extern "C" __global__ __launch_bounds__(256) void bt_half_plane(const float* __restrict__ in,
+                                                                uint16_t* __restrict__ out,
+                                                                uint32_t n) {
+  const uint32_t i = 2u * (blockIdx.x * 256u + threadIdx.x);
+  if (i + 1u < n) {
+    const float2 v = *reinterpret_cast<const float2*>(in + i);
+    const _Float16 a = (_Float16)v.x, b = (_Float16)v.y;
+    *reinterpret_cast<uint32_t*>(out + i) =
+        (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+  } else if (i < n) {
+    out[i] = __builtin_bit_cast(uint16_t, (_Float16)in[i]);
+  }
+}
+
 // --------------------------------------------------- upload / rewind set-up
 // Per-job set-up of an upload from the packed upload area (one H2D copy for
 // all jobs): each job's blur weights go to its own buffer and its status
@@ -1706,6 +1724,10 @@ void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32
 void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                float* out, uint32_t* cell_out, uint32_t want_cells, uint32_t split) {
   hipLaunchKernelGGL(bt_unpermute, L256(grid), jobs, base, nj, out, cell_out, want_cells, split);
+}
+void half_plane(hipStream_t s, const float* in, uint16_t* out, uint32_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(bt_half_plane, dim3((n + 511u) / 512u), dim3(256), 0, s, in, out, n);
 }
 void upload_setup(hipStream_t s, const BtJob* jobs, int nj, const uint32_t* area) {
   hipLaunchKernelGGL(bt_upload_setup, dim3((unsigned)nj), dim3(256), 0, s, jobs, nj, area);

@@ -27,6 +27,8 @@ void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32
 // (split = 0) or n positions (x, y) followed by n edge ids (split = 1).
 void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                float* out, uint32_t* cell_out, uint32_t want_cells, uint32_t split = 0);
+// Binary16 (round to nearest even) copy of n floats; in is 8-byte aligned.
+void half_plane(hipStream_t s, const float* in, uint16_t* out, uint32_t n);
 // Upload set-up (weights from the packed area, status cleared), one workgroup per job.
 void upload_setup(hipStream_t s, const BtJob* jobs, int nj, const uint32_t* area);
 // One 64-bit word per job: path-scan total (which = 0) or status word 0 (which = 1).

@@ -125,7 +125,9 @@ int bt_read_points(bt_ctx* ctx, void* rec12, size_t n_samples);
 
 /* Density readback (density.rs:82-166). kind: 0 = u32 counts, 1 = row-blurred
  * f32, 2 = blurred f32, 3 = normalized f32 (the R channel of the reference's
- * rgba16f texture, before f16 rounding). */
+ * rgba16f texture, before f16 rounding), 4 = that R channel as the texture
+ * holds it (density.rs:96, normalization.wgsl:16): n_pixels binary16 values
+ * (uint16_t bits, rounded to nearest even). */
 int bt_read_density(bt_ctx* ctx, uint32_t kind, void* out, size_t n_pixels);
 
 /* Grid of the last iteration: the cell of every sample (in Layout order) and

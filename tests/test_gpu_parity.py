@@ -558,3 +558,29 @@ def test_server_rejects_malformed_starts(bt):
             assert e.value.code == 1
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("case", ["reference", "gpu"])
+def test_density_binary16_readback(bt, orc, case):
+    """Density kind 4: the normalized R channel as the reference's rgba16f
+    texture holds it (density.rs:96, normalization.wgsl:16), bit-exact
+    against the oracle's binary16 rounding (or_half_round, ties to even,
+    subnormals kept) of the f32 plane, on the reference-mode map and on the
+    plain GPU case's 1024^2 map after upload and after one iteration."""
+    if case == "reference":
+        g, opts = workloads.plain_cpu_graph(), workloads.reference_cpu_options()
+    else:
+        g = workloads.plain_gpu_graph()
+        opts = bt.Options(spacing=0.002, radius=0.01, sigma=2.0, density=1024)
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(g)
+        for step in range(2):
+            f32 = gpu.density(3)
+            half = gpu.density(bt.api.DENSITY_HALF)
+            assert half.dtype == np.uint16 and half.shape == f32.shape
+            vals, inv = np.unique(f32.reshape(-1), return_inverse=True)
+            want = orc.half_round(vals)[inv].astype(np.float16).view(np.uint16)
+            np.testing.assert_array_equal(half.reshape(-1), want, err_msg=f"step {step}")
+            np.testing.assert_array_equal(half, f32.astype(np.float16).view(np.uint16))
+            assert np.any((half > 0) & (half < 0x0400)) or case == "reference"
+            gpu.iterate()
