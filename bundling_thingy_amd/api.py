@@ -107,6 +107,7 @@ SIGNATURES = {
     "bt_read_density": (C.c_int, [_P, _U32, _P, _SZ]),
     "bt_read_grid": (C.c_int, [_P, _PU32, _SZ, _PU32, _SZ]),
     "bt_read_endpoints": (C.c_int, [_P, _PU32, _SZ, _PU32]),
+    "bt_read_endpoints_noalloc": (C.c_int, [_P, _PU32, _SZ, _PU32]),
     "bt_palette": (C.c_int, [_U32, _PF, _PF]),
     "bt_render_options_default": (None, [C.POINTER(bt_render_options)]),
     "bt_render_size": (C.c_int, [_P, _PU32, _PU32]),

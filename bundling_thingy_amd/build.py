@@ -16,8 +16,8 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bt_kernels.hip", "bt_render.hip", "bt_common.cpp", "bt_engine.cpp", "bt_api.cpp",
-           "bt_image.cpp"]
+SOURCES = ["bt_kernels.hip", "bt_render.hip", "bt_noalloc.hip", "bt_common.cpp",
+           "bt_engine.cpp", "bt_api.cpp", "bt_image.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -170,6 +170,16 @@ int bt_read_grid(bt_ctx* ctx, uint32_t* cell_of, size_t n_samples, uint32_t* cou
                  [&] { ctx->engine->read_grid(0, cell_of, n_samples, counts, n_cells); });
 }
 
+int bt_read_endpoints_noalloc(bt_ctx* ctx, uint32_t* indices, size_t capacity, uint32_t* count) {
+  CTX_CHECK();
+  if (!count || (capacity && !indices)) return BT_E_INVALID;
+  *count = 0;
+  return guarded(&ctx->err, [&] {
+    ctx->engine->read_endpoints_noalloc(0, indices, capacity, count);
+    if (capacity < *count) bt::fail(BT_E_CAPACITY, "endpoint buffer too small");
+  });
+}
+
 int bt_read_endpoints(bt_ctx* ctx, uint32_t* indices, size_t capacity, uint32_t* count) {
   CTX_CHECK();
   if (!count || (capacity && !indices)) return BT_E_INVALID;

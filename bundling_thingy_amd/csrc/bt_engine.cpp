@@ -10,6 +10,7 @@
 #include <cstring>
 
 #include "bt_launch.h"
+#include "bt_noalloc.h"
 
 namespace bt {
 
@@ -137,6 +138,7 @@ Engine::Engine(const bt_options& o) : opts_(o), pool_(new DevPool) {
   HIPCHK(hipGetDeviceProperties(&prop, device_));
   cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   nm_per_cu_ = btk::neighbor_mean_blocks_per_cu();
+  noalloc_setup(device_);  // the allocation-free entry point's scratch on this device
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
   ctl_.alloc(sizeof(BtControl));
@@ -966,6 +968,13 @@ std::vector<uint32_t> Engine::read_endpoints(uint32_t job_i) {
     if (v) out.push_back(v - 1u);
   std::sort(out.begin(), out.end());
   return out;
+}
+
+void Engine::read_endpoints_noalloc(uint32_t job_i, uint32_t* indices, size_t capacity,
+                                    uint32_t* count) {
+  Job& j = job(job_i);
+  const float* rec = (j.parity ? j.rec1 : j.rec0).as<float>();
+  endpoints_noalloc(stream_, device_, rec, j.n, j.n_paths, indices, capacity, count);
 }
 
 }  // namespace bt

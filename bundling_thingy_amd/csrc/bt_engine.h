@@ -98,6 +98,8 @@ class Engine {
   void read_grid(uint32_t job, uint32_t* cell_of, size_t n, uint32_t* counts, size_t ncells);
   // Sorted sample indices of the distinct path endpoint positions.
   std::vector<uint32_t> read_endpoints(uint32_t job);
+  // The same through the per-device static scratch: no device allocation.
+  void read_endpoints_noalloc(uint32_t job, uint32_t* indices, size_t capacity, uint32_t* count);
   // The export image of a job (the blank frame without a graph), RGBA8.
   void render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, uint32_t job = 0);
   void render_size(uint32_t* w, uint32_t* h, uint32_t job = 0);

@@ -144,6 +144,22 @@ int bt_read_grid(bt_ctx* ctx, uint32_t* cell_of, size_t n_samples, uint32_t* cou
  * count) of them and BT_E_CAPACITY is returned when capacity < count. */
 int bt_read_endpoints(bt_ctx* ctx, uint32_t* indices, size_t capacity, uint32_t* count);
 
+/* bt_read_endpoints that allocates no device memory. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under hipGraph
+ * capture of its own launches, serving loops, harnesses that forbid
+ * hipMalloc). The result equals bt_read_endpoints'. Its hash table, index
+ * bitmap and index list sit in one static device array of
+ * BT_NOALLOC_SCRATCH_BYTES per GPU, set up when a context is created on that
+ * GPU and held until the process ends; graphs of more than
+ * BT_NOALLOC_MAX_PATHS paths or BT_NOALLOC_MAX_SAMPLES samples are refused
+ * with BT_E_CAPACITY. Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (from contexts used by different threads) race. */
+#define BT_NOALLOC_MAX_PATHS (1u << 18)
+#define BT_NOALLOC_MAX_SAMPLES (1u << 24)
+#define BT_NOALLOC_SCRATCH_BYTES (256u + 32u * BT_NOALLOC_MAX_PATHS + \
+                                  BT_NOALLOC_MAX_SAMPLES / 8u + 8u * BT_NOALLOC_MAX_PATHS)
+int bt_read_endpoints_noalloc(bt_ctx* ctx, uint32_t* indices, size_t capacity, uint32_t* count);
+
 /* ---------------------------------------------------------------- render */
 
 /* Palette (palette.rs:13-24; VIRIDIS is the reference default). */
