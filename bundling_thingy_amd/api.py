@@ -20,7 +20,7 @@ _LIB_PATH = os.path.join(_HERE, "libbundling.so")
 
 BT_OK, BT_E_INVALID, BT_E_NO_GRAPH, BT_E_CAPACITY = 0, 1, 2, 3
 BT_E_DEVICE, BT_E_STATE, BT_E_MISMATCH, BT_E_BUSY = 4, 5, 6, 7
-FLAG_DENSITY, FLAG_DISTORTION = 1, 2
+FLAG_DENSITY, FLAG_DISTORTION, FLAG_DENSITY_ON_READ = 1, 2, 4
 DENSITY_COUNTS, DENSITY_ROWS, DENSITY_BLURRED, DENSITY_NORMALIZED = 0, 1, 2, 3
 DENSITY_HALF = 4  # the normalized R channel as binary16 bits (the rgba16f texture)
 PALETTE_CIVIDIS, PALETTE_VIRIDIS, PALETTE_PLASMA, PALETTE_BATLOW = 0, 1, 2, 3
@@ -82,6 +82,13 @@ class bt_server_options(C.Structure):
     _fields_ = [("defaults", bt_options), ("memory_budget", C.c_uint64),
                 ("max_inflight", C.c_uint32), ("policy", C.c_uint32),
                 ("tick_samples", C.c_uint64)]
+
+
+class bt_server_info(C.Structure):
+    _fields_ = [("queued", C.c_uint32), ("running", C.c_uint32), ("finishing", C.c_uint32),
+                ("reserved", C.c_uint32), ("footprint_bytes", C.c_uint64),
+                ("held_bytes", C.c_uint64), ("held_peak_bytes", C.c_uint64),
+                ("memory_budget", C.c_uint64)]
 
 
 class bt_job_result(C.Structure):
@@ -147,6 +154,7 @@ SIGNATURES = {
     "bt_server_read_density": (C.c_int, [_P, _U64, _U32, _P, _SZ]),
     "bt_server_release": (C.c_int, [_P, _U64]),
     "bt_server_clock_ms": (_D, [_P]),
+    "bt_server_get_info": (C.c_int, [_P, C.POINTER(bt_server_info)]),
 }
 
 _lib = None
@@ -191,7 +199,7 @@ class Options:
     def __init__(self, map_size=1600, neighbor_radius=None, max_graph_memory_mib=1024,
                  color_by_density=True, evaluate_distortion=False, device=0, spacing=0.0,
                  radius=None, sigma=None, schedule_len=1, density=None, cell_split=0,
-                 map_h=None, x_split=0):
+                 map_h=None, x_split=0, density_on_read=False):
         o = bt_options()
         lib().bt_options_default(C.byref(o))
         o.map_w = map_size
@@ -199,7 +207,8 @@ class Options:
         o.radius_texels = neighbor_radius or 0
         o.max_graph_memory_bytes = int(max_graph_memory_mib * 1024 * 1024)
         o.flags = (FLAG_DENSITY if color_by_density else 0) | (
-            FLAG_DISTORTION if evaluate_distortion else 0)
+            FLAG_DISTORTION if evaluate_distortion else 0) | (
+            FLAG_DENSITY_ON_READ if density_on_read else 0)
         o.device = device
         o.spacing = spacing
         if radius is not None:
@@ -686,3 +695,10 @@ class Server:
 
     def clock_ms(self):
         return lib().bt_server_clock_ms(self._h)
+
+    def info(self):
+        """Admission accounting: queued/running/finishing jobs, the footprint
+        charged against the budget, the bytes the admitted jobs hold."""
+        i = bt_server_info()
+        self._check(lib().bt_server_get_info(self._h, C.byref(i)))
+        return i

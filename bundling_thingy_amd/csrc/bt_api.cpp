@@ -379,6 +379,7 @@ struct ServedJob {
   bt_options opts{};
   uint32_t iterations = 0;
   uint64_t estimate = 0;  // device bytes expected before admission
+  uint64_t held = 0;      // device bytes its buffers hold while admitted
   uint64_t est_samples = 0;  // samples expected (the resample length model)
   double last_ms = 0;     // server clock of admission or of the last iteration
   // retirement in flight: results copied into a pinned buffer, `done` marks
@@ -405,7 +406,9 @@ struct bt_server {
   std::unique_ptr<bt::Engine> engine;
   std::chrono::steady_clock::time_point t0;
   uint64_t next_ticket = 1;
-  uint64_t used = 0;
+  uint64_t used = 0;   // admitted jobs' footprints (estimate, or held if larger)
+  uint64_t held = 0;   // device bytes the admitted jobs' buffers hold
+  uint64_t held_peak = 0;
   std::deque<ServedJob*> queue;
   std::vector<ServedJob*> running;
   std::vector<ServedJob*> finishing;          // results being copied out
@@ -447,37 +450,33 @@ uint64_t estimate_bytes(const ServedJob& s, uint64_t* samples_out) {
     }
     samples += 2.0;
   }
+  // a path of length D texels yields at most floor(D) + 2 samples
   const uint64_t n = (uint64_t)(samples * 1.02) + 64;
   *samples_out = n;
-  uint64_t cells = 1;
-  for (uint32_t i = 0; i < bt::schedule_len(o); ++i) {
-    bt_iter_params q;
-    bt::iter_params(o, i, 1, &q);
-    const uint32_t k = o.cell_split ? o.cell_split
-                                    : (q.radius > 0.0f ? bt::density_split(q.radius, n, q.res_x,
-                                                                           q.res_y)
-                                                       : 1u);
-    bt::iter_params(o, i, k, &q);
-    cells = std::max<uint64_t>(cells, (uint64_t)q.gx * q.gy);
-  }
-  uint64_t bytes = n * (12 * 3 + 16 + 4 + 8 + 4) + (cells + 1) * 4 + s.xy.size() * 4 +
-                   s.starts.size() * 24 + 4096;
-  if (o.flags & BT_FLAG_DENSITY) bytes += (uint64_t)p.density_w * p.density_h * 32;
-  return bytes;
+  // the buffers upload_jobs allocates for n samples, as pool blocks
+  return bt::Job::footprint(o, n, np, (uint32_t)s.starts.size(), false);
 }
 
 // Finishes retirements whose copies are done (all of them if `block`):
 // results move to the job's host vectors, device state and pinned buffers
 // are released.
 void reap(bt_server* srv, bool block) {
-  std::vector<ServedJob*> still;
-  for (ServedJob* s : srv->finishing) {
+  // Each job leaves `finishing` before anything can throw: on an error the
+  // list holds exactly the jobs not handled yet, so a later release of a
+  // handled job cannot leave a dangling entry behind.
+  std::vector<ServedJob*> todo;
+  todo.swap(srv->finishing);
+  for (size_t k = 0; k < todo.size(); ++k) {
+    ServedJob* s = todo[k];
     const hipError_t q = block ? hipEventSynchronize(s->done) : hipEventQuery(s->done);
     if (q == hipErrorNotReady) {
-      still.push_back(s);
+      srv->finishing.push_back(s);
       continue;
     }
-    if (q != hipSuccess) bt::fail(BT_E_DEVICE, std::string("HIP: ") + hipGetErrorString(q));
+    if (q != hipSuccess) {
+      srv->finishing.insert(srv->finishing.end(), todo.begin() + k, todo.end());
+      bt::fail(BT_E_DEVICE, std::string("HIP: ") + hipGetErrorString(q));
+    }
     const size_t pb = (size_t)s->samples * 12;
     s->points.assign((const unsigned char*)s->pinned, (const unsigned char*)s->pinned + pb);
     if (!s->density.empty())
@@ -488,10 +487,11 @@ void reap(bt_server* srv, bool block) {
     s->done = nullptr;
     s->job.reset();
     srv->used -= std::min(srv->used, s->estimate);
+    srv->held -= std::min(srv->held, s->held);
+    s->held = 0;
     s->state = 2;
     s->done_ms = srv->now();
   }
-  srv->finishing.swap(still);
 }
 
 // A pinned host buffer of at least `bytes` (reused when one is free).
@@ -607,11 +607,20 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
       const uint32_t done = j->job ? j->job->iteration : 0u;
       return n * (uint64_t)(j->iterations - std::min(done, j->iterations));
     };
-    if (shortest)  // least work first; arrival order among equals
+    if (shortest) {
+      // least work first, arrival order among equals; a job queued for more
+      // than kStarveMs goes first (in arrival order), so continuous small
+      // arrivals cannot hold a large job back indefinitely
+      const double now = srv->now();
       std::stable_sort(srv->queue.begin(), srv->queue.end(),
                        [&](const ServedJob* a, const ServedJob* b) {
+                         const bool sa = now - a->submit_ms > kStarveMs;
+                         const bool sb = now - b->submit_ms > kStarveMs;
+                         if (sa != sb) return sa;
+                         if (sa) return a->ticket < b->ticket;
                          return remaining(a) < remaining(b);
                        });
+    }
     // Admission while the footprint fits the budget and the in-flight cap.
     std::vector<ServedJob*> admit;
     uint64_t used = srv->used;
@@ -652,14 +661,20 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
         if (!jobs[i]) continue;
         ServedJob* s = admit[i];
         s->job = std::move(jobs[i]);
-        // the density the server returns is that of the last iteration; the
-        // earlier ones are never read, so they are not computed
-        s->job->density_from = s->iterations - 1;
+        // The density the server returns is that of the last iteration. With
+        // BT_FLAG_DENSITY_ON_READ the earlier ones, never read, are not
+        // computed; without it every iteration computes its density, as
+        // Gpu::iterate does (both give the last iteration's bits).
+        s->job->density_from =
+            (s->opts.flags & BT_FLAG_DENSITY_ON_READ) ? s->iterations - 1 : 0u;
         s->state = 1;
         s->admit_ms = t;
         s->last_ms = t;
         s->samples = s->job->n;
-        s->estimate = std::max<uint64_t>(s->estimate, s->job->device_bytes());
+        s->held = s->job->device_bytes();
+        srv->held += s->held;
+        srv->held_peak = std::max(srv->held_peak, srv->held);
+        s->estimate = std::max<uint64_t>(s->estimate, s->held);
         srv->used += s->estimate;
         srv->running.push_back(s);
       }
@@ -719,6 +734,19 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
     if (running) *running = (uint32_t)(srv->running.size() + srv->finishing.size());
     if (queued) *queued = (uint32_t)srv->queue.size();
   });
+}
+
+int bt_server_get_info(const bt_server* srv, bt_server_info* out) {
+  if (!srv || !out) return BT_E_INVALID;
+  std::memset(out, 0, sizeof *out);
+  out->queued = (uint32_t)srv->queue.size();
+  out->running = (uint32_t)srv->running.size();
+  out->finishing = (uint32_t)srv->finishing.size();
+  out->footprint_bytes = srv->used;
+  out->held_bytes = srv->held;
+  out->held_peak_bytes = srv->held_peak;
+  out->memory_budget = srv->o.memory_budget;
+  return BT_OK;
 }
 
 int bt_server_poll(bt_server* srv, uint64_t ticket, bt_job_result* out) {

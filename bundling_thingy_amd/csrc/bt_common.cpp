@@ -75,14 +75,18 @@ void* DevPool::device_alloc(size_t bytes) {
   return q;
 }
 
-void* DevPool::get(size_t bytes, size_t* cap) {
-  // sizes round up to a power of two below 1 MiB, to 1 MiB multiples above
+size_t DevPool::block_size(size_t bytes) {
   size_t c = 256;
   if (bytes <= (1u << 20)) {
     while (c < bytes) c <<= 1;
   } else {
     c = (bytes + (1u << 20) - 1) >> 20 << 20;
   }
+  return c;
+}
+
+void* DevPool::get(size_t bytes, size_t* cap) {
+  const size_t c = block_size(bytes);
   auto it = free_.lower_bound(c);
   if (it != free_.end() && it->first <= c + c / 4) {
     void* q = it->second;

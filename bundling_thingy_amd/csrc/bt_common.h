@@ -37,6 +37,9 @@ class DevPool {
   DevPool(const DevPool&) = delete;
   DevPool& operator=(const DevPool&) = delete;
   ~DevPool();
+  // Bytes a request of `bytes` holds: a power of two (>= 256) up to 1 MiB,
+  // whole MiB above.
+  static size_t block_size(size_t bytes);
   void* get(size_t bytes, size_t* cap);
   void put(void* p, size_t cap);
   size_t cached() const { return cached_; }

@@ -33,7 +33,7 @@ static const size_t kSplitCandOff = kSplitAccOff + (size_t)BT_MAX_HEAVY * 192 * 
 static const size_t kSplitRecOff = kSplitCandOff + (size_t)BT_MAX_WG * sizeof(unsigned long long);
 static const size_t kSplitBytes = kSplitRecOff + (size_t)BT_MAX_HEAVY * BT_REC_WORDS * 4;
 
-BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
+BtJob Job::dev(uint32_t it, uint32_t parity_at, bool force_density) const {
   BtJob d;
   std::memset(&d, 0, sizeof d);
   d.rec[0] = rec0.as<float>();
@@ -47,7 +47,7 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at) const {
   d.pre_y = pre_y.as<uint64_t>();
   d.edge_of = edge_of.as<uint32_t>();
   // density stages run for this iteration (kernels test dens_cnt, dw x dh, peak)
-  const bool density = dens_cnt.p != nullptr && it >= density_from;
+  const bool density = dens_cnt.p != nullptr && (force_density || it >= density_from);
   d.dens_cnt = density ? dens_cnt.as<uint32_t>() : nullptr;
   d.dens_row = dens_row.as<float>();
   d.dens_col = dens_col.as<float>();
@@ -127,7 +127,46 @@ size_t Job::device_bytes() const {
                          &pre_x, &pre_y,
                          &dens_cnt, &dens_row, &dens_col, &dens_norm, &dens_saved, &peak, &wt};
   size_t s = 0;
-  for (const DevBuf* b : all) s += b->bytes;
+  for (const DevBuf* b : all) s += b->cap;
+  return s;
+}
+
+uint32_t Job::max_cells_for(const bt_options& o, uint64_t n) {
+  uint32_t cells = 1;
+  for (uint32_t i = 0; i < schedule_len(o); ++i) {
+    bt_iter_params q;
+    iter_params(o, i, o.cell_split, &q);
+    cells = std::max(cells, q.gx * q.gy);
+    if (o.cell_split == 0 && q.radius > 0.0f) {
+      iter_params(o, i, density_split(q.radius, n, q.res_x, q.res_y), &q);
+      cells = std::max(cells, q.gx * q.gy);
+    }
+  }
+  return cells;
+}
+
+size_t Job::footprint(const bt_options& o, uint64_t n, uint32_t n_points, uint32_t n_starts,
+                      bool rewindable) {
+  const uint32_t sched = schedule_len(o);
+  const uint64_t paths = n_starts ? n_starts - 1ull : 0ull;
+  const uint64_t cells = max_cells_for(o, n);
+  bt_iter_params p0;
+  iter_params(o, 0, 1, &p0);
+  auto blk = [](uint64_t b) { return b ? DevPool::block_size((size_t)b) : (size_t)0; };
+  const uint64_t nb = std::max<uint64_t>(
+      std::max<uint64_t>(cdiv(paths + 1, BT_SCAN_ITEMS), cdiv(cells + 1, BT_SCAN_ITEMS)),
+      2ull * cdiv(n + 1, BT_SCAN_ITEMS));
+  size_t s = blk(8ull * n_starts) + blk(12ull * std::max<uint64_t>(paths, 1)) + blk(16) +
+             blk(8 * nb) + blk(4ull * sched * kWt);
+  s += 3 * blk(12 * n) + blk(16 * n) + blk(4 * n) + blk(8 * n) + blk(4 * n) +
+       blk(4 * (cells + 1)) + 2 * blk(8 * (n + 1));
+  if (o.flags & BT_FLAG_DENSITY) {
+    const uint64_t pb = 4ull * p0.density_w * p0.density_h;
+    s += 4 * blk(pb) + blk(8) + (rewindable ? blk(4 * pb) : 0);
+  }
+  // upload area: control points, starts and weights, 64-word aligned
+  s += blk(4 * (align_up(2ull * n_points, 64) + align_up(n_starts, 64) +
+                align_up((size_t)sched * kWt, 64) + 64));
   return s;
 }
 
@@ -229,6 +268,15 @@ static uint32_t kind_blocks(int k, const Job& j, const BtJob& d) {
 }
 
 Plan Engine::make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>& iters) const {
+  std::vector<BtJob> entries(js.size());
+  for (size_t i = 0; i < js.size(); ++i) {
+    const uint32_t it = iters[i];
+    entries[i] = js[i]->dev(it, (js[i]->parity + (it - js[i]->iteration)) & 1u);
+  }
+  return make_plan_of(js, entries);
+}
+
+Plan Engine::make_plan_of(const std::vector<Job*>& js, const std::vector<BtJob>& entries) const {
   Plan p;
   const uint32_t nj = (uint32_t)js.size();
   p.njobs = nj;
@@ -236,10 +284,7 @@ Plan Engine::make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>&
   p.host.assign(p.base_off + (size_t)BT_K_COUNT * (nj + 1) * 4, 0);
   BtJob* J = reinterpret_cast<BtJob*>(p.host.data());
   uint32_t* B = reinterpret_cast<uint32_t*>(p.host.data() + p.base_off);
-  for (uint32_t i = 0; i < nj; ++i) {
-    const uint32_t it = iters[i];
-    J[i] = js[i]->dev(it, (js[i]->parity + (it - js[i]->iteration)) & 1u);
-  }
+  for (uint32_t i = 0; i < nj; ++i) J[i] = entries[i];
   for (int k = 0; k < BT_K_COUNT; ++k) {
     uint64_t acc = 0;
     for (uint32_t i = 0; i < nj; ++i) {
@@ -575,6 +620,8 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
   for (auto& j : nj) {
     j->iteration = 0;
     j->parity = 0;
+    j->density_at = 0;
+    if (j->o.flags & BT_FLAG_DENSITY_ON_READ) j->density_from = 0xffffffffu;
     out->push_back(std::move(j));
   }
 }
@@ -582,7 +629,11 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
 void Engine::rewind() {
   if (jobs_.empty()) fail(BT_E_NO_GRAPH, "no uploaded graph");
   std::vector<Job*> js = all_jobs();
-  Plan p = make_plan(js, std::vector<uint32_t>(js.size(), 0u));
+  // entries with the density planes whatever the density policy: the planes
+  // saved at upload come back too
+  std::vector<BtJob> entries;
+  for (Job* j : js) entries.push_back(j->dev(0, 0, true));
+  Plan p = make_plan_of(js, entries);
   DevBuf table;
   table.pool = pool_.get();
   table.alloc(p.host.size());
@@ -599,6 +650,7 @@ void Engine::rewind() {
   for (Job* j : js) {
     j->iteration = 0;
     j->parity = 0;
+    j->density_at = 0;
   }
   reset_split_estimate();
   sync();
@@ -635,10 +687,7 @@ void Engine::iterate_jobs(const std::vector<Job*>& js, bool wait) {
   unsigned char* dev = plans_.as<unsigned char>() + eager_slot_ * slot;
   put_plan(p, dev);
   launch_iteration(p, dev, density);
-  for (Job* j : js) {
-    ++j->iteration;
-    j->parity ^= 1u;
-  }
+  for (Job* j : js) j->advanced(1);
   if (wait) sync();
 }
 
@@ -663,10 +712,7 @@ void Engine::iterate(uint32_t n, bool use_graph) {
       unsigned char* dev = plans_.as<unsigned char>() + eager_slot_ * slot;
       put_plan(p, dev);
       launch_iteration(p, dev, density);
-      for (Job* j : js) {
-        ++j->iteration;
-        j->parity ^= 1u;
-      }
+      for (Job* j : js) j->advanced(1);
     }
     sync();
     return;
@@ -698,10 +744,7 @@ void Engine::iterate(uint32_t n, bool use_graph) {
     graph_bufs_.emplace(key, std::move(buf));
   }
   HIPCHK(hipGraphLaunch(found->second.exec, stream_));
-  for (Job* j : js) {
-    j->iteration += n;
-    j->parity ^= (n & 1u);
-  }
+  for (Job* j : js) j->advanced(n);
   sync();
 }
 
@@ -738,10 +781,7 @@ void Engine::profile(uint32_t n, double* stage_ms, uint32_t nst, double* total_m
       float ms = 0.0f;
       HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[9]));
       tot += ms;
-      for (Job* j : js) {
-        ++j->iteration;
-        j->parity ^= 1u;
-      }
+      for (Job* j : js) j->advanced(1);
     }
   } catch (...) {
     for (auto& e : ev) (void)hipEventDestroy(e);
@@ -820,10 +860,7 @@ void Engine::autotune(uint32_t n, const uint32_t* cand, uint32_t ncand, uint32_t
       restore();
       for (Job* j : js) j->set_split(j->entry(i), bk);
       (void)run_one(false);
-      for (Job* j : js) {
-        ++j->iteration;
-        j->parity ^= 1u;
-      }
+      for (Job* j : js) j->advanced(1);
       if (chosen) chosen[i] = bk;
       if (best_ms) best_ms[i] = best;
     }
@@ -881,9 +918,34 @@ void Engine::read_positions(uint32_t job_i, float* xy, uint32_t* ids, size_t n) 
   sync();
 }
 
+void Engine::ensure_density(Job& j) {
+  if (!j.dens_cnt.p || j.density_at == j.iteration) return;
+  // the density of the current records, with the parameters of the
+  // iteration that produced them (iteration k ran with entry k - 1)
+  const uint32_t e = j.iteration ? j.iteration - 1u : 0u;
+  std::vector<Job*> js = {&j};
+  Plan p = make_plan_of(js, {j.dev(e, j.parity, true)});
+  DevBuf table;
+  table.pool = pool_.get();
+  table.alloc(p.host.size());
+  put_plan(p, table.p);
+  const unsigned char* dev = table.as<unsigned char>();
+  const BtJob* J = reinterpret_cast<const BtJob*>(dev);
+  auto B = [&](int k) { return reinterpret_cast<const uint32_t*>(dev + p.base_off) + k * 2; };
+  btk::density_clear(stream_, p.grid[BT_K_NORM], J, B(BT_K_NORM), 1);
+  btk::splat_points(stream_, p.grid[BT_K_BIN], J, B(BT_K_BIN), 1);
+  btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), 1);
+  btk::blur_cols(stream_, p.grid[BT_K_COL], J, B(BT_K_COL), 1);
+  btk::normalize(stream_, p.grid[BT_K_NORM], J, B(BT_K_NORM), 1);
+  HIPCHK(hipGetLastError());
+  sync();
+  j.density_at = j.iteration;
+}
+
 void Engine::read_density_of(Engine& e, Job& j, uint32_t kind, void* out, size_t n) {
   if (!j.dens_cnt.p) fail(BT_E_STATE, "density is disabled (color_by != density)");
   if (n != (size_t)j.dw * j.dh) fail(BT_E_INVALID, "pixel count mismatch");
+  e.ensure_density(j);
   if (kind == 4) {  // binary16 R channel of the normalized density (the texture's bits)
     e.staging_.reserve(n * 2 + 4);
     btk::half_plane(e.stream_, j.dens_norm.as<float>(), e.staging_.as<uint16_t>(), (uint32_t)n);

@@ -24,6 +24,8 @@ struct Job {
   // one, as Gpu::iterate does; the served form computes only the density it
   // returns, that of a job's last iteration)
   uint32_t density_from = 0;
+  // iteration whose density the planes hold (density computed on read)
+  uint32_t density_at = 0;
   uint32_t sched = 1;           // distinct parameter sets
   std::vector<bt_iter_params> params;
   std::vector<uint32_t> splits; // grid cells per radius, per schedule entry
@@ -35,11 +37,30 @@ struct Job {
   DevBuf dens_cnt, dens_row, dens_col, dens_norm, dens_saved, peak, wt;
 
   const bt_iter_params& at(uint32_t it) const { return params[it < sched ? it : sched - 1]; }
+  // After n iterations ran from `iteration`: the count, the parity of the
+  // current records, and which iteration's density the planes hold.
+  void advanced(uint32_t n) {
+    if (n == 0) return;
+    if (dens_cnt.p && iteration + n - 1u >= density_from) density_at = iteration + n;
+    iteration += n;
+    parity ^= (n & 1u);
+  }
   uint32_t entry(uint32_t it) const { return it < sched ? it : sched - 1; }
   // Sets schedule entry e's cell split; returns true if the cell buffers grew.
   bool set_split(uint32_t e, uint32_t k);
-  BtJob dev(uint32_t it, uint32_t parity_at) const;
+  // Job-table entry for iteration `it` reading rec[parity_at]; `force_density`:
+  // with the density stages whatever density_from says.
+  BtJob dev(uint32_t it, uint32_t parity_at, bool force_density = false) const;
+  // Device bytes the job's buffers hold (pool blocks, rounded up).
   size_t device_bytes() const;
+  // Grid bins the largest schedule entry needs with n samples, as an upload
+  // sizes them (the split given in the options, else the default and then
+  // the sample-density split).
+  static uint32_t max_cells_for(const bt_options& o, uint64_t n);
+  // Device bytes (pool blocks) of a job of n samples as upload_jobs
+  // allocates it, plus its share of the upload area (held while resampling).
+  static size_t footprint(const bt_options& o, uint64_t n, uint32_t n_points, uint32_t n_starts,
+                          bool rewindable);
   // Allocate this job's buffers from `pool` (before any allocation).
   void set_pool(DevPool* pool);
 };
@@ -107,6 +128,9 @@ class Engine {
   // Unpermutes a job's current samples into staging_ (records, or split).
   static void unpermute_to_staging(Engine& e, Job& j, bool split);
   static void read_density_of(Engine& e, Job& j, uint32_t kind, void* out, size_t n);
+  // Computes the density of a job's current positions if its planes hold an
+  // older iteration's (BT_FLAG_DENSITY_ON_READ); blocks until done.
+  void ensure_density(Job& j);
 
   // Heavy tasks split by the last iteration run, and their parts.
   void split_counts(uint32_t* tasks, uint32_t* parts);
@@ -121,6 +145,8 @@ class Engine {
 
  private:
   Plan make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>& iters) const;
+  // A plan over explicit job-table entries (block bases from the entries).
+  Plan make_plan_of(const std::vector<Job*>& js, const std::vector<BtJob>& entries) const;
   // Copies a plan to device memory through a ring of pinned host slots (the
   // copy is asynchronous; a slot is reused once its previous copy is done).
   void put_plan(const Plan& p, void* dst, hipStream_t st = nullptr);

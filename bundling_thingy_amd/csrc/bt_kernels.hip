@@ -1263,6 +1263,45 @@ extern "C" __global__ __launch_bounds__(256) void bt_normalize(const BtJob* __re
   }
 }
 
+// ------------------------------------------------- density on read
+// The density of the current records outside an iteration
+// (BT_FLAG_DENSITY_ON_READ): counts and peak cleared, every sample splatted
+// as the neighbor-mean kernel splats its output (neighbor_mean.wgsl:66-73),
+// then the usual blur and normalize. Integer adds commute, so the counts
+// equal those an iteration writes.
+extern "C" __global__ __launch_bounds__(256) void bt_density_clear(const BtJob* __restrict__ jobs,
+                                                                   const uint32_t* __restrict__ base,
+                                                                   int njobs) {
+  JOB_PROLOGUE(BT_K_NORM);
+  const uint32_t npix = J.dw * J.dh;
+  const uint32_t i0 = lb * BT_NORM_ITEMS + threadIdx.x;
+#pragma unroll
+  for (uint32_t k = 0; k < BT_NORM_ITEMS / 256u; ++k) {
+    const uint32_t i = i0 + k * 256u;
+    if (i < npix) J.dens_cnt[i] = 0u;
+  }
+  if (lb == 0 && threadIdx.x == 0 && J.peak) J.peak[0] = 0u;
+}
+
+extern "C" __global__ __launch_bounds__(256) void bt_splat_points(const BtJob* __restrict__ jobs,
+                                                                  const uint32_t* __restrict__ base,
+                                                                  int njobs) {
+  JOB_PROLOGUE(BT_K_BIN);
+  const uint32_t i = lb * BT_BIN_BLOCK + threadIdx.x;
+  const bool valid = i < J.n;
+  const float* rec = J.rec[J.parity];
+  const uint32_t pix = valid ? splat_index(rec[3u * i], rec[3u * i + 1u], J.dw, J.dh) : 0u;
+  // records are in cell order: lanes of a wave mostly share pixels
+  uint64_t act = ballot(valid);
+  while (act) {
+    const int leader = __builtin_ctzll(act);
+    const uint32_t key = __builtin_amdgcn_readlane(pix, leader);
+    const uint64_t same = ballot(valid && pix == key);
+    if (LANE() == (uint32_t)leader) atomicAdd(&J.dens_cnt[key], (uint32_t)__popcll(same));
+    act &= ~same;
+  }
+}
+
 // ------------------------------------------------------------- resampling
 // resampling.wgsl:45-48
 __device__ __forceinline__ void normalized(const BtJob& J, uint32_t i, float& x, float& y) {
@@ -1706,6 +1745,12 @@ void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
 }
 void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_blur_cols, L256(grid), jobs, base, nj);
+}
+void density_clear(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  if (grid) hipLaunchKernelGGL(bt_density_clear, L256(grid), jobs, base, nj);
+}
+void splat_points(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
+  if (grid) hipLaunchKernelGGL(bt_splat_points, L256(grid), jobs, base, nj);
 }
 void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_normalize, L256(grid), jobs, base, nj);

@@ -20,6 +20,9 @@ uint32_t neighbor_mean_blocks_per_cu();
 void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+// Density of the current records outside an iteration: clear, then splat.
+void density_clear(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+void splat_points(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void resample_count(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void path_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);

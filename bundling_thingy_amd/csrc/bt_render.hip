@@ -544,6 +544,7 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
     Job& j = job(job_i);
     if (ro.color_by == BT_COLOR_DENSITY && !j.dens_norm.p)
       fail(BT_E_STATE, "density is disabled (color_by != density)");
+    if (ro.color_by == BT_COLOR_DENSITY) ensure_density(j);
     f.n = j.n;
     f.n_edges = j.n_paths;
     f.dens = j.dens_norm.as<float>();

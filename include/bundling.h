@@ -34,6 +34,14 @@ extern "C" {
 /* bt_options.flags */
 #define BT_FLAG_DENSITY 1u    /* ColorBy::Density (palette.rs:3-10, the reference default) */
 #define BT_FLAG_DISTORTION 2u /* Options::evaluate_distortion: staging bytes count in the budget */
+/* With BT_FLAG_DENSITY: iterations skip the splat, blur and normalize; the
+ * density of the current positions is computed when it is read
+ * (bt_read_density*, density colouring in bt_render*), with the bits an
+ * iteration would have produced. On the server (bt_server_submit options) a
+ * job computes only its last iteration's density, the one it returns;
+ * without the flag it computes every iteration's, as Gpu::iterate does.
+ * No counterpart in the reference. */
+#define BT_FLAG_DENSITY_ON_READ 4u
 
 /* Options of one context / job. Replaces the reference's `Options`
  * (main.rs:25-90) plus the map size (gpu.rs:21,99-102).
@@ -345,10 +353,11 @@ int bt_server_submit(bt_server* srv, const float* xy, uint32_t n_points, const u
 /* One scheduler tick: admits queued jobs that fit the memory budget and the
  * in-flight cap (and resamples them in one batched launch), advances running
  * jobs by one iteration in one batched launch per stage, retires finished
- * jobs. FIFO advances every running job; SHORTEST advances the jobs with the
- * least remaining work (samples x iterations left) until tick_samples
- * sample-updates are taken (at least one job), except that a job left
- * waiting for more than a second goes first. */
+ * jobs. FIFO admits in arrival order and advances every running job;
+ * SHORTEST admits and advances the jobs with the least remaining work
+ * (samples x iterations left) first, until tick_samples sample-updates are
+ * taken (at least one job), except that a job queued, or running but not
+ * advanced, for more than a second goes first. */
 int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued);
 
 int bt_server_poll(bt_server* srv, uint64_t ticket, bt_job_result* out);
@@ -361,6 +370,18 @@ int bt_server_read_density(bt_server* srv, uint64_t ticket, uint32_t kind, void*
 /* Frees a finished job's device state. */
 int bt_server_release(bt_server* srv, uint64_t ticket);
 double bt_server_clock_ms(const bt_server* srv);
+
+/* Admission accounting (no counterpart in the reference): footprint_bytes
+ * is what admission charges against memory_budget (each admitted job's
+ * estimate, or the bytes its buffers hold if larger); held_bytes is what the
+ * admitted jobs' device buffers hold now (allocator blocks), held_peak_bytes
+ * its maximum so far. Admission keeps held_bytes <= footprint_bytes <=
+ * memory_budget. */
+typedef struct bt_server_info {
+  uint32_t queued, running, finishing, reserved;
+  uint64_t footprint_bytes, held_bytes, held_peak_bytes, memory_budget;
+} bt_server_info;
+int bt_server_get_info(const bt_server* srv, bt_server_info* out);
 
 #ifdef __cplusplus
 }

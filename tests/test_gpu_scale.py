@@ -168,7 +168,7 @@ def served_case(bt):
     return jobs, alone
 
 
-def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=()):
+def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=(), flags=0):
     """Submits `per_step` arrivals before each tick (about one tick's worth of
     the 200 jobs/s stream at full load), so admissions keep arriving while
     earlier jobs run. Returns {job index: (positions hash, density hash)} and
@@ -181,6 +181,11 @@ def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=()):
             for _ in range(per_step):
                 if i < len(jobs):
                     _, g, o, it = jobs[i]
+                    if flags:
+                        o2 = bt.Options()
+                        o2.c = type(o.c).from_buffer_copy(o.c)
+                        o2.c.flags |= flags
+                        o = o2
                     pending[srv.submit(g, o, it)] = i
                     i += 1
             running, queued = srv.step()
@@ -205,13 +210,16 @@ def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=()):
         srv.close()
 
 
-@pytest.mark.parametrize("policy,tick", [(0, 0), (1, 1 << 20)])
-def test_served_stream_equals_alone(bt, served_case, policy, tick):
+@pytest.mark.parametrize("policy,tick,flags", [(0, 0, 0), (1, 1 << 20, 0), (0, 0, 4),
+                                               (1, 1 << 20, 4)])
+def test_served_stream_equals_alone(bt, served_case, policy, tick, flags):
     """Every job of the 100-job served stream, co-scheduled with the others
     under continuous batching (FIFO, and shortest-first with 1M
-    sample-updates per tick), ends with the bits of the job alone."""
+    sample-updates per tick), ends with the bits of the job alone, whether
+    the jobs compute their density every iteration or only at the last one
+    (flags 4 = BT_FLAG_DENSITY_ON_READ)."""
     jobs, alone = served_case
-    out, _ = _serve_compressed(bt, jobs, policy, tick)
+    out, _ = _serve_compressed(bt, jobs, policy, tick, flags=flags)
     for k in range(len(jobs)):
         assert out[k] == alone[k], f"job {k}"
 
