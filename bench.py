@@ -8,8 +8,9 @@ schedule (the scope of 30 Gpu::iterate calls) replayed as one hipGraph from
 the post-upload state; upload/resampling is timed separately.
 
 Prints one JSON line (see the keys below). Extra objects: roofline (dominant
-kernel vs HBM peak, from HIP events), valu (the same kernel's FP32 work vs the
-vector peak), cpu_baseline (the CPU oracle on a bounded sample), latency.
+kernel vs HBM peak, from HIP events), executed (the same kernel's executed
+test-loop instructions vs the SIMDs' issue capacity), cpu_baseline (the CPU
+oracle on a bounded sample of the same schedule), latency.
 """
 import argparse
 import json
@@ -23,7 +24,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+# VALU issue capacity: 256 CUs x 4 SIMDs at 2.4 GHz, 16 lanes per SIMD, so a
+# wave64 instruction every 4 cycles per SIMD (the published 157.3 TFLOP/s FP32
+# = 256 x 4 x 16 lanes x 2 (packed FP32) x 2 (FMA) x 2.4e9)
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+# VALU instructions per wave-step of the neighbor test loop (the unrolled loop
+# of bt_neighbor_mean in the gfx950 assembly: 47 per 4 candidates -- 2 sub,
+# 3 mul, 1 fma, 1 compare per test, 3 masked adds folded into add3/addc, one
+# pointer increment per 4)
+VALU_PER_STEP = 47 / 4
 ITERS = 30
 TUNE_CANDIDATES = (1, 2, 3, 4, 6, 8, 12, 16, 20, 24, 28)
 COPY_PROBE_BYTES = 256 << 20
@@ -33,68 +42,73 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def candidate_flops(counts2d):
-    """8 FP32 ops per candidate pair; candidates of a sample = samples in the
-    3 x 3 block of cells (side = radius) around its cell."""
-    c = counts2d.astype(np.int64)
-    p = np.pad(c, 1)
-    s = sum(p[1 + dy:1 + dy + c.shape[0], 1 + dx:1 + dx + c.shape[1]]
-            for dy in (-1, 0, 1) for dx in (-1, 0, 1))
-    return 8 * int((c * s).sum())
-
-
-def cpu_baseline(graph, opts, target_s=15.0):
-    """The CPU oracle (all host threads) on a bounded sample of the same
-    schedule: neighbor means of every `stride`-th sample for all 30 radii at
-    the initial positions, scaled by `stride`, plus the full density stages."""
+def cpu_baseline(graph, opts, positions, target_s=15.0):
+    """The CPU oracle on all host threads, following the schedule: iteration
+    i's neighbor means are computed from the positions the GPU run fed that
+    iteration (`positions[i]`, read back after each eager iteration), with
+    the radius of iteration i, on every stride-th sample, scaled by the
+    stride; the per-call work that does not scale with the sample (sorting
+    all samples into the grid, the cell summaries) is timed by a one-sample
+    call and counted once. The density stages (splat, blur, normalize) run
+    in full. Two forms of the mean: the reference's tiled algorithm (cells of
+    side r, fully-inside cells by their summaries, neighbor_mean.wgsl:40-161)
+    -- the baseline -- and the plain window scan of every candidate."""
     from oracle import oracle as orc
     p0 = opts.iter_params(0)
-    pl = orc.Pipeline(graph.points, graph.starts, graph.bbox, p0)
-    pts, fixed = pl.points, pl.fixed
+    pl = orc.Pipeline(graph.points, graph.starts, graph.bbox, p0, with_density=False)
+    fixed = pl.fixed
+    n = len(fixed)
     threads = os.cpu_count() or 1
     threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS", threads))))
-    grids = [(orc.grid(opts.iter_params(i)), opts.iter_params(i)) for i in range(ITERS)]
+    params = [opts.iter_params(i) for i in range(ITERS)]
+    grids = [orc.make_grid(p.res_x, p.radius) for p in params]
+    fns = {"cells": orc.neighbor_mean_cells, "window": orc.neighbor_mean_strided}
 
-    def schedule(stride, threads=threads):
-        """Returns the scaled schedule time, and the sample's own mean and
-        density times."""
-        total = tmean = tdens = 0.0
-        for g, p in grids:
-            # per-call cost that does not scale with the sample (binning all
-            # points, the pass over skipped ones): a call that means one sample
+    def schedule(fn, stride, threads=threads):
+        """Scaled schedule time, the sample's own mean time, and the time of
+        the parts that do not scale with the sample."""
+        total = tmean = tother = 0.0
+        for pts, g, p in zip(positions, grids, params):
             t = time.perf_counter()
-            orc.neighbor_mean_strided(pts, fixed, g, len(pts), threads)
+            fn(pts, fixed, g, n, threads)  # one sample: the per-call fixed work
             tf = time.perf_counter() - t
             t = time.perf_counter()
-            out = orc.neighbor_mean_strided(pts, fixed, g, stride, threads)
+            out = fn(pts, fixed, g, stride, threads)
             tm = max(time.perf_counter() - t - tf, 0.0)
             t = time.perf_counter()
-            c = orc.splat(out, p.density_w, p.density_h)
-            orc.density(c, p.sigma)
+            orc.density(orc.splat(out, p.density_w, p.density_h), p.sigma)
             td = time.perf_counter() - t
             total += tm * stride + tf + td
             tmean += tm
-            tdens += td
-        return total, tmean, tdens
+            tother += tf + td
+        return total, tmean, tother
 
-    # a coarse pass over the whole schedule sets the stride of the timed pass
-    coarse = 2048
-    _, tm, td = schedule(coarse)
-    stride = int(max(1, min(coarse, np.ceil(coarse * tm / max(target_s - td, 1.0)))))
-    total, tm, td = schedule(stride)
-    n = len(pts)
-    # the same on one thread over a sample `threads` times sparser, so it takes
-    # about as long as the timed pass
+    res = {}
+    for name, fn in fns.items():
+        # a coarse pass over the whole schedule sets the stride of the timed pass
+        coarse = 4096
+        _, tm, other = schedule(fn, coarse)
+        avail = max(target_s / 2.0 - other, 1.0)  # the timed pass takes about target_s / 2
+        stride = int(max(1, min(coarse, np.ceil(coarse * tm / avail))))
+        total, _, _ = schedule(fn, stride)
+        res[name] = (total, stride)
+        log(f"cpu baseline [{name}]: stride {stride}, est. {total:.1f} s per schedule")
+    total, stride = res["cells"]
     stride1 = int(stride * threads)
-    total1, _, _ = schedule(stride1, threads=1)
+    total1, _, _ = schedule(fns["cells"], stride1, threads=1)
+    wtotal, wstride = res["window"]
     return {"value": ITERS * n / total, "unit": "sample-point updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle neighbor means of one sample in {stride} for all {ITERS} "
-                      f"radii at the initial positions (x{stride}), full splat/blur/normalize; "
-                      f"est. {total:.1f} s per schedule",
+            "sample": f"oracle restatement of the reference's tiled neighbor mean (cells of side r, "
+                      f"fully-inside cells by summary) on the positions each of the {ITERS} "
+                      f"iterations of the GPU run started from, one sample in {stride} (x{stride}) "
+                      f"plus full splat/blur/normalize; est. {total:.1f} s per schedule",
             "value_1thread": ITERS * n / total1,
-            "sample_1thread": f"one sample in {stride1}, 1 thread; est. {total1:.1f} s per "
-                              f"schedule"}
+            "sample_1thread": f"the same, one sample in {stride1}, 1 thread; est. {total1:.1f} s "
+                              f"per schedule",
+            "window_scan_value": ITERS * n / wtotal,
+            "window_scan_sample": f"every candidate of the cells within reach tested, one sample "
+                                  f"in {wstride}; est. {wtotal:.1f} s per schedule"}
 
 
 def pct(xs, q):
@@ -211,12 +225,26 @@ def render_section(bt, gpu, opts, reps=5):
 
 # FIFO for throughput; shortest-first with 1M sample-updates per tick for
 # latency (shorter ticks let small jobs through sooner, at some throughput)
-SERVER_POLICIES = (("server", "FIFO", 0), ("server_shortest", "SHORTEST", 1 << 20))
+SERVER_POLICIES = (("fifo", "FIFO", 0), ("shortest_1M", "SHORTEST", 1 << 20))
+# Density policies, each applied to both forms: the density of every
+# iteration (as Gpu::iterate computes it), or only the one that is returned,
+# the last iteration's (BT_FLAG_DENSITY_ON_READ).
+DENSITY_POLICIES = (("final_only", 4), ("every_iteration", 0))
 
 
-def serve(bt, jobs, policy, tick):
-    """Feeds the arrival stream to a server in real time; returns (makespan s,
-    [(job index, latency ms)], ticks, (start, end) on the monotonic clock)."""
+def _flagged(bt, opts, flags):
+    """A copy of job options with extra bt_options.flags set."""
+    o = bt.Options()
+    o.c = type(opts.c).from_buffer_copy(opts.c)
+    o.c.flags |= flags
+    return o
+
+
+def serve(bt, jobs, policy, tick, flags=0, realtime=True):
+    """Feeds the jobs to a server: in real time at their arrival times
+    (realtime) or all at once (saturated: the server's capacity). Returns
+    (makespan s, [(job index, latency ms)], ticks, (start, end) on the
+    monotonic clock); latency = completion - arrival (saturated: - start)."""
     n_jobs = len(jobs)
     srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy,
                     tick_samples=tick)
@@ -226,19 +254,21 @@ def serve(bt, jobs, policy, tick):
     ticks = 0
     while done < n_jobs:
         now = time.perf_counter() - t0
-        while i < n_jobs and jobs[i][0] <= now:
+        while i < n_jobs and (not realtime or jobs[i][0] <= now):
             ta, g, o, it = jobs[i]
-            pending[srv.submit(g, o, it)] = (ta, i)
+            pending[srv.submit(g, _flagged(bt, o, flags) if flags else o, it)] = (
+                ta if realtime else 0.0, i)
             i += 1
         running, queued = srv.step()
         ticks += 1
-        for tk in list(pending):
+        for tk in srv.take_finished():
             r = srv.poll(tk)
-            if r.state >= 2:
-                ta, k = pending.pop(tk)
-                slat.append((k, r.done_ms - (clk0 + ta * 1e3)))
-                srv.release(tk)
-                done += 1
+            ta, k = pending.pop(tk)
+            if r.state != 2:
+                raise RuntimeError(f"served job {k} failed")
+            slat.append((k, r.done_ms - (clk0 + ta * 1e3)))
+            srv.release(tk)
+            done += 1
         if running == 0 and queued == 0 and i < n_jobs:
             time.sleep(max(0.0, jobs[i][0] - (time.perf_counter() - t0)))
     t1 = time.perf_counter()
@@ -262,60 +292,95 @@ def profile_batched(bt, workloads, n_graphs=256, iters=30):
         f"window {t0:.4f} {t1:.4f}")
 
 
-def profile_served(bt, workloads, n_jobs=100, rate_hz=200.0):
-    """For rocprofv3 runs: the served stream under each server policy; prints
-    each run's window."""
+def profile_served(bt, workloads, n_jobs=1000, rate_hz=200.0):
+    """For rocprofv3 runs: the served stream (final-only density) under each
+    server policy, in real time; prints each run's window."""
     jobs = workloads.served_jobs(n_jobs, rate_hz)
-    serve(bt, jobs[:3], bt.Server.FIFO, 0)  # warm the code paths
+    serve(bt, jobs[:3], bt.Server.FIFO, 0, 4)  # warm the code paths
     for name, policy, tick in SERVER_POLICIES:
-        srv_s, _, ticks, (t0, t1) = serve(bt, jobs, getattr(bt.Server, policy), tick)
+        srv_s, _, ticks, (t0, t1) = serve(bt, jobs, getattr(bt.Server, policy), tick, 4)
         log(f"served {name}: makespan {srv_s:.3f} s, {ticks} ticks, window {t0:.4f} {t1:.4f}")
 
 
+def _plain_per_job(bt, jobs, flags):
+    """The plain per-job drop-in over the jobs, one after another with no
+    waiting (its capacity): upload, one blocking bt_iterate per iteration,
+    read the samples and the density. Returns (seconds, [service seconds],
+    sample-updates)."""
+    gpu = bt.Gpu()
+    gpu.upload_batch([jobs[0][1]], [_flagged(bt, jobs[0][2], flags)])
+    gpu.iterate(2)
+    gpu.density(3, 0)
+    service, updates = [], 0
+    t0 = time.perf_counter()
+    for _, g, o, it in jobs:
+        ts = time.perf_counter()
+        n = gpu.upload_batch([g], [_flagged(bt, o, flags)])
+        gpu.iterate(it)
+        gpu.points(0)
+        gpu.density(3, 0)
+        service.append(time.perf_counter() - ts)
+        updates += it * int(n[0])
+    total = time.perf_counter() - t0
+    gpu.close()
+    return total, service, updates
+
+
+def _fifo_queue_latency(arrivals, service):
+    """Latencies (ms) of a single FIFO server with these arrival times and
+    service times: what the plain per-job form gives on the arrival stream
+    (it is busy from one job's arrival until the job is done)."""
+    out, free = [], 0.0
+    for ta, d in zip(arrivals, service):
+        free = max(free, ta) + d
+        out.append((free - ta) * 1e3)
+    return out
+
+
 def served_section(bt, workloads, n_jobs, rate_hz=200.0):
-    """Poisson arrivals at rate_hz: the plain per-job form (FIFO, one job at a
-    time, as the drop-in does) against the continuous-batching server."""
+    """Poisson arrivals at rate_hz (100-50,000 edges, 20-50 iterations,
+    512^2-2048^2 density, spacing 0.01): the plain per-job drop-in against the
+    continuous-batching server, both under the same density policy. For each
+    form: capacity (every job available at once) and latency on the arrival
+    stream (real time; the plain form's from a FIFO queue of its measured
+    service times, since it cannot keep up with the stream)."""
     t = time.perf_counter()
     jobs = workloads.served_jobs(n_jobs, rate_hz)
     log(f"served: generated {n_jobs} jobs in {time.perf_counter() - t:.1f} s")
-    total_updates = 0
-    # plain per-job FIFO
-    gpu = bt.Gpu()
-    gpu.upload_batch([jobs[0][1]], [jobs[0][2]])
-    gpu.iterate(2)
-    lat = []
-    t0 = time.perf_counter()
-    for ta, g, o, it in jobs:
-        now = time.perf_counter() - t0
-        if now < ta:
-            time.sleep(ta - now)
-        s = gpu.upload_batch([g], [o])
-        gpu.iterate(it)
-        gpu.points(0)
-        lat.append((time.perf_counter() - t0 - ta) * 1e3)
-        total_updates += it * int(s[0])
-    plain_s = time.perf_counter() - t0
-    gpu.close()
-    log(f"served: plain FIFO makespan {plain_s:.2f} s")
-    # continuous-batching server, both policies
-    runs = {}
-    for name, policy, tick in SERVER_POLICIES:
-        srv_s, slat, ticks, _ = serve(bt, jobs, getattr(bt.Server, policy), tick)
-        runs[name] = (srv_s, slat, ticks)
-        log(f"served: {name} makespan {srv_s:.2f} s in {ticks} ticks")
     small = set(k for k, (_, g, _, _) in enumerate(jobs) if g.edge_count <= 20000)
-    out = {"jobs": n_jobs, "rate_hz": rate_hz, "updates": total_updates,
-           "plain_fifo": {"updates_per_s": total_updates / plain_s, "makespan_s": plain_s,
-                          "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                          "p99_ms_le20k_edges": pct([lat[k] for k in sorted(small)], 99)}}
-    for name, (srv_s, slat, ticks) in runs.items():
-        out[name] = {"updates_per_s": total_updates / srv_s, "makespan_s": srv_s,
-                     "p50_ms": pct([v for _, v in slat], 50),
-                     "p99_ms": pct([v for _, v in slat], 99),
-                     "p50_ms_le20k_edges": pct([v for k, v in slat if k in small], 50),
-                     "p99_ms_le20k_edges": pct([v for k, v in slat if k in small], 99),
-                     "ticks": ticks, "speedup": plain_s / srv_s}
-    out["server_shortest"]["tick_samples"] = 1 << 20
+    arrivals = [j[0] for j in jobs]
+
+    def lat_stats(lat):
+        lat = dict(lat)
+        return {"p50_ms": pct(list(lat.values()), 50), "p99_ms": pct(list(lat.values()), 99),
+                "p50_ms_le20k_edges": pct([v for k, v in lat.items() if k in small], 50),
+                "p99_ms_le20k_edges": pct([v for k, v in lat.items() if k in small], 99)}
+
+    out = {"jobs": n_jobs, "rate_hz": rate_hz, "jobs_le20k_edges": len(small),
+           "spacing": 0.01, "arrivals_s": arrivals[-1]}
+    serve(bt, jobs[:3], bt.Server.FIFO, 0, 4)  # warm the code paths
+    for dname, flags in DENSITY_POLICIES:
+        plain_s, service, updates = _plain_per_job(bt, jobs, flags)
+        out["updates"] = updates
+        out["offered_updates_per_s"] = updates / arrivals[-1]
+        plain_cap = updates / plain_s
+        sec = {"plain": {"capacity_updates_per_s": plain_cap, "seconds": plain_s,
+                         **lat_stats(enumerate(_fifo_queue_latency(arrivals, service)))}}
+        log(f"served [{dname}]: plain per-job {plain_s:.2f} s, {plain_cap:.3e} updates/s")
+        for name, policy, tick in SERVER_POLICIES:
+            cap_s, _, cap_ticks, _ = serve(bt, jobs, getattr(bt.Server, policy), tick, flags,
+                                           realtime=False)
+            rt_s, rt_lat, rt_ticks, _ = serve(bt, jobs, getattr(bt.Server, policy), tick, flags)
+            sec[name] = {"capacity_updates_per_s": updates / cap_s, "capacity_seconds": cap_s,
+                         "capacity_vs_plain": plain_s / cap_s, "capacity_ticks": cap_ticks,
+                         "stream_updates_per_s": updates / rt_s, "stream_makespan_s": rt_s,
+                         "stream_ticks": rt_ticks, **lat_stats(rt_lat)}
+            if tick:
+                sec[name]["tick_samples"] = tick
+            log(f"served [{dname}] {name}: capacity {updates / cap_s:.3e} updates/s "
+                f"({plain_s / cap_s:.2f}x plain); stream p99 {sec[name]['p99_ms']:.1f} ms, "
+                f"jobs <= 20k edges p99 {sec[name]['p99_ms_le20k_edges']:.1f} ms")
+        out[dname] = sec
     return out
 
 
@@ -327,7 +392,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell-split", type=int, default=0)
     ap.add_argument("--batched", type=int, default=256, help="graphs in the batched case (0: skip)")
-    ap.add_argument("--served", type=int, default=100, help="jobs in the served case (0: skip)")
+    ap.add_argument("--served", type=int, default=1000, help="jobs in the served case (0: skip)")
     ap.add_argument("--large", type=int, default=1, help="run the 100,000-edge case (0: skip)")
     ap.add_argument("--render", type=int, default=1,
                     help="time the export image of the bundled plain GPU case (0: skip)")
@@ -403,28 +468,31 @@ def main():
     value = ITERS * n * args.steps / total
 
     # Per-stage device time from HIP events on the engine's stream (eager run of
-    # the same schedule), plus the grid counts for the FP32-work figure.
+    # the same schedule), plus the neighbor mean's executed work per iteration.
     gpu.rewind()
     stage_ms = {k: 0.0 for k in bt.api.STAGES}
-    flops = 0
+    steps_total = 0
+    movable = n - 2 * int(np.count_nonzero(gpu.paths()[:, 2] >= 2))
     per_iter = []
+    positions = []  # the input of every iteration, for the CPU baseline
     for i in range(ITERS):
+        if not args.no_cpu:
+            positions.append(gpu.points()[0])
         st, _ = gpu.profile(1)
         for k, v in st.items():
             stage_ms[k] += v
-        _, counts = gpu.grid()
         p = gpu.job_params(i)
         info = gpu.info()
+        steps_total += info.test_steps
+        nm_ms = st["neighbor_mean"]
         per_iter.append({"radius": round(p.radius, 3), "cell": round(p.cell, 3),
-                         "mean_ms": round(st["neighbor_mean"], 3),
-                         "split_tasks": info.split_tasks, "split_parts": info.split_parts})
-        ky = max(1, int(round(p.radius / p.cell)))      # rows per radius
-        kx = max(1, int(round(p.radius / p.cell_w)))    # bins per radius
-        gy, gx = counts.shape
-        pad = np.zeros(((gy + ky - 1) // ky * ky, (gx + kx - 1) // kx * kx), dtype=np.int64)
-        pad[:gy, :gx] = counts
-        coarse = pad.reshape(pad.shape[0] // ky, ky, pad.shape[1] // kx, kx).sum(axis=(1, 3))
-        flops += candidate_flops(coarse)
+                         "mean_ms": round(nm_ms, 3),
+                         "split_tasks": info.split_tasks, "split_parts": info.split_parts,
+                         "wave_steps": info.test_steps,
+                         "tests_per_sample": round(64 * info.test_steps / movable, 1),
+                         "neighbors_per_sample": round(info.neighbors / movable, 1),
+                         "valu_issue_frac": round(info.test_steps * VALU_PER_STEP
+                                                  / (nm_ms * 1e-3) / VALU_ISSUE_PEAK, 3)})
     dom = max(stage_ms, key=stage_ms.get)
     mean_ms = stage_ms["neighbor_mean"] / ITERS
     P = 1024 * 1024
@@ -439,11 +507,17 @@ def main():
             "unit": "GB/s", "traffic": traffic, "algorithmic_bytes": alg_bytes,
             "avg_launch_ms": mean_ms}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    valu = {"kernel": "bt_neighbor_mean+bt_neighbor_mean_heavy", "bound": "valu",
-            "achieved": flops / (stage_ms["neighbor_mean"] * 1e-3) / 1e12,
-            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "flops_per_schedule": flops}
-    valu["frac"] = valu["achieved"] / valu["peak"]
+    # executed work: wave-steps of the test loop x its VALU instructions,
+    # against the SIMDs' issue capacity (the kernel's real bound)
+    executed = {"kernel": "bt_neighbor_mean+bt_neighbor_mean_heavy", "bound": "valu",
+                "wave_steps_per_schedule": steps_total,
+                "lane_tests_per_schedule": 64 * steps_total,
+                "valu_per_step": VALU_PER_STEP,
+                "achieved": steps_total * VALU_PER_STEP / (stage_ms["neighbor_mean"] * 1e-3),
+                "peak": VALU_ISSUE_PEAK, "unit": "wave-instructions/s",
+                "note": "test-loop VALU instructions only (set-up, range walks and the "
+                        "density splat are not counted)"}
+    executed["frac"] = executed["achieved"] / executed["peak"]
     iter_bytes = 24 * n + 28 * P
     path = {"bytes_per_iteration": iter_bytes,
             "achieved_GBs": iter_bytes * ITERS / (total / args.steps) / 1e9}
@@ -457,7 +531,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
-        cpu = cpu_baseline(graph, opts)
+        cpu = cpu_baseline(graph, opts, positions)
         log(f"cpu baseline: {cpu['value']:.3e} updates/s ({time.perf_counter() - t:.1f} s)")
 
     out = {
@@ -485,7 +559,7 @@ def main():
         "stage_ms_per_schedule": stage_ms,
         "dominant_stage": dom,
         "roofline": roof,
-        "valu": valu,
+        "executed": executed,
         "path": path,
         "cpu_baseline": cpu,
         "batched": batched,

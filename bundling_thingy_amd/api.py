@@ -58,7 +58,9 @@ class bt_info(C.Structure):
     _fields_ = [("jobs", C.c_uint32), ("samples", C.c_uint32), ("paths", C.c_uint32),
                 ("iteration", C.c_uint32), ("capacity", C.c_uint64),
                 ("reserved_bytes", C.c_uint64), ("device_bytes", C.c_uint64),
-                ("split_tasks", C.c_uint32), ("split_parts", C.c_uint32)]
+                ("split_tasks", C.c_uint32), ("split_parts", C.c_uint32),
+                ("test_steps", C.c_uint64), ("neighbors", C.c_uint64),
+                ("candidates", C.c_uint64)]
 
 
 class bt_render_options(C.Structure):
@@ -155,6 +157,7 @@ SIGNATURES = {
     "bt_server_release": (C.c_int, [_P, _U64]),
     "bt_server_clock_ms": (_D, [_P]),
     "bt_server_get_info": (C.c_int, [_P, C.POINTER(bt_server_info)]),
+    "bt_server_take_finished": (C.c_int, [_P, _PU64, _SZ, C.POINTER(_SZ)]),
 }
 
 _lib = None
@@ -695,6 +698,13 @@ class Server:
 
     def clock_ms(self):
         return lib().bt_server_clock_ms(self._h)
+
+    def take_finished(self, capacity=4096):
+        """Tickets finished (done or failed) since the last call, in order."""
+        buf = (C.c_uint64 * capacity)()
+        n = _SZ()
+        self._check(lib().bt_server_take_finished(self._h, buf, capacity, C.byref(n)))
+        return list(buf[:n.value])
 
     def info(self):
         """Admission accounting: queued/running/finishing jobs, the footprint

@@ -258,7 +258,11 @@ int bt_get_info(bt_ctx* ctx, bt_info* info) {
     info->reserved_bytes = j.reserved;
   }
   return guarded(&ctx->err,
-                 [&] { ctx->engine->split_counts(&info->split_tasks, &info->split_parts); });
+                 [&] {
+                   ctx->engine->split_counts(&info->split_tasks, &info->split_parts);
+                   ctx->engine->work_counts(&info->test_steps, &info->neighbors,
+                                            &info->candidates);
+                 });
 }
 
 int bt_iter_params_for(const bt_options* opts, uint32_t iteration, bt_iter_params* out) {
@@ -380,7 +384,8 @@ struct ServedJob {
   uint32_t iterations = 0;
   uint64_t estimate = 0;  // device bytes expected before admission
   uint64_t held = 0;      // device bytes its buffers hold while admitted
-  uint64_t est_samples = 0;  // samples expected (the resample length model)
+  uint64_t est_samples = 0;  // samples of the count pass (exact)
+  std::vector<uint64_t> path_scan;  // the count pass's path offsets, for the upload
   double last_ms = 0;     // server clock of admission or of the last iteration
   // retirement in flight: results copied into a pinned buffer, `done` marks
   // the end of the copies
@@ -414,6 +419,7 @@ struct bt_server {
   std::vector<ServedJob*> finishing;          // results being copied out
   std::multimap<size_t, void*> pinned_free;   // pinned result buffers for reuse
   std::deque<hipEvent_t> ticks;               // ends of the ticks still on the GPU
+  std::deque<uint64_t> finished;              // tickets done or failed, not yet taken
   std::map<uint64_t, std::unique_ptr<ServedJob>> all;
   std::string err;
   double now() const {
@@ -424,10 +430,11 @@ struct bt_server {
 
 namespace {
 
-// Device bytes a job will hold, estimated before resampling from the polyline
-// lengths in texels (the count pass yields the exact figure after admission).
-// Device bytes of a job (and, in *samples, its expected sample count).
-uint64_t estimate_bytes(const ServedJob& s, uint64_t* samples_out) {
+// The count pass on the host at submission (the exact sample count, and the
+// path offsets the upload reuses), then the device bytes of the job's
+// buffers as upload_jobs allocates them (pool blocks). Returns 0 when the
+// graph cannot be normalized.
+uint64_t estimate_bytes(ServedJob& s, uint64_t* samples_out) {
   bt_options o = bt::resolve_options(&s.opts);
   float bbox[4] = {3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f};
   const uint32_t np = (uint32_t)(s.xy.size() / 2);
@@ -440,20 +447,19 @@ uint64_t estimate_bytes(const ServedJob& s, uint64_t* samples_out) {
   float norm[5];
   if (!bt::normalization(bbox, norm)) return 0;
   bt_iter_params p;
-  bt::iter_params(o, 0, 1, &p);
-  double samples = 0.0;
-  for (size_t e = 0; e + 1 < s.starts.size(); ++e) {
-    for (uint32_t k = s.starts[e] + 1; k < s.starts[e + 1]; ++k) {
-      const double dx = (double)(s.xy[2 * k] - s.xy[2 * k - 2]) * norm[2] * p.res_x;
-      const double dy = (double)(s.xy[2 * k + 1] - s.xy[2 * k - 1]) * norm[2] * p.res_y;
-      samples += std::sqrt(dx * dx + dy * dy);
-    }
-    samples += 2.0;
-  }
-  // a path of length D texels yields at most floor(D) + 2 samples
-  const uint64_t n = (uint64_t)(samples * 1.02) + 64;
+  bt::iter_params(o, 0, o.cell_split, &p);
+  uint64_t cap = 0, reserved = 0;
+  std::string err;
+  if (bt::capacity_model(o, np, (uint32_t)s.starts.size(), &cap, &reserved, &err) != BT_OK)
+    cap = 0xfffffffeull;  // the upload reports the budget error at admission
+  const uint32_t n_paths = (uint32_t)s.starts.size() - 1u;
+  s.path_scan.resize(s.starts.size());
+  const uint64_t total = bt::resample_counts(s.xy.data(), s.starts.data(), n_paths, norm,
+                                             p.res_x * 0.5f, p.res_y * 0.5f,
+                                             (uint32_t)std::min<uint64_t>(cap, 0xfffffffeull),
+                                             s.path_scan.data());
+  const uint64_t n = std::max<uint64_t>(total >> 32, 2);
   *samples_out = n;
-  // the buffers upload_jobs allocates for n samples, as pool blocks
   return bt::Job::footprint(o, n, np, (uint32_t)s.starts.size(), false);
 }
 
@@ -478,9 +484,20 @@ void reap(bt_server* srv, bool block) {
       bt::fail(BT_E_DEVICE, std::string("HIP: ") + hipGetErrorString(q));
     }
     const size_t pb = (size_t)s->samples * 12;
-    s->points.assign((const unsigned char*)s->pinned, (const unsigned char*)s->pinned + pb);
-    if (!s->density.empty())
-      std::memcpy(s->density.data(), (const unsigned char*)s->pinned + pb, s->density.size() * 4);
+    const unsigned char* pin = (const unsigned char*)s->pinned;
+    uint32_t status = 0;
+    std::memcpy(&status, pin + pb + s->density.size() * 4, 4);
+    if (status & 2u) {
+      // the emit pass disagreed with the count pass (resampling.rs:296-299)
+      s->points.clear();
+      s->density.clear();
+      s->state = 3;
+      s->err = "GPU resampling count and write passes disagree";
+    } else {
+      s->points.assign(pin, pin + pb);
+      if (!s->density.empty()) std::memcpy(s->density.data(), pin + pb, s->density.size() * 4);
+      s->state = 2;
+    }
     srv->pinned_free.emplace(s->pinned_bytes, s->pinned);
     s->pinned = nullptr;
     (void)hipEventDestroy(s->done);
@@ -489,8 +506,8 @@ void reap(bt_server* srv, bool block) {
     srv->used -= std::min(srv->used, s->estimate);
     srv->held -= std::min(srv->held, s->held);
     s->held = 0;
-    s->state = 2;
     s->done_ms = srv->now();
+    srv->finished.push_back(s->ticket);
   }
 }
 
@@ -635,10 +652,11 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
       std::vector<bt::GraphIn> gs;
       for (ServedJob* s : admit)
         gs.push_back({s->xy.data(), (uint32_t)(s->xy.size() / 2), s->starts.data(),
-                      (uint32_t)s->starts.size(), s->opts});
+                      (uint32_t)s->starts.size(), s->opts, s->path_scan.data()});
       std::vector<std::unique_ptr<bt::Job>> jobs;
       try {
-        e.upload_jobs(gs, &jobs, true, false);  // overlaps queued ticks
+        // queued behind the ticks on the GPU; the host does not wait
+        e.upload_jobs(gs, &jobs, true, false);
       } catch (const bt::Error& err) {
         // Isolate failures: retry one by one so one bad graph fails alone.
         jobs.clear();
@@ -651,6 +669,7 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
             admit[i]->state = 3;
             admit[i]->err = err1.msg;
             admit[i]->done_ms = srv->now();
+            srv->finished.push_back(admit[i]->ticket);
             jobs.push_back(nullptr);
           }
         }
@@ -703,6 +722,7 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
       std::vector<bt::Job*> js;
       for (ServedJob* s : adv) js.push_back(s->job.get());
       e.iterate_jobs(js, false);  // queued; the next tick's host work overlaps it
+
       hipEvent_t ev = nullptr;
       HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
       const hipError_t rq = hipEventRecord(ev, e.stream());
@@ -723,7 +743,8 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
         s->dh = dens ? s->job->dh : 0;
         s->density.assign((size_t)s->dw * s->dh, 0.0f);
         const size_t pb = (size_t)s->samples * 12;
-        s->pinned = get_pinned(srv, pb + s->density.size() * 4, &s->pinned_bytes);
+        // records, density, then the upload's status word
+        s->pinned = get_pinned(srv, pb + s->density.size() * 4 + 4, &s->pinned_bytes);
         HIPCHK(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
         e.queue_readback(*s->job, s->pinned,
                          dens ? (float*)((unsigned char*)s->pinned + pb) : nullptr, s->done);
@@ -734,6 +755,19 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
     if (running) *running = (uint32_t)(srv->running.size() + srv->finishing.size());
     if (queued) *queued = (uint32_t)srv->queue.size();
   });
+}
+
+int bt_server_take_finished(bt_server* srv, uint64_t* tickets, size_t capacity, size_t* n) {
+  if (!srv || !n || (capacity && !tickets)) return BT_E_INVALID;
+  int rc = guarded(&srv->err, [&] { reap(srv, false); });
+  if (rc != BT_OK) return rc;
+  size_t k = 0;
+  while (k < capacity && !srv->finished.empty()) {
+    tickets[k++] = srv->finished.front();
+    srv->finished.pop_front();
+  }
+  *n = k;
+  return BT_OK;
 }
 
 int bt_server_get_info(const bt_server* srv, bt_server_info* out) {

@@ -295,6 +295,104 @@ bool normalization(const float* bbox, float* out) {
   return true;
 }
 
+namespace {
+
+// resampling.wgsl:45-48
+inline void normalized_at(const float* xy, const float* nm, uint32_t i, float& x, float& y) {
+  x = ((xy[2u * i] - nm[0]) * nm[2] + nm[3] - 0.5f) * 2.0f;
+  y = ((xy[2u * i + 1u] - nm[1]) * nm[2] + nm[4] - 0.5f) * 2.0f;
+}
+
+inline uint32_t steps_for(float distance, float remaining) {
+  const float f = std::max(std::floor(distance - remaining) + 1.0f, 0.0f);
+  return f >= 4294967296.0f ? 0xffffffffu : (uint32_t)f;
+}
+
+// One path: the walk of resampling.wgsl:82-134 counting pushes. A segment
+// of D texels takes steps = max(floor(D - remaining) + 1, 0) samples at
+// prev + delta * ((remaining + k) / D); only its first can repeat the last
+// sample pushed, and only its last is compared with what follows.
+uint32_t count_path(const float* xy, uint32_t s, uint32_t e, const float* nm, float tsx,
+                    float tsy, uint32_t cap) {
+  float fx, fy;
+  normalized_at(xy, nm, s, fx, fy);
+  uint32_t count = 0;
+  float lx = fx, ly = fy;
+  auto push = [&](float x, float y) {
+    if (count >= cap) {
+      count = cap + 1u;
+    } else {
+      lx = x;
+      ly = y;
+      ++count;
+    }
+  };
+  push(fx, fy);
+  float px = fx, py = fy, remaining = 1.0f;
+  bool over = count > cap;
+  for (uint32_t k = s + 1u; k < e && !over; ++k) {
+    float cx, cy;
+    normalized_at(xy, nm, k, cx, cy);
+    const float dx = cx - px, dy = cy - py;
+    const float ex = dx * tsx, ey = dy * tsy;
+    const float distance = std::sqrt(std::fma(ex, ex, ey * ey));
+    if (distance > 0.0f) {
+      const uint32_t steps = steps_for(distance, remaining);
+      if (steps > 0u) {
+        const float t0 = remaining / distance;
+        const float t1 = (remaining + (float)(steps - 1u)) / distance;
+        const float sx = px + dx * t0, sy = py + dy * t0;
+        const float ux = px + dx * t1, uy = py + dy * t1;
+        const uint64_t c = (uint64_t)count + steps - ((sx == lx && sy == ly) ? 1u : 0u);
+        lx = ux;
+        ly = uy;
+        if (c > cap) {
+          count = cap + 1u;
+          over = true;
+        } else {
+          count = (uint32_t)c;
+        }
+      }
+      remaining = remaining + (float)steps - distance;
+    }
+    px = cx;
+    py = cy;
+  }
+  if (!over) {
+    // the last control point (resampling.wgsl:117-119)
+    if (!(count > 0u && px == lx && py == ly)) push(px, py);
+    if (count == 1u) {
+      // a closed path that collapsed to one sample keeps its first distinct
+      // point and its end (resampling.wgsl:124-134)
+      for (uint32_t k = s + 1u; k < e; ++k) {
+        float nx, ny;
+        normalized_at(xy, nm, k, nx, ny);
+        if (nx != fx || ny != fy) {
+          if (!(nx == lx && ny == ly)) push(nx, ny);
+          if (!(count > 0u && px == lx && py == ly)) push(px, py);
+          break;
+        }
+      }
+    }
+  }
+  return count;
+}
+
+}  // namespace
+
+uint64_t resample_counts(const float* xy, const uint32_t* starts, uint32_t n_paths,
+                         const float* norm, float ts_x, float ts_y, uint32_t cap,
+                         uint64_t* path_scan) {
+  uint64_t acc = 0;
+  for (uint32_t i = 0; i < n_paths; ++i) {
+    path_scan[i] = acc;
+    const uint32_t c = count_path(xy, starts[i], starts[i + 1u], norm, ts_x, ts_y, cap);
+    if (c >= 2u) acc += ((uint64_t)c << 32) | 1ull;
+  }
+  path_scan[n_paths] = acc;
+  return acc;
+}
+
 // NeighborMean::grid_bytes (neighbor_mean.rs:176-183) for a cell count.
 static uint64_t grid_bytes_for(uint64_t cells) { return cells * 8 + 8 + cells * 24 + 52; }
 

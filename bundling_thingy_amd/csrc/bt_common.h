@@ -109,6 +109,18 @@ uint32_t blur_weights(float sigma, float* w, uint32_t max_taps);
 // graph.rs:79-91
 bool normalization(const float* bbox, float* out);
 
+// The count pass of resampling.wgsl:82-159 (count_paths) on the host, in the
+// f32 arithmetic the emit kernel uses (no contraction, correctly rounded
+// sqrtf and division): each path's sample count in closed form per segment,
+// duplicates of the previous sample dropped, the closed-path special case,
+// counts over `cap` stopped at cap + 1. path_scan (n_paths + 1 entries)
+// receives the exclusive prefix sums of (count << 32 | 1) over the paths
+// with two or more samples (0 for the others); returns the total, i.e.
+// samples << 32 | drawable paths. norm = {min_x, min_y, scale, tr_x, tr_y}.
+uint64_t resample_counts(const float* xy, const uint32_t* starts, uint32_t n_paths,
+                         const float* norm, float ts_x, float ts_y, uint32_t cap,
+                         uint64_t* path_scan);
+
 // resampling.rs:72-89,321-397; returns BT_OK or an error code with message.
 int capacity_model(const bt_options& o, uint32_t n_points, uint32_t n_starts, uint64_t* capacity,
                    uint64_t* reserved, std::string* err);

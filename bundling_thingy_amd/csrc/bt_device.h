@@ -107,7 +107,8 @@ struct BtControl {
   uint32_t pad0;
   unsigned long long cand_prev;  // candidates tested by the previous neighbor-mean
                                  // launch (0: none since upload or rewind)
-  uint32_t pad[BT_CTR_STRIDE - 6];
+  unsigned long long samples_prev;  // samples of that launch's jobs
+  uint32_t pad[BT_CTR_STRIDE - 8];
   // per-XCD work-queue cursors (part x at [x * BT_CTR_STRIDE]) of the
   // neighbor-mean kernel (xq) and of the heavy-part kernel (iq)
   uint32_t xq[BT_XCDS * BT_CTR_STRIDE];
@@ -117,9 +118,11 @@ struct BtControl {
 // Heavy tasks (candidate stream longer than a threshold H) are split into
 // parts run by many waves; their exact partial sums meet in per-task
 // accumulators, and the last part to finish writes the means. H is half the
-// previous launch's candidates per wave, clamped to [BT_HEAVY_LO, BT_HEAVY_HI]
+// expected candidates per wave, clamped to [BT_HEAVY_LO, BT_HEAVY_HI]
 // (BT_HEAVY_HI when there is no previous launch): a task much longer than the
-// average would otherwise run alone at the end. The main kernel stores each
+// average would otherwise run alone at the end. The expectation is the
+// previous launch's candidates scaled by this launch's samples over that
+// launch's (the server alternates launches of very different sizes). The main kernel stores each
 // heavy task's set-up (lane-group ranges and inside sums, BT_REC_WORDS words)
 // so the parts only stream.
 #define BT_HEAVY_LO 2048u
@@ -144,6 +147,13 @@ struct BtSplit {          // scratch of the heavy-task split (one per launch)
   long long* acc;         // [BT_MAX_HEAVY][3][64]: per-lane x, y, count sums
   unsigned long long* wg_cand;  // [BT_MAX_WG]: candidates tested per workgroup
   uint32_t* rec;          // [BT_MAX_HEAVY][BT_REC_WORDS]: set-up of each heavy task
+  // executed work of the last launch per workgroup (main kernel, heavy
+  // kernel): wave-steps (one candidate against the 64 lanes' samples) and
+  // neighbors found for the samples finished there
+  unsigned long long* wg_steps;   // [BT_MAX_WG]
+  unsigned long long* wg_nbr;     // [BT_MAX_WG]
+  unsigned long long* wg_hsteps;  // [BT_MAX_WG]
+  unsigned long long* wg_hnbr;    // [BT_MAX_WG]
 };
 
 struct __attribute__((aligned(16))) BtTask {

@@ -31,7 +31,8 @@ static const size_t kSplitItemsOff = (size_t)BT_MAX_HEAVY * sizeof(BtHeavy);
 static const size_t kSplitAccOff = kSplitItemsOff + (size_t)BT_MAX_ITEMS * sizeof(uint2);
 static const size_t kSplitCandOff = kSplitAccOff + (size_t)BT_MAX_HEAVY * 192 * sizeof(long long);
 static const size_t kSplitRecOff = kSplitCandOff + (size_t)BT_MAX_WG * sizeof(unsigned long long);
-static const size_t kSplitBytes = kSplitRecOff + (size_t)BT_MAX_HEAVY * BT_REC_WORDS * 4;
+static const size_t kSplitWorkOff = kSplitRecOff + (size_t)BT_MAX_HEAVY * BT_REC_WORDS * 4;
+static const size_t kSplitBytes = kSplitWorkOff + 4 * (size_t)BT_MAX_WG * sizeof(unsigned long long);
 
 BtJob Job::dev(uint32_t it, uint32_t parity_at, bool force_density) const {
   BtJob d;
@@ -56,7 +57,7 @@ BtJob Job::dev(uint32_t it, uint32_t parity_at, bool force_density) const {
   d.status = status.as<uint32_t>();
   d.ctrl = in_ctrl;
   d.starts = in_starts;
-  d.path_scan = path_scan.as<uint64_t>();
+  d.path_scan = nullptr;  // set by an upload's plan (the offsets sit in its input area)
   d.paths = paths.as<uint32_t>();
   d.n = n;
   d.n_paths = n_paths;
@@ -114,7 +115,7 @@ bool Job::set_split(uint32_t e, uint32_t k) {
 }
 
 void Job::set_pool(DevPool* pool) {
-  DevBuf* all[] = {&path_scan, &paths, &bsum, &status, &rec0,
+  DevBuf* all[] = {&paths, &bsum, &status, &rec0,
                    &rec1, &rec_saved, &sorted, &sid, &cellrank, &cells, &edge_of,
                    &pre_x, &pre_y, &dens_cnt, &dens_row, &dens_col, &dens_norm, &dens_saved,
                    &peak, &wt};
@@ -122,7 +123,7 @@ void Job::set_pool(DevPool* pool) {
 }
 
 size_t Job::device_bytes() const {
-  const DevBuf* all[] = {&path_scan, &paths, &bsum, &status, &rec0,
+  const DevBuf* all[] = {&paths, &bsum, &status, &rec0,
                          &rec1, &rec_saved, &sorted, &sid, &cellrank, &cells, &edge_of,
                          &pre_x, &pre_y,
                          &dens_cnt, &dens_row, &dens_col, &dens_norm, &dens_saved, &peak, &wt};
@@ -156,17 +157,17 @@ size_t Job::footprint(const bt_options& o, uint64_t n, uint32_t n_points, uint32
   const uint64_t nb = std::max<uint64_t>(
       std::max<uint64_t>(cdiv(paths + 1, BT_SCAN_ITEMS), cdiv(cells + 1, BT_SCAN_ITEMS)),
       2ull * cdiv(n + 1, BT_SCAN_ITEMS));
-  size_t s = blk(8ull * n_starts) + blk(12ull * std::max<uint64_t>(paths, 1)) + blk(16) +
-             blk(8 * nb) + blk(4ull * sched * kWt);
+  size_t s = blk(12ull * std::max<uint64_t>(paths, 1)) + blk(16) + blk(8 * nb) +
+             blk(4ull * sched * kWt);
   s += 3 * blk(12 * n) + blk(16 * n) + blk(4 * n) + blk(8 * n) + blk(4 * n) +
        blk(4 * (cells + 1)) + 2 * blk(8 * (n + 1));
   if (o.flags & BT_FLAG_DENSITY) {
     const uint64_t pb = 4ull * p0.density_w * p0.density_h;
     s += 4 * blk(pb) + blk(8) + (rewindable ? blk(4 * pb) : 0);
   }
-  // upload area: control points, starts and weights, 64-word aligned
+  // upload area: control points, starts, weights and path offsets, 64-word aligned
   s += blk(4 * (align_up(2ull * n_points, 64) + align_up(n_starts, 64) +
-                align_up((size_t)sched * kWt, 64) + 64));
+                align_up((size_t)sched * kWt, 64) + align_up(2ull * n_starts, 64) + 64));
   return s;
 }
 
@@ -179,15 +180,14 @@ Engine::Engine(const bt_options& o) : opts_(o), pool_(new DevPool) {
   nm_per_cu_ = btk::neighbor_mean_blocks_per_cu();
   noalloc_setup(device_);  // the allocation-free entry point's scratch on this device
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
   ctl_.alloc(sizeof(BtControl));
   HIPCHK(hipMemset(ctl_.p, 0, sizeof(BtControl)));
-  ctl_side_.alloc(sizeof(BtControl));
-  HIPCHK(hipMemset(ctl_side_.p, 0, sizeof(BtControl)));
   split_.alloc(kSplitBytes);
   HIPCHK(hipMemset(split_.p, 0, kSplitBytes));
   pinned_.resize(kPinnedSlots);
   for (Pinned& s : pinned_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+  up_pinned_.resize(4);
+  for (Pinned& s : up_pinned_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
   retire_plans_.alloc(kRetireSlots * kRetireSlotBytes);
 }
 
@@ -197,15 +197,14 @@ Engine::~Engine() {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
   }
-  if (side_) {
-    (void)hipStreamSynchronize(side_);
-    (void)hipStreamDestroy(side_);
-  }
   for (Pinned& s : pinned_) {
     if (s.ev) (void)hipEventDestroy(s.ev);
     if (s.p) (void)hipHostFree(s.p);
   }
-  if (up_host_) (void)hipHostFree(up_host_);
+  for (Pinned& s : up_pinned_) {
+    if (s.ev) (void)hipEventDestroy(s.ev);
+    if (s.p) (void)hipHostFree(s.p);
+  }
 }
 
 void Engine::sync() { HIPCHK(hipStreamSynchronize(stream_)); }
@@ -216,6 +215,24 @@ void Engine::split_counts(uint32_t* tasks, uint32_t* parts) {
   sync();
   *tasks = std::min(w[1], (uint32_t)BT_MAX_HEAVY);
   *parts = std::min(w[2], (uint32_t)BT_MAX_ITEMS);
+}
+
+void Engine::work_counts(uint64_t* steps, uint64_t* neighbors, uint64_t* candidates) {
+  const size_t wg = std::min<size_t>((size_t)cus_ * nm_per_cu_, BT_MAX_WG);
+  std::vector<unsigned long long> h(5 * wg);
+  const char* base = split_.as<char>();
+  for (int k = 0; k < 4; ++k)
+    HIPCHK(hipMemcpyAsync(h.data() + k * wg, base + kSplitWorkOff + (size_t)k * BT_MAX_WG * 8,
+                          wg * 8, hipMemcpyDeviceToHost, stream_));
+  HIPCHK(hipMemcpyAsync(h.data() + 4 * wg, base + kSplitCandOff, wg * 8, hipMemcpyDeviceToHost,
+                        stream_));
+  sync();
+  uint64_t a[5] = {0, 0, 0, 0, 0};
+  for (int k = 0; k < 5; ++k)
+    for (size_t b = 0; b < wg; ++b) a[k] += h[k * wg + b];
+  *steps = a[0] + a[2];
+  *neighbors = a[1] + a[3];
+  *candidates = a[4];
 }
 
 void Engine::drop_graphs() {
@@ -284,7 +301,10 @@ Plan Engine::make_plan_of(const std::vector<Job*>& js, const std::vector<BtJob>&
   p.host.assign(p.base_off + (size_t)BT_K_COUNT * (nj + 1) * 4, 0);
   BtJob* J = reinterpret_cast<BtJob*>(p.host.data());
   uint32_t* B = reinterpret_cast<uint32_t*>(p.host.data() + p.base_off);
-  for (uint32_t i = 0; i < nj; ++i) J[i] = entries[i];
+  for (uint32_t i = 0; i < nj; ++i) {
+    J[i] = entries[i];
+    p.samples += entries[i].n;
+  }
   for (int k = 0; k < BT_K_COUNT; ++k) {
     uint64_t acc = 0;
     for (uint32_t i = 0; i < nj; ++i) {
@@ -332,9 +352,15 @@ void Engine::queue_readback(Job& j, void* rec12, float* density, hipEvent_t done
                  j.rec_saved.as<float>(), nullptr, 0u);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(rec12, j.rec_saved.p, (size_t)j.n * 12, hipMemcpyDeviceToHost, stream_));
-  if (density && j.dens_norm.p)
+  size_t at = (size_t)j.n * 12;
+  if (density && j.dens_norm.p) {
     HIPCHK(hipMemcpyAsync(density, j.dens_norm.p, (size_t)j.dw * j.dh * 4, hipMemcpyDeviceToHost,
                           stream_));
+    at += (size_t)j.dw * j.dh * 4;
+  }
+  // the upload's status word (an asynchronous upload leaves it unchecked)
+  HIPCHK(hipMemcpyAsync(static_cast<unsigned char*>(rec12) + at, j.status.p, 4,
+                        hipMemcpyDeviceToHost, stream_));
   HIPCHK(hipEventRecord(done, stream_));
 }
 
@@ -379,7 +405,13 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   split.acc = reinterpret_cast<long long*>(split_.as<char>() + kSplitAccOff);
   split.wg_cand = reinterpret_cast<unsigned long long*>(split_.as<char>() + kSplitCandOff);
   split.rec = reinterpret_cast<uint32_t*>(split_.as<char>() + kSplitRecOff);
-  btk::neighbor_mean(stream_, (uint32_t)cus_ * nm_per_cu_, J, tasks_.as<BtTask>(), ctl, split);
+  unsigned long long* work = reinterpret_cast<unsigned long long*>(split_.as<char>() + kSplitWorkOff);
+  split.wg_steps = work;
+  split.wg_nbr = work + BT_MAX_WG;
+  split.wg_hsteps = work + 2 * BT_MAX_WG;
+  split.wg_hnbr = work + 3 * BT_MAX_WG;
+  btk::neighbor_mean(stream_, (uint32_t)cus_ * nm_per_cu_, J, tasks_.as<BtTask>(), ctl, split,
+                     p.samples);
   mark(6);
   if (density && p.grid[BT_K_ROW]) {
     btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj);
@@ -411,13 +443,33 @@ void Engine::upload(const std::vector<GraphIn>& gs, std::vector<uint32_t>* sampl
   }
 }
 
+void* Engine::upload_staging(size_t bytes, Pinned** slot) {
+  Pinned& s = up_pinned_[up_next_];
+  up_next_ = (up_next_ + 1u) % (uint32_t)up_pinned_.size();
+  if (s.pending) {  // its previous copy must be done before it is refilled
+    HIPCHK(hipEventSynchronize(s.ev));
+    s.pending = false;
+  }
+  if (s.bytes < bytes) {
+    if (s.p) HIPCHK(hipHostFree(s.p));
+    s.p = nullptr;
+    s.bytes = 0;
+    const size_t b = align_up(std::max<size_t>(bytes, 1 << 20), 1 << 20);
+    HIPCHK(hipHostMalloc(&s.p, b, hipHostMallocDefault));
+    s.bytes = b;
+  }
+  *slot = &s;
+  return s.p;
+}
+
 void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out,
-                         bool side, bool rewindable) {
-  hipStream_t const st = side ? side_ : stream_;
-  BtControl* const ctl = side ? ctl_side_.as<BtControl>() : ctl_.as<BtControl>();
+                         bool async, bool rewindable) {
+  hipStream_t const st = stream_;
+  BtControl* const ctl = ctl_.as<BtControl>();
   std::vector<std::unique_ptr<Job>> nj;
   std::vector<Job*> js;
   std::vector<std::vector<float>> wts;
+  std::vector<std::vector<uint64_t>> scans;
   for (const GraphIn& g : gs) {
     auto j = std::make_unique<Job>();
     j->set_pool(pool_.get());
@@ -457,166 +509,163 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
     j->dw = j->params[0].density_w;
     j->dh = j->params[0].density_h;
     if (j->dw < 1 || j->dh < 1) fail(BT_E_INVALID, "density map dimensions must be nonzero");
-    if (j->max_cells + 1ull > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS)
-      fail(BT_E_CAPACITY, "neighbor grid exceeds GPU dispatch limits");
     if (j->n_paths + 1ull > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS)
       fail(BT_E_CAPACITY, "paths exceed GPU dispatch limits");
-    j->path_scan.alloc((size_t)g.n_starts * 8);
+    // Count pass (count_paths, resampling.wgsl:138-159) on the host, with
+    // path offsets from a scan instead of atomics: the sample count is known
+    // before anything is launched, so no device round trip is needed.
+    std::vector<uint64_t> scan;
+    if (g.path_scan) {
+      scan.assign(g.path_scan, g.path_scan + g.n_starts);
+    } else {
+      scan.resize(g.n_starts);
+      resample_counts(g.xy, g.starts, j->n_paths, j->norm, j->params[0].res_x * 0.5f,
+                      j->params[0].res_y * 0.5f,
+                      (uint32_t)std::min<uint64_t>(j->capacity, 0xfffffffeull), scan.data());
+    }
+    const uint64_t n = scan[j->n_paths] >> 32, valid = scan[j->n_paths] & 0xffffffffull;
+    if (n > j->capacity)
+      fail(BT_E_CAPACITY, "GPU resampling exceeds the " + std::to_string(j->capacity) +
+                              "-sample buffer capacity; increase --max-graph-memory-mib or reduce "
+                              "the graph or render dimensions. Device buffer limits also apply.");
+    if (valid == 0 || n < 2) fail(BT_E_INVALID, "resampled graph has no drawable edges");
+    if (n > 0x7fffffffull) fail(BT_E_CAPACITY, "sample count exceeds 2^31");
+    if (n + 1 > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS) fail(BT_E_CAPACITY, "too many samples");
+    j->n = (uint32_t)n;
+    j->valid_paths = (uint32_t)valid;
+    if (j->o.cell_split == 0)  // no split given: rows sized for the sample density
+      for (uint32_t k = 0; k < j->sched; ++k)
+        if (j->params[k].radius > 0.0f)
+          j->set_split(k, density_split(j->params[k].radius, n, j->params[k].res_x,
+                                        j->params[k].res_y));
+    if (j->max_cells + 1ull > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS)
+      fail(BT_E_CAPACITY, "neighbor grid exceeds GPU dispatch limits");
     j->paths.alloc((size_t)std::max(j->n_paths, 1u) * 12);
     j->status.alloc(16);
-    const uint32_t nb = std::max(cdiv(j->n_paths + 1ull, BT_SCAN_ITEMS),
-                                 cdiv(j->max_cells + 1ull, BT_SCAN_ITEMS));
-    j->bsum.alloc((size_t)nb * 8);
+    // block sums: paths and cells (one per block), sample prefix sums (x, y per block)
+    const size_t nb = std::max<size_t>(
+        std::max<size_t>(cdiv(j->n_paths + 1ull, BT_SCAN_ITEMS), cdiv(j->max_cells + 1ull,
+                                                                      BT_SCAN_ITEMS)),
+        2 * (size_t)cdiv(n + 1, BT_SCAN_ITEMS));
+    j->bsum.alloc(nb * 8);
     j->wt.alloc(wt.size() * 4);
+    j->rec0.alloc((size_t)n * 12);
+    j->rec1.alloc((size_t)n * 12);
+    j->rec_saved.alloc((size_t)n * 12);
+    j->sorted.alloc((size_t)n * 16);
+    j->sid.alloc((size_t)n * 4);
+    j->cellrank.alloc((size_t)n * 8);
+    j->edge_of.alloc((size_t)n * 4);
+    j->cells.alloc(((size_t)j->max_cells + 1) * 4);
+    j->pre_x.alloc(((size_t)n + 1) * 8);
+    j->pre_y.alloc(((size_t)n + 1) * 8);
+    if (j->o.flags & BT_FLAG_DENSITY) {
+      const size_t pb = (size_t)j->dw * j->dh * 4;
+      j->dens_cnt.alloc(pb);
+      j->dens_row.alloc(pb);
+      j->dens_col.alloc(pb);
+      j->dens_norm.alloc(pb);
+      if (rewindable) j->dens_saved.alloc(pb * 4);
+      j->peak.alloc(8);
+    }
     wts.push_back(std::move(wt));
+    scans.push_back(std::move(scan));
     js.push_back(j.get());
     nj.push_back(std::move(j));
   }
   if (js.empty()) fail(BT_E_INVALID, "no graphs");
   // Every job's inputs in one copy through pinned memory. The upload area
   // holds a (weights offset, weights length) word pair per job, then each
-  // job's control points, path starts and blur weights, 64-word aligned; the
-  // set-up kernel moves the weights to the jobs' own buffers and clears
-  // their status words.
+  // job's control points, path starts, blur weights and path offsets (u64),
+  // 64-word aligned; the set-up kernel moves the weights to the jobs' own
+  // buffers and clears their status words.
   const size_t njs = js.size();
-  std::vector<size_t> off(3 * njs);
+  std::vector<size_t> off(4 * njs);
   size_t words = align_up(2 * njs, 64);
   for (size_t i = 0; i < njs; ++i) {
-    off[3 * i] = words;
+    off[4 * i] = words;
     words += align_up(2ull * gs[i].n_points, 64);
-    off[3 * i + 1] = words;
+    off[4 * i + 1] = words;
     words += align_up(gs[i].n_starts, 64);
-    off[3 * i + 2] = words;
+    off[4 * i + 2] = words;
     words += align_up(wts[i].size(), 64);
+    off[4 * i + 3] = words;
+    words += align_up(2ull * gs[i].n_starts, 64);
   }
   if (words > 0xffffffffull) fail(BT_E_CAPACITY, "upload exceeds 2^32 words");
-  if (up_host_bytes_ < words * 4) {
-    if (up_host_) HIPCHK(hipHostFree(up_host_));
-    up_host_ = nullptr;
-    up_host_bytes_ = 0;
-    const size_t b = align_up(std::max<size_t>(words * 4, 1 << 20), 1 << 20);
-    HIPCHK(hipHostMalloc(&up_host_, b, hipHostMallocDefault));
-    up_host_bytes_ = b;
-  }
-  uint32_t* host = static_cast<uint32_t*>(up_host_);
+  Pinned* slot = nullptr;
+  uint32_t* host = static_cast<uint32_t*>(upload_staging(words * 4, &slot));
   for (size_t i = 0; i < njs; ++i) {
-    host[2 * i] = (uint32_t)off[3 * i + 2];
+    host[2 * i] = (uint32_t)off[4 * i + 2];
     host[2 * i + 1] = (uint32_t)wts[i].size();
-    std::memcpy(host + off[3 * i], gs[i].xy, (size_t)gs[i].n_points * 8);
-    std::memcpy(host + off[3 * i + 1], gs[i].starts, (size_t)gs[i].n_starts * 4);
-    std::memcpy(host + off[3 * i + 2], wts[i].data(), wts[i].size() * 4);
+    std::memcpy(host + off[4 * i], gs[i].xy, (size_t)gs[i].n_points * 8);
+    std::memcpy(host + off[4 * i + 1], gs[i].starts, (size_t)gs[i].n_starts * 4);
+    std::memcpy(host + off[4 * i + 2], wts[i].data(), wts[i].size() * 4);
+    std::memcpy(host + off[4 * i + 3], scans[i].data(), scans[i].size() * 8);
   }
+  // Blocks of the pool are reused in stream order: the area, the plan and
+  // the status words go back to it on return while the launches below may
+  // still be queued, which is safe because every user is on this stream.
   DevBuf area;
   area.pool = pool_.get();
   area.alloc(words * 4);
   HIPCHK(hipMemcpyAsync(area.p, host, words * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(slot->ev, st));
+  slot->pending = true;
+  std::vector<BtJob> entries;
   for (size_t i = 0; i < njs; ++i) {
-    js[i]->in_ctrl = reinterpret_cast<const float*>(area.as<uint32_t>() + off[3 * i]);
-    js[i]->in_starts = area.as<uint32_t>() + off[3 * i + 1];
-  }
-  // per-job words read back by the host (totals, then status)
-  DevBuf jw;
-  jw.pool = pool_.get();
-  jw.alloc(njs * 8);
-  // Pass 1: per-path sample counts and their exclusive scan (count_paths,
-  // resampling.wgsl:138-159, with offsets from a scan instead of atomics).
-  std::vector<uint32_t> zeros(js.size(), 0);
-  Plan p1 = make_plan(js, zeros);
-  DevBuf table;
-  table.pool = pool_.get();
-  table.alloc(p1.host.size());
-  put_plan(p1, table.p, st);
-  {
-    const unsigned char* dev = table.as<unsigned char>();
-    const BtJob* J = reinterpret_cast<const BtJob*>(dev);
-    auto B = [&](int k) {
-      return reinterpret_cast<const uint32_t*>(dev + p1.base_off) + (size_t)k * (p1.njobs + 1);
-    };
-    btk::upload_setup(st, J, (int)p1.njobs, area.as<uint32_t>());
-    btk::resample_count(st, p1.grid[BT_K_PATH], J, B(BT_K_PATH), (int)p1.njobs);
-    btk::path_scan(st, p1.grid[BT_K_PSCAN], J, B(BT_K_PSCAN), (int)p1.njobs);
-    btk::job_words(st, J, (int)p1.njobs, 0u, jw.as<unsigned long long>());
-    HIPCHK(hipGetLastError());
-  }
-  std::vector<uint64_t> totals(njs);
-  HIPCHK(hipMemcpyAsync(totals.data(), jw.p, njs * 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  for (size_t i = 0; i < js.size(); ++i) {
     Job& j = *js[i];
-    const uint64_t n = totals[i] >> 32, valid = totals[i] & 0xffffffffull;
-    if (n > j.capacity)
-      fail(BT_E_CAPACITY, "GPU resampling exceeds the " + std::to_string(j.capacity) +
-                              "-sample buffer capacity; increase --max-graph-memory-mib or reduce "
-                              "the graph or render dimensions. Device buffer limits also apply.");
-    if (valid == 0 || n < 2) fail(BT_E_INVALID, "resampled graph has no drawable edges");
-    if (n > 0x7fffffffull) fail(BT_E_CAPACITY, "sample count exceeds 2^31");
-    j.n = (uint32_t)n;
-    j.valid_paths = (uint32_t)valid;
-    if (j.o.cell_split == 0)  // no split given: rows sized for the sample density
-      for (uint32_t e = 0; e < j.sched; ++e)
-        if (j.params[e].radius > 0.0f)
-          j.set_split(e, density_split(j.params[e].radius, n, j.params[e].res_x,
-                                       j.params[e].res_y));
-    j.rec0.alloc((size_t)n * 12);
-    j.rec1.alloc((size_t)n * 12);
-    j.rec_saved.alloc((size_t)n * 12);
-    j.sorted.alloc((size_t)n * 16);
-    j.sid.alloc((size_t)n * 4);
-    j.cellrank.alloc((size_t)n * 8);
-    j.edge_of.alloc((size_t)n * 4);
-    j.cells.alloc(((size_t)j.max_cells + 1) * 4);
-    j.pre_x.alloc(((size_t)n + 1) * 8);
-    j.pre_y.alloc(((size_t)n + 1) * 8);
-    if (n + 1 > (uint64_t)BT_SCAN_ITEMS * BT_SCAN_ITEMS) fail(BT_E_CAPACITY, "too many samples");
-    // block sums: cells (one per block) and sample prefix sums (x, y per block)
-    const size_t nb = std::max<size_t>(std::max<size_t>(j.bsum.bytes / 8, 2 * cdiv(n + 1, BT_SCAN_ITEMS)),
-                                       cdiv(j.max_cells + 1ull, BT_SCAN_ITEMS));
-    if (nb * 8 > j.bsum.bytes) j.bsum.alloc(nb * 8);
-    if (j.o.flags & BT_FLAG_DENSITY) {
-      const size_t pb = (size_t)j.dw * j.dh * 4;
-      j.dens_cnt.alloc(pb);
-      j.dens_row.alloc(pb);
-      j.dens_col.alloc(pb);
-      j.dens_norm.alloc(pb);
-      if (rewindable) j.dens_saved.alloc(pb * 4);
-      j.peak.alloc(8);
-    }
+    j.in_ctrl = reinterpret_cast<const float*>(area.as<uint32_t>() + off[4 * i]);
+    j.in_starts = area.as<uint32_t>() + off[4 * i + 1];
+    BtJob d = j.dev(0, 0, true);
+    d.path_scan = reinterpret_cast<uint64_t*>(area.as<uint32_t>() + off[4 * i + 3]);
+    entries.push_back(d);
+    j.in_ctrl = nullptr;  // the upload area is released on return
+    j.in_starts = nullptr;
   }
-  // Pass 2: emit samples (emit_paths, resampling.wgsl:161-183) with the
-  // initial density splat, then the density chain (gpu.rs:512-514).
-  Plan p2 = make_plan(js, zeros);
-  put_plan(p2, table.p, st);
+  // Emit the samples (emit_paths, resampling.wgsl:161-183) with the initial
+  // density splat, then the density chain (gpu.rs:512-514).
+  const Plan p = make_plan_of(js, entries);
+  DevBuf table, jw;
+  table.pool = pool_.get();
+  jw.pool = pool_.get();
+  table.alloc(p.host.size());
+  put_plan(p, table.p, st);
   {
     const unsigned char* dev = table.as<unsigned char>();
     const BtJob* J = reinterpret_cast<const BtJob*>(dev);
-    const uint32_t n2 = p2.njobs;
+    const uint32_t n2 = p.njobs;
     auto B = [&](int k) {
-      return reinterpret_cast<const uint32_t*>(dev + p2.base_off) + (size_t)k * (n2 + 1);
+      return reinterpret_cast<const uint32_t*>(dev + p.base_off) + (size_t)k * (n2 + 1);
     };
-    btk::clear(st, p2.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)n2, ctl, 1);
-    btk::resample_emit(st, p2.grid[BT_K_EMIT], J, B(BT_K_EMIT), (int)n2);
-    if (p2.grid[BT_K_ROW]) {
-      btk::blur_rows(st, p2.grid[BT_K_ROW], J, B(BT_K_ROW), (int)n2);
-      btk::blur_cols(st, p2.grid[BT_K_COL], J, B(BT_K_COL), (int)n2);
-      btk::normalize(st, p2.grid[BT_K_NORM], J, B(BT_K_NORM), (int)n2);
+    btk::upload_setup(st, J, (int)n2, area.as<uint32_t>());
+    btk::clear(st, p.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)n2, ctl, 1);
+    btk::resample_emit(st, p.grid[BT_K_EMIT], J, B(BT_K_EMIT), (int)n2);
+    if (p.grid[BT_K_ROW]) {
+      btk::blur_rows(st, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)n2);
+      btk::blur_cols(st, p.grid[BT_K_COL], J, B(BT_K_COL), (int)n2);
+      btk::normalize(st, p.grid[BT_K_NORM], J, B(BT_K_NORM), (int)n2);
     }
-    btk::job_words(st, J, (int)n2, 1u, jw.as<unsigned long long>());
     // the state after upload: rewinds return to it (the served form stages
     // its results in rec_saved instead)
-    btk::save_points(st, p2.grid[BT_K_BIN], J, B(BT_K_BIN), (int)n2, 0);
-    if (rewindable && p2.grid[BT_K_NORM])
-      btk::save_density(st, p2.grid[BT_K_NORM], J, B(BT_K_NORM), (int)n2, 0);
+    btk::save_points(st, p.grid[BT_K_BIN], J, B(BT_K_BIN), (int)n2, 0);
+    if (rewindable && p.grid[BT_K_NORM])
+      btk::save_density(st, p.grid[BT_K_NORM], J, B(BT_K_NORM), (int)n2, 0);
+    if (!async) {
+      jw.alloc(njs * 8);
+      btk::job_words(st, J, (int)n2, jw.as<unsigned long long>());
+    }
     HIPCHK(hipGetLastError());
   }
-  std::vector<uint64_t> status(njs);
-  HIPCHK(hipMemcpyAsync(status.data(), jw.p, njs * 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  for (Job* j : js) {
-    j->in_ctrl = nullptr;  // the upload area is released on return
-    j->in_starts = nullptr;
+  if (!async) {
+    // the emit pass must agree with the counts (resampling.rs:296-299); the
+    // asynchronous form leaves the status word for its owner to check
+    std::vector<uint64_t> status(njs);
+    HIPCHK(hipMemcpyAsync(status.data(), jw.p, njs * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (size_t i = 0; i < njs; ++i)
+      if (status[i] & 2u) fail(BT_E_MISMATCH, "GPU resampling count and write passes disagree");
   }
-  for (size_t i = 0; i < njs; ++i)
-    if (status[i] & 2u) fail(BT_E_MISMATCH, "GPU resampling count and write passes disagree");
   for (auto& j : nj) {
     j->iteration = 0;
     j->parity = 0;

@@ -69,6 +69,7 @@ struct Plan {
   std::vector<unsigned char> host;
   uint32_t grid[BT_K_COUNT] = {};
   uint32_t njobs = 0;
+  uint64_t samples = 0;  // samples of the plan's jobs
   size_t base_off = 0;  // byte offset of the bases block
 };
 
@@ -79,6 +80,8 @@ struct GraphIn {
   const uint32_t* starts;
   uint32_t n_starts;
   bt_options opts;
+  // the count pass's path offsets (resample_counts), if already computed
+  const uint64_t* path_scan = nullptr;
 };
 
 class Engine {
@@ -89,19 +92,23 @@ class Engine {
   const bt_options& options() const { return opts_; }
   // Replaces the current jobs with the given graphs; returns sample counts.
   void upload(const std::vector<GraphIn>& gs, std::vector<uint32_t>* samples);
-  // Uploads graphs as new jobs appended to `out`. `side`: on the engine's
-  // second stream (with its own control block), so the upload overlaps
-  // iterations queued on the main stream (the server's admissions).
-  // `rewindable`: keep the post-upload copies bt_rewind restores.
+  // Uploads graphs as new jobs appended to `out`: sample counts on the host,
+  // then one copy and one kernel pass on the engine's stream. `async`:
+  // return once the launches are queued (behind any queued iterations; the
+  // server's admissions), leaving the emit status word unchecked for the
+  // caller (Job::status); otherwise wait and check it. `rewindable`: keep
+  // the post-upload copies bt_rewind restores.
   void upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out,
-                   bool side = false, bool rewindable = true);
+                   bool async = false, bool rewindable = true);
   void iterate(uint32_t n, bool use_graph);
   // Advances an explicit job set by one iteration (server tick); with
   // wait = false it returns once the launches are queued.
   void iterate_jobs(const std::vector<Job*>& js, bool wait = true);
   // Queues the readback of a job's points (12-byte records, Layout order)
   // and, if density is on, its normalized density into host buffers (pinned,
-  // so the copies run asynchronously), then records `done` on the stream.
+  // so the copies run asynchronously), then the upload's status word (4
+  // bytes) right after the records and the density at rec12, and records
+  // `done` on the stream. `density` must be rec12 + 12 n when given.
   // Uses the job's rewind copy as staging: the job must not be rewound after.
   void queue_readback(Job& j, void* rec12, float* density, hipEvent_t done);
   void rewind();
@@ -134,6 +141,10 @@ class Engine {
 
   // Heavy tasks split by the last iteration run, and their parts.
   void split_counts(uint32_t* tasks, uint32_t* parts);
+  // Executed work of the last neighbor-mean launch: wave-steps (one
+  // candidate tested against the 64 lanes' samples), neighbors found for
+  // the movable samples, and the candidates of the tasks' streams.
+  void work_counts(uint64_t* steps, uint64_t* neighbors, uint64_t* candidates);
 
   size_t njobs() const { return jobs_.size(); }
   Job& job(uint32_t i);
@@ -165,9 +176,8 @@ class Engine {
   int cus_ = 256;
   uint32_t nm_per_cu_ = 4;  // resident neighbor-mean workgroups per CU
   hipStream_t stream_ = nullptr;
-  hipStream_t side_ = nullptr;  // uploads that overlap queued iterations
   std::vector<std::unique_ptr<Job>> jobs_;
-  DevBuf ctl_, ctl_side_, tasks_, plans_, graph_plans_, staging_, staging2_, split_, retire_plans_;
+  DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_, retire_plans_;
   struct Pinned {
     void* p = nullptr;
     size_t bytes = 0;
@@ -175,8 +185,11 @@ class Engine {
     bool pending = false;
   };
   std::vector<Pinned> pinned_;
-  void* up_host_ = nullptr;  // pinned staging of the upload inputs
-  size_t up_host_bytes_ = 0;
+  // pinned staging of upload inputs: a ring, so an asynchronous upload never
+  // refills a buffer whose copy is still queued
+  std::vector<Pinned> up_pinned_;
+  uint32_t up_next_ = 0;
+  void* upload_staging(size_t bytes, Pinned** slot);
   uint32_t pinned_next_ = 0;
   uint32_t retire_next_ = 0;
   size_t slot_bytes_ = 0;

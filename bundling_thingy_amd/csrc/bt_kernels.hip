@@ -1,7 +1,8 @@
 // Hand-written gfx950 kernels for the edge-bundling hot path.
 //
 // Reference (lpstankus/bundling_thingy) kernels replaced here, per stage:
-//   resample_count / resample_emit  <- resampling.wgsl:138-183 (count_paths, emit_paths)
+//   resample_emit                    <- resampling.wgsl:161-183 (emit_paths; the count pass,
+//                                      count_paths :138-159, runs on the host: resample_counts)
 //   bin / cell scan / tasks / scatter <- spatial_grid.wgsl:72-138 (count_cells,
 //                                      reserve_cells, scatter_points, prepare_dispatch)
 //   neighbor_mean                    <- neighbor_mean.wgsl:75-203 (tiled + sparse)
@@ -759,7 +760,7 @@ __device__ __forceinline__ float4 nm_candidate(const float4* S, const uint32_t* 
 // group 0's LDS ranges; adds the lane's accepted sums to (ax, ay, n).
 __device__ __forceinline__ void nm_stream(const NmTask& T, NmLds& L, uint32_t m, uint32_t nranges,
                                           uint32_t first, uint32_t step, long long& ax,
-                                          long long& ay, long long& n) {
+                                          long long& ay, long long& n, uint64_t& steps) {
   const uint32_t lane = LANE(), w = threadIdx.x >> 6;
   const BtJob& J = *T.J;
   const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2, px = T.px, py = T.py;
@@ -781,6 +782,7 @@ __device__ __forceinline__ void nm_stream(const NmTask& T, NmLds& L, uint32_t m,
     if (ch + step < nch) nxt = fetch(ch + step);
     __builtin_amdgcn_wave_barrier();
     const uint32_t cnt = min(64u, m - (ch << 6));
+    steps += cnt;
     int sx = 0, sy = 0, sc = 0;
     uint32_t k = 0;
     for (; k + 4u <= cnt; k += 4u) {
@@ -867,7 +869,7 @@ __device__ __forceinline__ NmGroups nm_group_rows(const NmTask& T, NmLds& L) {
 // adds the lane's accepted sums to (ax, ay, n).
 __device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds& L, const NmGroups& G,
                                                 uint32_t first, uint32_t step, long long& ax,
-                                                long long& ay, long long& n) {
+                                                long long& ay, long long& n, uint64_t& steps) {
   const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / NM_GLANES;
   const BtJob& J = *T.J;
   const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2, px = T.px, py = T.py;
@@ -899,6 +901,7 @@ __device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds& L, const
       for (uint32_t i = 0; i < NM_GCHUNK / NM_GLANES; ++i) nxt[i] = fetch(ch + step, i);
     }
     const uint32_t cnt = min((uint32_t)NM_GCHUNK, G.mmax - ch * (uint32_t)NM_GCHUNK);
+    steps += cnt;
     int sx = 0, sy = 0, sc = 0;
     uint32_t k = 0;
     for (; k + 4u <= cnt; k += 4u) {
@@ -968,10 +971,12 @@ __device__ __forceinline__ NmGroups nm_load_groups(NmLds& L, const uint32_t* rec
 
 // Mean of a movable sample from its exact sums, output record, density splat
 // (one atomic per distinct pixel of the wave).
-__device__ __forceinline__ void nm_finish(const NmTask& T, long long sx, long long sy,
-                                          long long tn) {
+__device__ __forceinline__ uint64_t nm_finish(const NmTask& T, long long sx, long long sy,
+                                              long long tn) {
   const uint32_t lane = LANE();
   const BtJob& J = *T.J;
+  // neighbors of the wave's movable samples (executed-work report)
+  const uint64_t nbr = wave_sum_u64(T.movable ? (unsigned long long)tn : 0ull);
   float mx = T.px, my = T.py;
   if (T.movable) {
     const long long dxs = sx - tn * (long long)T.pfx, dys = sy - tn * (long long)T.pfy;
@@ -996,25 +1001,51 @@ __device__ __forceinline__ void nm_finish(const NmTask& T, long long sx, long lo
       act &= ~same;
     }
   }
+  return nbr;
+}
+
+// Per-workgroup totals of a launch (plain stores at exit, summed on the
+// host): wave-steps executed (one candidate tested against each of the 64
+// lanes' samples) and neighbors found.
+__device__ __forceinline__ void nm_store_work(unsigned long long* steps_out,
+                                              unsigned long long* nbr_out, uint64_t steps,
+                                              uint64_t nbr, unsigned long long (*red)[2]) {
+  if (LANE() == 0) {
+    red[threadIdx.x >> 6][0] = steps;
+    red[threadIdx.x >> 6][1] = nbr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && blockIdx.x < BT_MAX_WG) {
+    unsigned long long a = 0, b = 0;
+    for (int w = 0; w < NM_WAVES; ++w) {
+      a += red[w][0];
+      b += red[w][1];
+    }
+    steps_out[blockIdx.x] = a;
+    nbr_out[blockIdx.x] = b;
+  }
 }
 
 extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
-    BtSplit split) {
+    BtSplit split, unsigned long long samples) {
   __shared__ NmLds L;
   __shared__ unsigned long long wave_cand[NM_WAVES];
   const uint32_t lane = LANE(), w = threadIdx.x >> 6;
   const uint32_t total = uni(__atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
   // split threshold from the previous launch's candidates per wave
-  const unsigned long long prev = ctl->cand_prev;
+  const unsigned long long prev = ctl->cand_prev, sprev = ctl->samples_prev;
   uint32_t heavy = BT_HEAVY_HI;
   if (prev) {
-    const unsigned long long per_wave = prev / ((unsigned long long)gridDim.x * NM_WAVES);
+    const double expect = sprev ? (double)prev * ((double)samples / (double)sprev) : (double)prev;
+    const unsigned long long per_wave =
+        (unsigned long long)(expect / ((double)gridDim.x * NM_WAVES));
     heavy = (uint32_t)min(max(per_wave / 2ull, (unsigned long long)BT_HEAVY_LO),
                           (unsigned long long)BT_HEAVY_HI);
   }
   const uint32_t part_steps = max(heavy / 4u, BT_PART_STEPS_MIN);
   unsigned long long cand = 0;
+  uint64_t steps = 0, nbr = 0;
   uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
   for (;;) {
     const uint32_t t = uni(claim_work(ctl->xq, total, part, tried));
@@ -1068,17 +1099,17 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
           ax += G.sx;
           ay += G.sy;
           n += G.sn;
-          nm_group_stream(T, L, G, 0u, 1u, ax, ay, n);
+          nm_group_stream(T, L, G, 0u, 1u, ax, ay, n, steps);
           break;
         }
         ux += sx;
         uy += sy;
         un += sn;
-        nm_stream(T, L, m, nranges, 0u, 1u, ax, ay, n);
+        nm_stream(T, L, m, nranges, 0u, 1u, ax, ay, n, steps);
       }
       if (deferred) continue;
     }
-    nm_finish(T, ax + ux, ay + uy, n + un);
+    nbr += nm_finish(T, ax + ux, ay + uy, n + un);
   }
   // this workgroup's candidates (every wave leaves the loop above)
   if (lane == 0) wave_cand[threadIdx.x >> 6] = cand;
@@ -1088,6 +1119,8 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
     for (int w = 0; w < NM_WAVES; ++w) s += wave_cand[w];
     split.wg_cand[blockIdx.x] = s;
   }
+  __shared__ unsigned long long work[NM_WAVES][2];
+  nm_store_work(split.wg_steps, split.wg_nbr, steps, nbr, work);
 }
 
 // Runs the parts of heavy tasks: part p tests group chunks p, p + P, p + 2P,
@@ -1099,7 +1132,7 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
 // into the control block, for the next launch's split threshold.
 extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
-    BtSplit split) {
+    BtSplit split, unsigned long long samples) {
   __shared__ NmLds L;
   const uint32_t lane = LANE();
   const uint32_t items = min(uni(__atomic_load_n(&ctl->item_count, __ATOMIC_RELAXED)),
@@ -1112,9 +1145,13 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
     s = wave_sum_u64(s);
     if (lane == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) ctl->cand_prev = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) {
+      ctl->cand_prev = red[0] + red[1] + red[2] + red[3];
+      ctl->samples_prev = samples;
+    }
   }
   uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
+  uint64_t steps = 0, nbr = 0;
   for (;;) {
     const uint32_t i = uni(claim_work(ctl->iq, items, part, tried));
     if (i >= items) break;
@@ -1132,7 +1169,7 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
     ux = G.sx;
     uy = G.sy;
     un = G.sn;
-    nm_group_stream(T, L, G, p, parts, ax, ay, n);
+    nm_group_stream(T, L, G, p, parts, ax, ay, n, steps);
     if (p == 0u) {
       ax += ux;
       ay += uy;
@@ -1154,9 +1191,11 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const long long sn = (long long)__hip_atomic_load((unsigned long long*)&acc[128u + lane],
                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      nm_finish(T, sx, sy, sn);
+      nbr += nm_finish(T, sx, sy, sn);
     }
   }
+  __shared__ unsigned long long work[NM_WAVES][2];
+  nm_store_work(split.wg_hsteps, split.wg_hnbr, steps, nbr, work);
 }
 
 // ------------------------------------------------------------------- blur
@@ -1315,96 +1354,6 @@ __device__ __forceinline__ uint32_t steps_of(float distance, float remaining) {
   return f >= 4294967296.0f ? 0xffffffffu : (uint32_t)f;
 }
 
-// resampling.wgsl:82-159 count mode: one thread per path, closed-form step
-// count per segment. Writes (count << 32) | 1 for paths with >= 2 samples,
-// else 0; entry n_paths is zeroed so the scan yields totals.
-extern "C" __global__ __launch_bounds__(256) void bt_resample_count(
-    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
-  JOB_PROLOGUE(BT_K_PATH);
-  const uint32_t i = lb * 256u + threadIdx.x;
-  if (i > J.n_paths) return;
-  if (i == J.n_paths) {
-    J.path_scan[i] = 0ull;
-    return;
-  }
-  const uint32_t cap = J.capacity;
-  const uint32_t s = J.starts[i], e = J.starts[i + 1u];
-  float fx, fy;
-  normalized(J, s, fx, fy);
-  const float tsx = J.ts_x, tsy = J.ts_y;
-  uint32_t count = 0;
-  float lx = fx, ly = fy;
-  // push(first)
-  if (count >= cap) count = cap + 1u; else { lx = fx; ly = fy; count = 1u; }
-  float px = fx, py = fy, remaining = 1.0f;
-  bool over = count > cap;
-  for (uint32_t k = s + 1u; k < e && !over; ++k) {
-    float cx, cy;
-    normalized(J, k, cx, cy);
-    const float dx = cx - px, dy = cy - py;
-    const float ex = dx * tsx, ey = dy * tsy;
-    const float distance = sqrtf(fmaf(ex, ex, ey * ey));
-    if (distance > 0.0f) {
-      const uint32_t steps = steps_of(distance, remaining);
-      if (steps > 0u) {
-        const float t0 = remaining / distance;
-        const float t1 = (remaining + (float)(steps - 1u)) / distance;
-        const float sx = px + dx * t0, sy = py + dy * t0;
-        const float ux = px + dx * t1, uy = py + dy * t1;
-        count += steps - ((sx == lx && sy == ly) ? 1u : 0u);
-        lx = ux;
-        ly = uy;
-        if (count > cap) { count = cap + 1u; over = true; }
-      }
-      remaining = remaining + (float)steps - distance;
-    }
-    px = cx;
-    py = cy;
-  }
-  if (!over) {
-    // push(previous)
-    if (!(count > 0u && px == lx && py == ly)) {
-      if (count >= cap) count = cap + 1u; else { lx = px; ly = py; ++count; }
-    }
-    if (count == 1u) {
-      for (uint32_t k = s + 1u; k < e; ++k) {
-        float nx, ny;
-        normalized(J, k, nx, ny);
-        if (nx != fx || ny != fy) {
-          if (!(nx == lx && ny == ly)) {
-            if (count >= cap) count = cap + 1u; else { lx = nx; ly = ny; ++count; }
-          }
-          if (!(count > 0u && px == lx && py == ly)) {
-            if (count >= cap) count = cap + 1u; else { lx = px; ly = py; ++count; }
-          }
-          break;
-        }
-      }
-    }
-  }
-  J.path_scan[i] = count < 2u ? 0ull : (((unsigned long long)count << 32) | 1ull);
-}
-
-extern "C" __global__ __launch_bounds__(256) void bt_path_scan_reduce(
-    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
-  JOB_PROLOGUE(BT_K_PSCAN);
-  scan_reduce_body<unsigned long long>((const unsigned long long*)J.path_scan, J.n_paths + 1u, lb,
-                                       (unsigned long long*)J.bsum);
-}
-
-extern "C" __global__ __launch_bounds__(256) void bt_path_scan_bsum(const BtJob* __restrict__ jobs,
-                                                                    int njobs) {
-  const BtJob& J = jobs[blockIdx.x];
-  const uint32_t nb = (J.n_paths + 1u + BT_SCAN_ITEMS - 1u) / BT_SCAN_ITEMS;
-  scan_bsum_body<unsigned long long>((unsigned long long*)J.bsum, nb);
-}
-
-extern "C" __global__ __launch_bounds__(256) void bt_path_scan_apply(
-    const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
-  JOB_PROLOGUE(BT_K_PSCAN);
-  scan_apply_body<unsigned long long>((unsigned long long*)J.path_scan, J.n_paths + 1u, lb,
-                                      (const unsigned long long*)J.bsum);
-}
 
 // resampling.wgsl:50-136 write mode + emit_paths (:161-183): one wave per
 // path. Within a segment the samples are independent (sample k sits at
@@ -1587,16 +1536,13 @@ extern "C" __global__ __launch_bounds__(256) void bt_upload_setup(const BtJob* _
   if (threadIdx.x < 4u) J.status[threadIdx.x] = 0u;
 }
 
-// One 64-bit word per job for the host: the path scan's total (which = 0:
-// samples << 32 | drawable paths) or the first status word (which = 1).
+// One 64-bit word per job for the host: the job's first status word.
 extern "C" __global__ __launch_bounds__(256) void bt_job_words(const BtJob* __restrict__ jobs,
-                                                               int njobs, uint32_t which,
+                                                               int njobs,
                                                                unsigned long long* __restrict__ out) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= njobs) return;
-  const BtJob& J = jobs[j];
-  out[j] = which == 0u ? (unsigned long long)J.path_scan[J.n_paths]
-                       : (unsigned long long)J.status[0];
+  out[j] = (unsigned long long)jobs[j].status[0];
 }
 
 // The samples after upload: rec[0] -> rec_saved (restore = 0) or back.
@@ -1736,9 +1682,9 @@ uint32_t neighbor_mean_blocks_per_cu() {
   return (uint32_t)std::max(1, std::min(a, b));
 }
 void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
-                   BtControl* ctl, BtSplit split) {
-  hipLaunchKernelGGL(bt_neighbor_mean, L256(grid), jobs, tasks_in, ctl, split);
-  hipLaunchKernelGGL(bt_neighbor_mean_heavy, L256(grid), jobs, tasks_in, ctl, split);
+                   BtControl* ctl, BtSplit split, unsigned long long samples) {
+  hipLaunchKernelGGL(bt_neighbor_mean, L256(grid), jobs, tasks_in, ctl, split, samples);
+  hipLaunchKernelGGL(bt_neighbor_mean_heavy, L256(grid), jobs, tasks_in, ctl, split, samples);
 }
 void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_blur_rows, L256(grid), jobs, base, nj);
@@ -1755,14 +1701,6 @@ void splat_points(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_
 void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_normalize, L256(grid), jobs, base, nj);
 }
-void resample_count(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
-  hipLaunchKernelGGL(bt_resample_count, L256(grid), jobs, base, nj);
-}
-void path_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
-  hipLaunchKernelGGL(bt_path_scan_reduce, L256(grid), jobs, base, nj);
-  hipLaunchKernelGGL(bt_path_scan_bsum, L256((uint32_t)nj), jobs, nj);
-  hipLaunchKernelGGL(bt_path_scan_apply, L256(grid), jobs, base, nj);
-}
 void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_resample_emit, L256(grid), jobs, base, nj);
 }
@@ -1777,10 +1715,9 @@ void half_plane(hipStream_t s, const float* in, uint16_t* out, uint32_t n) {
 void upload_setup(hipStream_t s, const BtJob* jobs, int nj, const uint32_t* area) {
   hipLaunchKernelGGL(bt_upload_setup, dim3((unsigned)nj), dim3(256), 0, s, jobs, nj, area);
 }
-void job_words(hipStream_t s, const BtJob* jobs, int nj, uint32_t which,
-               unsigned long long* out) {
+void job_words(hipStream_t s, const BtJob* jobs, int nj, unsigned long long* out) {
   hipLaunchKernelGGL(bt_job_words, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, s, jobs, nj,
-                     which, out);
+                     out);
 }
 void save_points(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                  int restore) {

@@ -13,8 +13,9 @@ void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base
            BtControl* ctl, BtTask* tasks_out);
 void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void pre_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+// `samples`: the launch's samples, summed over its jobs.
 void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
-                   BtControl* ctl, BtSplit split);
+                   BtControl* ctl, BtSplit split, unsigned long long samples);
 // Workgroups of the (persistent) neighbor-mean kernels resident per CU.
 uint32_t neighbor_mean_blocks_per_cu();
 void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
@@ -23,8 +24,6 @@ void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
 // Density of the current records outside an iteration: clear, then splat.
 void density_clear(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void splat_points(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
-void resample_count(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
-void path_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void resample_emit(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 // Current positions in original sample order: 12-byte records {x, y, edge id}
 // (split = 0) or n positions (x, y) followed by n edge ids (split = 1).
@@ -34,8 +33,8 @@ void unpermute(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
 void half_plane(hipStream_t s, const float* in, uint16_t* out, uint32_t n);
 // Upload set-up (weights from the packed area, status cleared), one workgroup per job.
 void upload_setup(hipStream_t s, const BtJob* jobs, int nj, const uint32_t* area);
-// One 64-bit word per job: path-scan total (which = 0) or status word 0 (which = 1).
-void job_words(hipStream_t s, const BtJob* jobs, int nj, uint32_t which, unsigned long long* out);
+// One 64-bit word per job: its status word 0.
+void job_words(hipStream_t s, const BtJob* jobs, int nj, unsigned long long* out);
 // Saved state for rewinds (restore = 1 copies it back).
 void save_points(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                  int restore);

@@ -93,6 +93,10 @@ typedef struct bt_info {
   uint32_t split_tasks; /* neighbor-mean tasks of the last iteration run that were
                            split across waves (long candidate streams) */
   uint32_t split_parts; /* their parts */
+  /* executed work of that iteration's neighbor mean (all jobs): */
+  uint64_t test_steps;  /* wave-steps: one candidate tested against a wave's 64 samples */
+  uint64_t neighbors;   /* neighbors found for the movable samples, |N(p)| summed */
+  uint64_t candidates;  /* candidates in the tasks' union streams (before lane groups) */
 } bt_info;
 
 typedef struct bt_ctx bt_ctx;
@@ -361,6 +365,11 @@ int bt_server_submit(bt_server* srv, const float* xy, uint32_t n_points, const u
 int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued);
 
 int bt_server_poll(bt_server* srv, uint64_t ticket, bt_job_result* out);
+/* Tickets of the jobs that finished (done or failed) since the last call,
+ * in completion order: up to `capacity` of them, *n receives how many. Picks
+ * up completed retirements first, like bt_server_poll. A serving loop calls
+ * this instead of polling every pending ticket. */
+int bt_server_take_finished(bt_server* srv, uint64_t* tickets, size_t capacity, size_t* n);
 /* A finished job's results: its samples after the last iteration, and the
  * normalized density of that iteration (the only density a job computes on
  * the server; bits equal to running the job alone through bt_iterate). */

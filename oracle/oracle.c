@@ -499,6 +499,97 @@ static void neighbor_mean_impl(const float* in, const uint8_t* fixed, uint32_t n
     free_sorted(&s);
 }
 
+/* neighbor_mean.wgsl:40-59 (classify_cell, cell_displacement) with
+ * spatial_grid.wgsl:161-220 (summarize_cells): the reference's fully-inside
+ * shortcut over per-cell summaries, in exact sums. */
+void or_neighbor_mean_cells(const float* in, const uint8_t* fixed, uint32_t n,
+                            const or_grid* g, uint32_t stride, float* out, int threads) {
+    or_derived d;
+    or_derive(g, &d);
+    sorted_t s;
+    sort_points(in, n, g, &s);
+    const uint32_t gx = g->gx;
+    const size_t cells = (size_t)g->gx * g->gy;
+    int64_t* sum = (int64_t*)calloc(2 * cells, sizeof(int64_t));
+    float* box = (float*)malloc(4 * cells * sizeof(float));
+    for (size_t c = 0; c < cells; c++) {
+        float x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+        int64_t sx = 0, sy = 0;
+        for (uint32_t j = s.start[c]; j < s.start[c + 1]; j++) {
+            x0 = fminf(x0, s.x[j]);
+            y0 = fminf(y0, s.y[j]);
+            x1 = fmaxf(x1, s.x[j]);
+            y1 = fmaxf(y1, s.y[j]);
+            sx += s.fx[j];
+            sy += s.fy[j];
+        }
+        sum[2 * c] = sx;
+        sum[2 * c + 1] = sy;
+        box[4 * c] = x0;
+        box[4 * c + 1] = y0;
+        box[4 * c + 2] = x1;
+        box[4 * c + 3] = y1;
+    }
+    const double r2 = (double)g->radius * (double)g->radius;
+    const double tsx = d.ts_x, tsy = d.ts_y, padx = d.pad_x, pady = d.pad_y;
+    if (stride == 0) stride = 1;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+#endif
+    for (uint32_t i = 0; i < n; i++) {
+        float px = in[2 * i], py = in[2 * i + 1];
+        if (fixed[i] || i % stride != 0) {
+            out[2 * i] = px;
+            out[2 * i + 1] = py;
+            continue;
+        }
+        float tx = texel(px, g->res_x), ty = texel(py, g->res_y);
+        uint32_t x0 = cell_axis(tx - d.reach_x, bin_w(g), g->gx);
+        uint32_t x1 = cell_axis(tx + d.reach_x, bin_w(g), g->gx);
+        uint32_t y0 = cell_axis(ty - d.reach_y, g->cell, g->gy);
+        uint32_t y1 = cell_axis(ty + d.reach_y, g->cell, g->gy);
+        int64_t ax = 0, ay = 0, cnt = 0;
+        for (uint32_t cy = y0; cy <= y1; cy++) {
+            for (uint32_t cx = x0; cx <= x1; cx++) {
+                const size_t c = (size_t)cy * gx + cx;
+                const uint32_t b = s.start[c], e = s.start[c + 1];
+                if (b == e) continue;
+                const float* bx = box + 4 * c;
+                const double nx = fmax(fmax((double)bx[0] - px, px - (double)bx[2]), 0.0);
+                const double ny = fmax(fmax((double)bx[1] - py, py - (double)bx[3]), 0.0);
+                const double ntx = fmax(nx * tsx - padx, 0.0), nty = fmax(ny * tsy - pady, 0.0);
+                if (ntx * ntx + nty * nty > r2) continue;  /* outside */
+                const double fx = fmax(fabs((double)bx[0] - px), fabs((double)bx[2] - px));
+                const double fy = fmax(fabs((double)bx[1] - py), fabs((double)bx[3] - py));
+                const double ftx = fx * tsx + padx, fty = fy * tsy + pady;
+                if (ftx * ftx + fty * fty < r2) {  /* fully inside: the summary */
+                    ax += sum[2 * c];
+                    ay += sum[2 * c + 1];
+                    cnt += e - b;
+                    continue;
+                }
+                for (uint32_t j = b; j < e; j++) {
+                    float ddx = (s.x[j] - px) * d.ts_x;
+                    float ddy = (s.y[j] - py) * d.ts_y;
+                    int ok = fmaf(ddx, ddx, ddy * ddy) <= d.r2;
+                    ax += ok ? s.fx[j] : 0;
+                    ay += ok ? s.fy[j] : 0;
+                    cnt += ok;
+                }
+            }
+        }
+        int64_t pfx = (int64_t)rintf(px * FIX_ONE), pfy = (int64_t)rintf(py * FIX_ONE);
+        int64_t dx = ax - cnt * pfx, dy = ay - cnt * pfy;
+        double den = (double)cnt * 16777216.0;
+        out[2 * i] = (float)((double)px + (double)dx / den);
+        out[2 * i + 1] = (float)((double)py + (double)dy / den);
+    }
+    free(sum);
+    free(box);
+    free_sorted(&s);
+}
+
 void or_neighbor_mean_f32(const float* in, const uint8_t* fixed, uint32_t n,
                           const or_grid* g, float* out) {
     or_derived d;

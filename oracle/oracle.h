@@ -105,6 +105,19 @@ void or_neighbor_mean(const float* in, const uint8_t* fixed, uint32_t n,
 void or_neighbor_mean_strided(const float* in, const uint8_t* fixed, uint32_t n,
                               const or_grid* g, uint32_t stride, float* out, int threads);
 
+/* The same means through the reference's tiled algorithm
+ * (neighbor_mean.wgsl:40-59,75-161): per grid cell a summary (sample count,
+ * coordinate sums, bounding box of its samples); a cell whose box lies
+ * entirely within the radius (with the reference's padding of 2.4e-7 texels
+ * per texel of map size added to its farthest corner) contributes its
+ * summary, a cell entirely outside nothing, and the samples of the others
+ * are tested one by one. Sums are the exact fixed-point sums, so the result
+ * equals or_neighbor_mean's bit for bit; the classification is evaluated in
+ * double precision. Computes every stride-th sample (the others are copied)
+ * for the timed CPU baseline. */
+void or_neighbor_mean_cells(const float* in, const uint8_t* fixed, uint32_t n,
+                            const or_grid* g, uint32_t stride, float* out, int threads);
+
 /* The reference's literal sparse form (neighbor_mean.wgsl:163-203): f32
  * sequential sums over the cells in [texel - reach, texel + reach] with
  * reach = r + pad, points visited in index order within each cell. Used only

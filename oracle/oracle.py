@@ -57,6 +57,8 @@ def lib():
             "or_neighbor_mean_strided": (None, [_PF, _PU8, C.c_uint32, C.POINTER(OrGrid),
                                                 C.c_uint32, _PF, C.c_int]),
             "or_neighbor_mean_f32": (None, [_PF, _PU8, C.c_uint32, C.POINTER(OrGrid), _PF]),
+            "or_neighbor_mean_cells": (None, [_PF, _PU8, C.c_uint32, C.POINTER(OrGrid),
+                                              C.c_uint32, _PF, C.c_int]),
             "or_candidate_pairs": (C.c_uint64, [_PF, C.c_uint32, C.POINTER(OrGrid)]),
             "or_splat": (None, [_PF, C.c_uint32, C.c_uint32, C.c_uint32, _PU32]),
             "or_blur_weights": (C.c_uint32, [C.c_float, _PF, C.c_uint32]),
@@ -183,6 +185,17 @@ def neighbor_mean_strided(points, fixed, g, stride, threads=0):
     fx = np.ascontiguousarray(fixed, dtype=np.uint8)
     lib().or_neighbor_mean_strided(_f(p), fx.ctypes.data_as(_PU8), len(p) // 2, C.byref(g),
                                    stride, _f(out), threads)
+    return out.reshape(-1, 2)
+
+
+def neighbor_mean_cells(points, fixed, g, stride=1, threads=0):
+    """The reference's tiled algorithm (cell summaries with the fully-inside
+    shortcut) in exact sums: the same bits as neighbor_mean."""
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1)
+    out = np.zeros_like(p)
+    fx = np.ascontiguousarray(fixed, dtype=np.uint8)
+    lib().or_neighbor_mean_cells(_f(p), fx.ctypes.data_as(_PU8), len(p) // 2, C.byref(g),
+                                 stride, _f(out), threads)
     return out.reshape(-1, 2)
 
 

@@ -229,6 +229,41 @@ def test_neighbor_mean_independent_of_cell_size(orc, cell, cell_w):
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+@pytest.mark.parametrize("seed,res,radius,cell", [(0, 200.0, 10.0, None), (1, 500.0, 30.0, None),
+                                                 (2, 500.0, 5.3, 2.0), (3, 200.0, 40.0, None)])
+def test_cell_summary_shortcut_bitexact_vs_bruteforce(orc, seed, res, radius, cell):
+    """The reference's tiled algorithm (fully-inside cells contribute their
+    summary, the others are tested point by point) gives the brute-force
+    bits: tiny clusters, where many cells are fully inside, and every
+    stride-th sample of the plain CPU case."""
+    pts, fixed = _tiny(seed, 600)
+    g = orc.make_grid(res, radius, cell)
+    want = brute.neighbor_mean_bruteforce(pts, fixed, res, res, radius)
+    got = orc.neighbor_mean_cells(pts, fixed, g, 1, threads=2)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_cell_summary_shortcut_equals_window_scan_on_graph(orc):
+    from bundling_thingy_amd import workloads
+    from bundling_thingy_amd.api import Options
+    g = workloads.plain_cpu_graph()
+    opts = Options(map_size=201, neighbor_radius=10)
+    p = opts.iter_params(0)
+    pl = orc.Pipeline(g.points, g.starts, g.bbox, p)
+    pts = pl.points
+    for _ in range(2):
+        grid = orc.make_grid(p.res_x, p.radius)  # the reference's cells: side = radius
+        a = orc.neighbor_mean(pts, pl.fixed, grid)
+        b = orc.neighbor_mean_cells(pts, pl.fixed, grid, 1)
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+        c = orc.neighbor_mean_cells(pts, pl.fixed, grid, 7)
+        k = np.arange(len(pts))
+        sel = (k % 7 == 0) & (pl.fixed == 0)
+        np.testing.assert_array_equal(c[sel].view(np.uint32), a[sel].view(np.uint32))
+        np.testing.assert_array_equal(c[~sel].view(np.uint32), pts[~sel].view(np.uint32))
+        pts = a
+
+
 def test_neighbor_mean_close_to_f64_and_reference_f32(orc):
     pts, fixed = _tiny(9, 600)
     g = orc.make_grid(200.0, 12.0)
