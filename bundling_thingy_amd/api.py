@@ -157,6 +157,9 @@ SIGNATURES = {
     "bt_server_release": (C.c_int, [_P, _U64]),
     "bt_server_clock_ms": (_D, [_P]),
     "bt_server_get_info": (C.c_int, [_P, C.POINTER(bt_server_info)]),
+    "bt_autotune_shapes": (C.c_int, [_P, _U32, _PU32, _PD]),
+    "bt_set_shapes": (C.c_int, [_P, _U32, _U32, _U32, _U32]),
+    "bt_get_shapes": (C.c_int, [_P, _U32, _U32, _PU32]),
     "bt_server_take_finished": (C.c_int, [_P, _PU64, _SZ, C.POINTER(_SZ)]),
 }
 
@@ -561,6 +564,24 @@ class Gpu:
         self._check(lib().bt_autotune(self._h, iterations, _u32p(cand), len(cand),
                                       _u32p(chosen), best.ctypes.data_as(_PD)))
         return chosen, best
+
+    def autotune_shapes(self, iterations):
+        """Times every kernel shape of the neighbor mean, row blur and column
+        blur per iteration; keeps the fastest. Returns (shapes (n, 3), stage
+        ms (n, 3))."""
+        shapes = np.zeros((iterations, 3), dtype=np.uint32)
+        ms = np.zeros((iterations, 3), dtype=np.float64)
+        self._check(lib().bt_autotune_shapes(self._h, iterations, _u32p(shapes),
+                                             ms.ctypes.data_as(_PD)))
+        return shapes, ms
+
+    def set_shapes(self, nm=0, row=0, col=0, iteration=0xffffffff):
+        self._check(lib().bt_set_shapes(self._h, iteration, nm, row, col))
+
+    def shapes(self, iteration, job=0):
+        out = np.zeros(3, dtype=np.uint32)
+        self._check(lib().bt_get_shapes(self._h, job, iteration, _u32p(out)))
+        return tuple(int(v) for v in out)
 
     def grid(self):
         """(cell of each sample in Layout order, per-cell counts) of the last iteration."""

@@ -286,6 +286,30 @@ int bt_autotune(bt_ctx* ctx, uint32_t n, const uint32_t* candidates, uint32_t n_
   });
 }
 
+int bt_autotune_shapes(bt_ctx* ctx, uint32_t n, uint32_t* shapes, double* stage_ms) {
+  CTX_CHECK();
+  return guarded(&ctx->err, [&] { ctx->engine->autotune_shapes(n, shapes, stage_ms); });
+}
+
+int bt_set_shapes(bt_ctx* ctx, uint32_t iteration, uint32_t nm_shape, uint32_t row_shape,
+                  uint32_t col_shape) {
+  CTX_CHECK();
+  return guarded(&ctx->err,
+                 [&] { ctx->engine->set_shapes(iteration, nm_shape, row_shape, col_shape); });
+}
+
+int bt_get_shapes(bt_ctx* ctx, uint32_t job, uint32_t iteration, uint32_t* shapes3) {
+  CTX_CHECK();
+  if (!shapes3) return BT_E_INVALID;
+  return guarded(&ctx->err, [&] {
+    const bt::Job& j = ctx->engine->job(job);
+    const uint32_t v = j.kshape[j.entry(iteration)];
+    shapes3[0] = v & 0xffu;
+    shapes3[1] = (v >> 8) & 0xffu;
+    shapes3[2] = (v >> 16) & 0xffu;
+  });
+}
+
 int bt_copy_probe(bt_ctx* ctx, uint64_t bytes, double* ms) {
   CTX_CHECK();
   return guarded(&ctx->err, [&] {

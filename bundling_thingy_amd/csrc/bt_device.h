@@ -20,10 +20,6 @@
 #define BT_SCAN_ITEMS 4096         // cells per workgroup in the cell scan (256 x 16)
 #define BT_TASK_BLOCKS 256         // task blocks per workgroup in task emission (one per thread)
 #define BT_CLEAR_ITEMS 4096        // words per workgroup in the clear kernel
-#define BT_ROW_TILE_W 64           // row blur tile: 64 columns x 4 rows
-#define BT_ROW_TILE_H 4
-#define BT_COL_TILE_W 64           // column blur tile: 64 columns x 32 rows
-#define BT_COL_TILE_H 32
 #define BT_NORM_ITEMS 1024         // pixels per workgroup in normalize
 #define BT_EMIT_PATHS 4            // paths per workgroup in resample_emit (one wave each)
 

@@ -177,7 +177,8 @@ Engine::Engine(const bt_options& o) : opts_(o), pool_(new DevPool) {
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device_));
   cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-  nm_per_cu_ = btk::neighbor_mean_blocks_per_cu();
+  for (uint32_t k = 0; k < BT_NM_SHAPE_COUNT; ++k)
+    nm_per_cu_[k] = btk::neighbor_mean_blocks_per_cu(k);
   noalloc_setup(device_);  // the allocation-free entry point's scratch on this device
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   ctl_.alloc(sizeof(BtControl));
@@ -218,7 +219,11 @@ void Engine::split_counts(uint32_t* tasks, uint32_t* parts) {
 }
 
 void Engine::work_counts(uint64_t* steps, uint64_t* neighbors, uint64_t* candidates) {
-  const size_t wg = std::min<size_t>((size_t)cus_ * nm_per_cu_, BT_MAX_WG);
+  const size_t wg = std::min<size_t>(last_nm_grid_, BT_MAX_WG);
+  if (wg == 0) {
+    *steps = *neighbors = *candidates = 0;
+    return;
+  }
   std::vector<unsigned long long> h(5 * wg);
   const char* base = split_.as<char>();
   for (int k = 0; k < 4; ++k)
@@ -265,7 +270,7 @@ size_t Engine::device_bytes() const {
 }
 
 // Workgroups each job needs for each kernel kind.
-static uint32_t kind_blocks(int k, const Job& j, const BtJob& d) {
+static uint32_t kind_blocks(int k, const Job& j, const BtJob& d, uint32_t kshape) {
   const uint64_t npix = (uint64_t)d.dw * d.dh;
   switch (k) {
     case BT_K_CLEAR: return cdiv(std::max<uint64_t>(d.n_cells + 1ull, npix), BT_CLEAR_ITEMS);
@@ -273,8 +278,12 @@ static uint32_t kind_blocks(int k, const Job& j, const BtJob& d) {
     case BT_K_RANK: return cdiv(j.n, BT_RANK_BLOCK);
     case BT_K_SCAN: return cdiv(d.n_cells + 1ull, BT_SCAN_ITEMS);
     case BT_K_TASK: return cdiv((uint64_t)d.task_bx * d.gy, BT_TASK_BLOCKS);
-    case BT_K_ROW: return cdiv(d.dw, BT_ROW_TILE_W) * cdiv(d.dh, BT_ROW_TILE_H);
-    case BT_K_COL: return cdiv(d.dw, BT_COL_TILE_W) * cdiv(d.dh, BT_COL_TILE_H);
+    case BT_K_ROW: {
+      uint32_t tw, th;
+      btk::row_tile((kshape >> 8) & 0xffu, &tw, &th);
+      return cdiv(d.dw, tw) * cdiv(d.dh, th);
+    }
+    case BT_K_COL: return cdiv(d.dw, 64) * cdiv(d.dh, btk::col_tile_h((kshape >> 16) & 0xffu));
     case BT_K_NORM: return cdiv(npix, BT_NORM_ITEMS);
     case BT_K_PATH: return cdiv(j.n_paths + 1ull, 256);
     case BT_K_EMIT: return cdiv(j.n_paths, BT_EMIT_PATHS);
@@ -290,11 +299,15 @@ Plan Engine::make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>&
     const uint32_t it = iters[i];
     entries[i] = js[i]->dev(it, (js[i]->parity + (it - js[i]->iteration)) & 1u);
   }
-  return make_plan_of(js, entries);
+  // one kernel shape per launch: the first job's choice for its iteration
+  const uint32_t ks = js.empty() ? 0u : js[0]->kshape[js[0]->entry(iters[0])];
+  return make_plan_of(js, entries, ks);
 }
 
-Plan Engine::make_plan_of(const std::vector<Job*>& js, const std::vector<BtJob>& entries) const {
+Plan Engine::make_plan_of(const std::vector<Job*>& js, const std::vector<BtJob>& entries,
+                          uint32_t kshape) const {
   Plan p;
+  p.kshape = kshape;
   const uint32_t nj = (uint32_t)js.size();
   p.njobs = nj;
   p.base_off = align_up(nj * sizeof(BtJob), 256);
@@ -309,7 +322,7 @@ Plan Engine::make_plan_of(const std::vector<Job*>& js, const std::vector<BtJob>&
     uint64_t acc = 0;
     for (uint32_t i = 0; i < nj; ++i) {
       B[k * (nj + 1) + i] = (uint32_t)acc;
-      acc += kind_blocks(k, *js[i], J[i]);
+      acc += kind_blocks(k, *js[i], J[i], kshape);
     }
     if (acc > 0x7fffffffull) fail(BT_E_CAPACITY, "launch exceeds the workgroup limit");
     B[k * (nj + 1) + nj] = (uint32_t)acc;
@@ -410,13 +423,15 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   split.wg_nbr = work + BT_MAX_WG;
   split.wg_hsteps = work + 2 * BT_MAX_WG;
   split.wg_hnbr = work + 3 * BT_MAX_WG;
-  btk::neighbor_mean(stream_, (uint32_t)cus_ * nm_per_cu_, J, tasks_.as<BtTask>(), ctl, split,
-                     p.samples);
+  const uint32_t shape = p.kshape & 0xffu;
+  last_nm_grid_ = (uint32_t)cus_ * nm_per_cu_[shape];
+  btk::neighbor_mean(stream_, last_nm_grid_, J, tasks_.as<BtTask>(), ctl, split, p.samples,
+                     shape);
   mark(6);
   if (density && p.grid[BT_K_ROW]) {
-    btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj);
+    btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj, (p.kshape >> 8) & 0xffu);
     mark(7);
-    btk::blur_cols(stream_, p.grid[BT_K_COL], J, B(BT_K_COL), (int)nj);
+    btk::blur_cols(stream_, p.grid[BT_K_COL], J, B(BT_K_COL), (int)nj, (p.kshape >> 16) & 0xffu);
     mark(8);
     btk::normalize(stream_, p.grid[BT_K_NORM], J, B(BT_K_NORM), (int)nj);
     mark(9);
@@ -497,6 +512,7 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
     j->sched = schedule_len(j->o);
     j->params.resize(j->sched);
     j->splits.resize(j->sched);
+    j->kshape.assign(j->sched, 0u);
     std::vector<float> wt((size_t)j->sched * kWt, 0.0f);
     for (uint32_t i = 0; i < j->sched; ++i) {
       iter_params(j->o, i, j->o.cell_split, &j->params[i]);
@@ -789,10 +805,12 @@ void Engine::iterate(uint32_t n, bool use_graph) {
       launch_iteration(plans[i], buf->as<unsigned char>() + i * slot, density);
     HIPCHK(hipStreamEndCapture(stream_, &g.graph));
     HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    g.last_nm_grid = last_nm_grid_;
     found = graphs_.emplace(key, g).first;
     graph_bufs_.emplace(key, std::move(buf));
   }
   HIPCHK(hipGraphLaunch(found->second.exec, stream_));
+  last_nm_grid_ = found->second.last_nm_grid;
   for (Job* j : js) j->advanced(n);
   sync();
 }
@@ -920,6 +938,109 @@ void Engine::autotune(uint32_t n, const uint32_t* cand, uint32_t ncand, uint32_t
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  rewind();
+}
+
+void Engine::set_shapes(uint32_t it, uint32_t nm, uint32_t row, uint32_t col) {
+  if (nm >= BT_NM_SHAPE_COUNT || row >= BT_ROW_SHAPE_COUNT || col >= BT_COL_SHAPE_COUNT)
+    fail(BT_E_INVALID, "unknown kernel shape");
+  if (jobs_.empty()) fail(BT_E_NO_GRAPH, "no uploaded graph");
+  const uint32_t v = nm | (row << 8) | (col << 16);
+  for (auto& j : jobs_) {
+    if (it == ~0u)
+      std::fill(j->kshape.begin(), j->kshape.end(), v);
+    else
+      j->kshape[j->entry(it)] = v;
+  }
+  drop_graphs();  // captured launches hold the old shapes
+}
+
+void Engine::autotune_shapes(uint32_t n, uint32_t* shapes, double* stage_ms) {
+  std::vector<Job*> js = all_jobs();
+  check_iterable(js);
+  uint32_t sched = 1;
+  for (Job* j : js) sched = std::max(sched, j->sched);
+  n = std::min(n, sched);
+  rewind();
+  drop_graphs();
+  std::vector<std::unique_ptr<DevBuf>> snap;
+  for (Job* j : js) {
+    snap.push_back(std::make_unique<DevBuf>());
+    snap.back()->alloc((size_t)j->n * 12);
+  }
+  auto cur = [](Job* j) { return j->parity ? j->rec1.p : j->rec0.p; };
+  auto restore = [&]() {
+    for (size_t k = 0; k < js.size(); ++k)
+      HIPCHK(hipMemcpyAsync(cur(js[k]), snap[k]->p, (size_t)js[k]->n * 12,
+                            hipMemcpyDeviceToDevice, stream_));
+  };
+  bool density = false;
+  for (Job* j : js) density |= j->dens_cnt.p != nullptr;
+  hipEvent_t ev[10];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  // one iteration from the current state; the times of the neighbor-mean,
+  // row-blur and column-blur stages
+  auto run_one = [&](double* t3) {
+    ensure_tasks(js);
+    std::vector<uint32_t> it;
+    for (Job* j : js) it.push_back(j->iteration);
+    Plan p = make_plan(js, it);
+    const size_t slot = align_up(p.host.size(), 256);
+    if (2 * slot > plans_.bytes) {
+      sync();
+      plans_.reserve(2 * slot + 4096);
+    }
+    eager_slot_ ^= 1u;
+    unsigned char* dev = plans_.as<unsigned char>() + eager_slot_ * slot;
+    put_plan(p, dev);
+    launch_iteration(p, dev, density, ev);
+    sync();
+    float a = 0.0f, b = 0.0f, c = 0.0f;
+    HIPCHK(hipEventElapsedTime(&a, ev[5], ev[6]));
+    HIPCHK(hipEventElapsedTime(&b, ev[6], ev[7]));
+    HIPCHK(hipEventElapsedTime(&c, ev[7], ev[8]));
+    t3[0] = a;
+    t3[1] = b;
+    t3[2] = c;
+  };
+  try {
+    for (uint32_t i = 0; i < n; ++i) {
+      for (size_t k = 0; k < js.size(); ++k)
+        HIPCHK(hipMemcpyAsync(snap[k]->p, cur(js[k]), (size_t)js[k]->n * 12,
+                              hipMemcpyDeviceToDevice, stream_));
+      uint32_t best[3] = {0, 0, 0};
+      double best_t[3] = {1e300, 1e300, 1e300};
+      const uint32_t counts[3] = {BT_NM_SHAPE_COUNT, BT_ROW_SHAPE_COUNT, BT_COL_SHAPE_COUNT};
+      const uint32_t stages = density ? 3u : 1u;
+      for (uint32_t st = 0; st < stages; ++st) {  // one stage at a time, the others fixed
+        for (uint32_t c = 0; c < counts[st]; ++c) {
+          uint32_t v[3] = {best[0], best[1], best[2]};
+          v[st] = c;
+          restore();
+          for (Job* j : js) j->kshape[j->entry(i)] = v[0] | (v[1] << 8) | (v[2] << 16);
+          double t3[3];
+          run_one(t3);
+          if (t3[st] < best_t[st]) {
+            best_t[st] = t3[st];
+            best[st] = c;
+          }
+        }
+      }
+      restore();
+      for (Job* j : js) j->kshape[j->entry(i)] = best[0] | (best[1] << 8) | (best[2] << 16);
+      double t3[3];
+      run_one(t3);
+      for (Job* j : js) j->advanced(1);
+      for (uint32_t k = 0; k < 3; ++k) {
+        if (shapes) shapes[3 * i + k] = best[k];
+        if (stage_ms) stage_ms[3 * i + k] = best_t[k] < 1e300 ? best_t[k] : 0.0;
+      }
+    }
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
   rewind();
 }
 

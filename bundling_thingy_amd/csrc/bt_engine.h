@@ -8,6 +8,7 @@
 
 #include "bt_common.h"
 #include "bt_device.h"
+#include "bt_launch.h"
 
 namespace bt {
 
@@ -29,6 +30,9 @@ struct Job {
   uint32_t sched = 1;           // distinct parameter sets
   std::vector<bt_iter_params> params;
   std::vector<uint32_t> splits; // grid cells per radius, per schedule entry
+  // kernel shapes per schedule entry: neighbor mean | row blur << 8 | column
+  // blur << 16 (btk shape indices; 0 = defaults)
+  std::vector<uint32_t> kshape;
   // control points and path starts: in the upload area, during the upload only
   const float* in_ctrl = nullptr;
   const uint32_t* in_starts = nullptr;
@@ -68,6 +72,7 @@ struct Job {
 struct Plan {
   std::vector<unsigned char> host;
   uint32_t grid[BT_K_COUNT] = {};
+  uint32_t kshape = 0;  // kernel shapes of the launch (Job::kshape encoding)
   uint32_t njobs = 0;
   uint64_t samples = 0;  // samples of the plan's jobs
   size_t base_off = 0;  // byte offset of the bases block
@@ -117,6 +122,14 @@ class Engine {
   // every candidate from the same state; leaves the jobs rewound.
   void autotune(uint32_t n, const uint32_t* cand, uint32_t ncand, uint32_t* chosen,
                 double* best_ms);
+  // Picks the kernel shapes (neighbor mean, row blur, column blur) of each
+  // of the first n iterations by timing each stage under every shape from
+  // the same state (the others at their current choice); leaves the jobs
+  // rewound. shapes[3 i + k], stage_ms[3 i + k] (may be null).
+  void autotune_shapes(uint32_t n, uint32_t* shapes, double* stage_ms);
+  // Sets schedule entry `it`'s kernel shapes on every job (all entries when
+  // it is ~0u).
+  void set_shapes(uint32_t it, uint32_t nm, uint32_t row, uint32_t col);
 
   void read_paths(uint32_t job, uint32_t* rec3, size_t n);
   void read_points(uint32_t job, void* rec12, size_t n);
@@ -156,8 +169,10 @@ class Engine {
 
  private:
   Plan make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>& iters) const;
-  // A plan over explicit job-table entries (block bases from the entries).
-  Plan make_plan_of(const std::vector<Job*>& js, const std::vector<BtJob>& entries) const;
+  // A plan over explicit job-table entries (block bases from the entries and
+  // the kernel shapes).
+  Plan make_plan_of(const std::vector<Job*>& js, const std::vector<BtJob>& entries,
+                    uint32_t kshape = 0) const;
   // Copies a plan to device memory through a ring of pinned host slots (the
   // copy is asynchronous; a slot is reused once its previous copy is done).
   void put_plan(const Plan& p, void* dst, hipStream_t st = nullptr);
@@ -174,7 +189,7 @@ class Engine {
   std::unique_ptr<DevPool> pool_;  // declared before the jobs: outlives their buffers
   int device_ = 0;
   int cus_ = 256;
-  uint32_t nm_per_cu_ = 4;  // resident neighbor-mean workgroups per CU
+  uint32_t nm_per_cu_[BT_NM_SHAPE_COUNT] = {};  // resident neighbor-mean workgroups per CU
   hipStream_t stream_ = nullptr;
   std::vector<std::unique_ptr<Job>> jobs_;
   DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_, retire_plans_;
@@ -198,7 +213,9 @@ class Engine {
   struct GraphEntry {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
+    uint32_t last_nm_grid = 0;  // neighbor-mean grid of its last iteration
   };
+  uint32_t last_nm_grid_ = 0;   // neighbor-mean grid of the last launched iteration
   std::map<std::vector<uint64_t>, GraphEntry> graphs_;
   std::map<std::vector<uint64_t>, std::unique_ptr<DevBuf>> graph_bufs_;
 };

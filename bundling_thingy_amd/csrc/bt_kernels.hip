@@ -588,22 +588,40 @@ __device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total
   return 0xffffffffu;
 }
 
-#define NM_WAVES 4          // waves per workgroup
 #define NM_ROWS 64          // window rows set up per pass (one per lane)
 #define NM_RANGES (2 * NM_ROWS)  // ranges per row pass (two per row)
-#define NM_GROUPS 4         // lane groups of the grouped stream
-#define NM_GLANES 16        // lanes per group
 #define NM_GCHUNK 64        // candidates per group per chunk of the grouped stream
 #define NM_GSTRIDE 65       // float4 per lane group in a tile (one of padding: no bank conflicts)
-#define NM_TILE (NM_GROUPS * NM_GSTRIDE > 128 ? NM_GROUPS * NM_GSTRIDE : 128)
 #define NM_GROUP_MIN 2048u  // candidates above which a task streams per lane group
+
+// Kernel shape, a template parameter the autotuner chooses per iteration:
+// lanes per group of the grouped stream (16: 4 groups, 32: 2 groups) and
+// waves per workgroup. Results do not depend on it (exact sums).
+template <int GL, int W>
+struct NmCfg {
+  static constexpr uint32_t GLANES = GL;
+  static constexpr uint32_t GROUPS = 64 / GL;
+  static constexpr uint32_t WAVES = W;
+  static constexpr uint32_t TILE = GROUPS * NM_GSTRIDE > 128 ? GROUPS * NM_GSTRIDE : 128;
+  // record of a heavy task's set-up (BT_REC_WORDS words): per lane group its
+  // ranges (start[128], pref[129]), then stream lengths m[GROUPS], range
+  // counts n[GROUPS], then per group the inside sums x, y, count (int64)
+  static constexpr uint32_t REC_GROUP = NM_RANGES + NM_RANGES + 1u;
+  static constexpr uint32_t REC_M = GROUPS * REC_GROUP;
+  static constexpr uint32_t REC_N = REC_M + GROUPS;
+  static constexpr uint32_t REC_SUMS = REC_N + GROUPS + ((REC_N + GROUPS) & 1u);
+  static_assert(REC_SUMS + 6 * GROUPS <= BT_REC_WORDS, "heavy record layout");
+  static_assert(REC_SUMS % 2 == 0, "int64 sums must be 8-byte aligned");
+  static_assert(GL == 16 || GL == 32, "lane groups of 16 or 32");
+};
 
 // The tile holds either two chunks of 64 candidates (plain stream, double
 // buffered) or one chunk of 64 candidates per lane group (grouped stream).
+template <class C>
 struct NmLds {
-  float4 tile[NM_WAVES][NM_TILE];
-  uint32_t start[NM_WAVES][NM_GROUPS][NM_RANGES];
-  uint32_t pref[NM_WAVES][NM_GROUPS][NM_RANGES + 1];
+  float4 tile[C::WAVES][C::TILE];
+  uint32_t start[C::WAVES][C::GROUPS][NM_RANGES];
+  uint32_t pref[C::WAVES][C::GROUPS][NM_RANGES + 1];
 };
 
 // Per-wave state of one task.
@@ -758,7 +776,8 @@ __device__ __forceinline__ float4 nm_candidate(const float4* S, const uint32_t* 
 
 // Tests the candidate chunks first, first + step, ... of the stream set up in
 // group 0's LDS ranges; adds the lane's accepted sums to (ax, ay, n).
-__device__ __forceinline__ void nm_stream(const NmTask& T, NmLds& L, uint32_t m, uint32_t nranges,
+template <class C>
+__device__ __forceinline__ void nm_stream(const NmTask& T, NmLds<C>& L, uint32_t m, uint32_t nranges,
                                           uint32_t first, uint32_t step, long long& ax,
                                           long long& ay, long long& n, uint64_t& steps) {
   const uint32_t lane = LANE(), w = threadIdx.x >> 6;
@@ -804,19 +823,21 @@ __device__ __forceinline__ void nm_stream(const NmTask& T, NmLds& L, uint32_t m,
   }
 }
 
+template <class C>
 __device__ __forceinline__ float group_min(float v) {
 #pragma unroll
-  for (int d = NM_GLANES / 2; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+  for (int d = C::GLANES / 2; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
   return v;
 }
+template <class C>
 __device__ __forceinline__ float group_max(float v) {
 #pragma unroll
-  for (int d = NM_GLANES / 2; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  for (int d = C::GLANES / 2; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
   return v;
 }
 
 // Grouped streams of a task whose window is one row group: the wave's lanes
-// form NM_GROUPS groups of NM_GLANES consecutive samples (neighbours along x),
+// form 64 / GLANES groups of GLANES consecutive samples (neighbours along x),
 // each with its own query box, inside sums and boundary ranges, which are
 // narrower than the whole task's. Per step, each group's lanes test the next
 // candidate of their group's stream (exhausted streams read candidates at
@@ -829,21 +850,22 @@ struct NmGroups {
 };
 
 // Sets up every group's ranges (LDS) and inside sums.
-__device__ __forceinline__ NmGroups nm_group_rows(const NmTask& T, NmLds& L) {
-  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / NM_GLANES;
+template <class C>
+__device__ __forceinline__ NmGroups nm_group_rows(const NmTask& T, NmLds<C>& L) {
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / C::GLANES;
   const BtJob& J = *T.J;
   const bool mv = T.movable;
-  const float gx0 = group_min(mv ? T.px : 3.0e38f), gx1 = group_max(mv ? T.px : -3.0e38f);
-  const float gy0 = group_min(mv ? T.py : 3.0e38f), gy1 = group_max(mv ? T.py : -3.0e38f);
+  const float gx0 = group_min<C>(mv ? T.px : 3.0e38f), gx1 = group_max<C>(mv ? T.px : -3.0e38f);
+  const float gy0 = group_min<C>(mv ? T.py : 3.0e38f), gy1 = group_max<C>(mv ? T.py : -3.0e38f);
   const float rr = J.radius + J.eps;
   NmGroups G;
   G.my_m = G.my_n = G.mmax = 0u;
   G.sx = G.sy = G.sn = 0;
   const uint64_t movable = ballot(mv);
 #pragma unroll
-  for (uint32_t g = 0; g < NM_GROUPS; ++g) {
-    const uint32_t l = g * NM_GLANES;
-    if (!(movable & (((1ull << NM_GLANES) - 1ull) << l))) continue;  // no movable sample
+  for (uint32_t g = 0; g < C::GROUPS; ++g) {
+    const uint32_t l = g * C::GLANES;
+    if (!(movable & (((1ull << C::GLANES) - 1ull) << l))) continue;  // no movable sample
     const float tx0 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gx0, (int)l, 64), J.res_x))));
     const float tx1 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gx1, (int)l, 64), J.res_x))));
     const float ty0 = __uint_as_float(uni(__float_as_uint(texel(__shfl(gy0, (int)l, 64), J.res_y))));
@@ -867,10 +889,11 @@ __device__ __forceinline__ NmGroups nm_group_rows(const NmTask& T, NmLds& L) {
 
 // Tests the group chunks first, first + step, ... (NM_GCHUNK steps each);
 // adds the lane's accepted sums to (ax, ay, n).
-__device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds& L, const NmGroups& G,
+template <class C>
+__device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds<C>& L, const NmGroups& G,
                                                 uint32_t first, uint32_t step, long long& ax,
                                                 long long& ay, long long& n, uint64_t& steps) {
-  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / NM_GLANES;
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / C::GLANES;
   const BtJob& J = *T.J;
   const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2, px = T.px, py = T.py;
   const float4* S = J.sorted;
@@ -878,27 +901,27 @@ __device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds& L, const
   const uint32_t* pref = L.pref[w][grp];
   const float inf = __int_as_float(0x7f800000);
   const uint32_t nch = (G.mmax + NM_GCHUNK - 1u) / NM_GCHUNK;
-  const uint32_t sub = lane % NM_GLANES;
+  const uint32_t sub = lane % C::GLANES;
   uint32_t cur = 0;
   auto fetch = [&](uint32_t ch, uint32_t i) -> float4 {
-    const uint32_t v = ch * NM_GCHUNK + i * NM_GLANES + sub;
+    const uint32_t v = ch * NM_GCHUNK + i * C::GLANES + sub;
     return v < G.my_m ? nm_candidate(S, start, pref, G.my_n, v, cur)
                       : make_float4(inf, inf, 0.f, 0.f);
   };
   float4* tile = &L.tile[w][grp * NM_GSTRIDE];
-  float4 nxt[NM_GCHUNK / NM_GLANES];
+  float4 nxt[NM_GCHUNK / C::GLANES];
   if (first < nch) {
 #pragma unroll
-    for (uint32_t i = 0; i < NM_GCHUNK / NM_GLANES; ++i) nxt[i] = fetch(first, i);
+    for (uint32_t i = 0; i < NM_GCHUNK / C::GLANES; ++i) nxt[i] = fetch(first, i);
   }
   for (uint32_t ch = first; ch < nch; ch += step) {
     // one buffer: the previous chunk's reads are complete (wave barrier below)
 #pragma unroll
-    for (uint32_t i = 0; i < NM_GCHUNK / NM_GLANES; ++i) tile[i * NM_GLANES + sub] = nxt[i];
+    for (uint32_t i = 0; i < NM_GCHUNK / C::GLANES; ++i) tile[i * C::GLANES + sub] = nxt[i];
     __builtin_amdgcn_wave_barrier();
     if (ch + step < nch) {
 #pragma unroll
-      for (uint32_t i = 0; i < NM_GCHUNK / NM_GLANES; ++i) nxt[i] = fetch(ch + step, i);
+      for (uint32_t i = 0; i < NM_GCHUNK / C::GLANES; ++i) nxt[i] = fetch(ch + step, i);
     }
     const uint32_t cnt = min((uint32_t)NM_GCHUNK, G.mmax - ch * (uint32_t)NM_GCHUNK);
     steps += cnt;
@@ -922,46 +945,39 @@ __device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds& L, const
   }
 }
 
-// Record of a heavy task's set-up (BT_REC_WORDS words): per lane group its
-// ranges (start[128], pref[129]), then stream lengths m[4], range counts
-// n[4], then per group the inside sums x, y, count (int64, 24 words).
-#define NM_REC_GROUP (NM_RANGES + NM_RANGES + 1u)
-#define NM_REC_M (NM_GROUPS * NM_REC_GROUP)
-#define NM_REC_N (NM_REC_M + NM_GROUPS)
-#define NM_REC_SUMS (NM_REC_N + NM_GROUPS)
-static_assert(NM_REC_SUMS + 6 * NM_GROUPS <= BT_REC_WORDS, "heavy record layout");
-static_assert(NM_REC_SUMS % 2 == 0, "int64 sums must be 8-byte aligned");
-
-__device__ __forceinline__ void nm_store_groups(const NmGroups& G, NmLds& L, uint32_t* rec) {
-  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / NM_GLANES;
-  for (uint32_t i = lane; i < NM_REC_M; i += 64u) {
-    const uint32_t g = i / NM_REC_GROUP, o = i - g * NM_REC_GROUP;
+// Stores a heavy task's set-up as its record (NmCfg's REC_ layout).
+template <class C>
+__device__ __forceinline__ void nm_store_groups(const NmGroups& G, NmLds<C>& L, uint32_t* rec) {
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / C::GLANES;
+  for (uint32_t i = lane; i < C::REC_M; i += 64u) {
+    const uint32_t g = i / C::REC_GROUP, o = i - g * C::REC_GROUP;
     rec[i] = o < NM_RANGES ? L.start[w][g][o] : L.pref[w][g][o - NM_RANGES];
   }
-  if (lane % NM_GLANES == 0) {
-    rec[NM_REC_M + grp] = G.my_m;
-    rec[NM_REC_N + grp] = G.my_n;
-    long long* sums = (long long*)(rec + NM_REC_SUMS) + 3u * grp;
+  if (lane % C::GLANES == 0) {
+    rec[C::REC_M + grp] = G.my_m;
+    rec[C::REC_N + grp] = G.my_n;
+    long long* sums = (long long*)(rec + C::REC_SUMS) + 3u * grp;
     sums[0] = G.sx;
     sums[1] = G.sy;
     sums[2] = G.sn;
   }
 }
 
-__device__ __forceinline__ NmGroups nm_load_groups(NmLds& L, const uint32_t* rec) {
-  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / NM_GLANES;
-  for (uint32_t i = lane; i < NM_REC_M; i += 64u) {
-    const uint32_t g = i / NM_REC_GROUP, o = i - g * NM_REC_GROUP;
+template <class C>
+__device__ __forceinline__ NmGroups nm_load_groups(NmLds<C>& L, const uint32_t* rec) {
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / C::GLANES;
+  for (uint32_t i = lane; i < C::REC_M; i += 64u) {
+    const uint32_t g = i / C::REC_GROUP, o = i - g * C::REC_GROUP;
     const uint32_t v = rec[i];
     if (o < NM_RANGES) L.start[w][g][o] = v; else L.pref[w][g][o - NM_RANGES] = v;
   }
   NmGroups G;
   G.mmax = 0u;
 #pragma unroll
-  for (uint32_t g = 0; g < NM_GROUPS; ++g) G.mmax = max(G.mmax, uni(rec[NM_REC_M + g]));
-  G.my_m = rec[NM_REC_M + grp];
-  G.my_n = rec[NM_REC_N + grp];
-  const long long* sums = (const long long*)(rec + NM_REC_SUMS) + 3u * grp;
+  for (uint32_t g = 0; g < C::GROUPS; ++g) G.mmax = max(G.mmax, uni(rec[C::REC_M + g]));
+  G.my_m = rec[C::REC_M + grp];
+  G.my_n = rec[C::REC_N + grp];
+  const long long* sums = (const long long*)(rec + C::REC_SUMS) + 3u * grp;
   G.sx = sums[0];
   G.sy = sums[1];
   G.sn = sums[2];
@@ -1007,6 +1023,7 @@ __device__ __forceinline__ uint64_t nm_finish(const NmTask& T, long long sx, lon
 // Per-workgroup totals of a launch (plain stores at exit, summed on the
 // host): wave-steps executed (one candidate tested against each of the 64
 // lanes' samples) and neighbors found.
+template <class C>
 __device__ __forceinline__ void nm_store_work(unsigned long long* steps_out,
                                               unsigned long long* nbr_out, uint64_t steps,
                                               uint64_t nbr, unsigned long long (*red)[2]) {
@@ -1017,7 +1034,7 @@ __device__ __forceinline__ void nm_store_work(unsigned long long* steps_out,
   __syncthreads();
   if (threadIdx.x == 0 && blockIdx.x < BT_MAX_WG) {
     unsigned long long a = 0, b = 0;
-    for (int w = 0; w < NM_WAVES; ++w) {
+    for (uint32_t w = 0; w < C::WAVES; ++w) {
       a += red[w][0];
       b += red[w][1];
     }
@@ -1026,11 +1043,16 @@ __device__ __forceinline__ void nm_store_work(unsigned long long* steps_out,
   }
 }
 
-extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
+// 4 waves per SIMD (16 per CU) whatever the workgroup size: at most 128
+// VGPRs per lane.
+template <int GL, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4, 4))) void
+bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split, unsigned long long samples) {
-  __shared__ NmLds L;
-  __shared__ unsigned long long wave_cand[NM_WAVES];
+  typedef NmCfg<GL, W> C;
+  __shared__ NmLds<C> L;
+  __shared__ unsigned long long wave_cand[C::WAVES];
   const uint32_t lane = LANE(), w = threadIdx.x >> 6;
   const uint32_t total = uni(__atomic_load_n(&ctl->task_total, __ATOMIC_RELAXED));
   // split threshold from the previous launch's candidates per wave
@@ -1039,7 +1061,7 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
   if (prev) {
     const double expect = sprev ? (double)prev * ((double)samples / (double)sprev) : (double)prev;
     const unsigned long long per_wave =
-        (unsigned long long)(expect / ((double)gridDim.x * NM_WAVES));
+        (unsigned long long)(expect / ((double)gridDim.x * C::WAVES));
     heavy = (uint32_t)min(max(per_wave / 2ull, (unsigned long long)BT_HEAVY_LO),
                           (unsigned long long)BT_HEAVY_HI);
   }
@@ -1064,7 +1086,7 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
         const bool one_group = rb == T.cy0 && T.cy1 - T.cy0 < NM_ROWS;
         if (one_group && m > NM_GROUP_MIN) {
           // grouped streams; a long one is recorded for the split kernel
-          const NmGroups G = nm_group_rows(T, L);  // its own ranges and inside sums
+          const NmGroups G = nm_group_rows<C>(T, L);  // its own ranges and inside sums
           if (m > heavy) {
             const uint32_t parts = min(BT_MAX_PARTS, (G.mmax + part_steps - 1u) / part_steps);
             uint32_t h = 0, base = 0;
@@ -1075,7 +1097,7 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
             h = uni(h);
             base = uni(base);
             if (h < BT_MAX_HEAVY && base + parts <= BT_MAX_ITEMS) {
-              nm_store_groups(G, L, split.rec + (size_t)h * BT_REC_WORDS);
+              nm_store_groups<C>(G, L, split.rec + (size_t)h * BT_REC_WORDS);
               for (uint32_t p = lane; p < parts; p += 64u) split.items[base + p] = make_uint2(h, p);
               long long* acc = split.acc + (size_t)h * 192u;
               acc[lane] = 0;
@@ -1099,13 +1121,13 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
           ax += G.sx;
           ay += G.sy;
           n += G.sn;
-          nm_group_stream(T, L, G, 0u, 1u, ax, ay, n, steps);
+          nm_group_stream<C>(T, L, G, 0u, 1u, ax, ay, n, steps);
           break;
         }
         ux += sx;
         uy += sy;
         un += sn;
-        nm_stream(T, L, m, nranges, 0u, 1u, ax, ay, n, steps);
+        nm_stream<C>(T, L, m, nranges, 0u, 1u, ax, ay, n, steps);
       }
       if (deferred) continue;
     }
@@ -1116,11 +1138,11 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
   __syncthreads();
   if (threadIdx.x == 0 && blockIdx.x < BT_MAX_WG) {
     unsigned long long s = 0;
-    for (int w = 0; w < NM_WAVES; ++w) s += wave_cand[w];
+    for (uint32_t w = 0; w < C::WAVES; ++w) s += wave_cand[w];
     split.wg_cand[blockIdx.x] = s;
   }
-  __shared__ unsigned long long work[NM_WAVES][2];
-  nm_store_work(split.wg_steps, split.wg_nbr, steps, nbr, work);
+  __shared__ unsigned long long work[C::WAVES][2];
+  nm_store_work<C>(split.wg_steps, split.wg_nbr, steps, nbr, work);
 }
 
 // Runs the parts of heavy tasks: part p tests group chunks p, p + P, p + 2P,
@@ -1130,23 +1152,29 @@ extern "C" __global__ __launch_bounds__(256, 4) void bt_neighbor_mean(
 // last (a counter; no wave waits) reads the accumulators and writes the
 // means. Block 0 first sums the main launch's per-workgroup candidate counts
 // into the control block, for the next launch's split threshold.
-extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
+template <int GL, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void
+bt_neighbor_mean_heavy(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split, unsigned long long samples) {
-  __shared__ NmLds L;
+  typedef NmCfg<GL, W> C;
+  __shared__ NmLds<C> L;
   const uint32_t lane = LANE();
   const uint32_t items = min(uni(__atomic_load_n(&ctl->item_count, __ATOMIC_RELAXED)),
                              BT_MAX_ITEMS);
   if (blockIdx.x == 0) {
     // total candidates of the main launch (same grid), for the next threshold
-    __shared__ unsigned long long red[4];
+    __shared__ unsigned long long red[C::WAVES];
     unsigned long long s = 0;
-    for (uint32_t b = threadIdx.x; b < min(gridDim.x, BT_MAX_WG); b += 256u) s += split.wg_cand[b];
+    for (uint32_t b = threadIdx.x; b < min(gridDim.x, BT_MAX_WG); b += 64u * C::WAVES)
+      s += split.wg_cand[b];
     s = wave_sum_u64(s);
     if (lane == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-      ctl->cand_prev = red[0] + red[1] + red[2] + red[3];
+      unsigned long long t = 0;
+      for (uint32_t w = 0; w < C::WAVES; ++w) t += red[w];
+      ctl->cand_prev = t;
       ctl->samples_prev = samples;
     }
   }
@@ -1165,11 +1193,11 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
     long long ax = 0, ay = 0, n = 0, ux = 0, uy = 0, un = 0;
     // the set-up the main kernel stored; part p takes group chunks p,
     // p + parts, ...
-    const NmGroups G = nm_load_groups(L, split.rec + (size_t)h * BT_REC_WORDS);
+    const NmGroups G = nm_load_groups<C>(L, split.rec + (size_t)h * BT_REC_WORDS);
     ux = G.sx;
     uy = G.sy;
     un = G.sn;
-    nm_group_stream(T, L, G, p, parts, ax, ay, n, steps);
+    nm_group_stream<C>(T, L, G, p, parts, ax, ay, n, steps);
     if (p == 0u) {
       ax += ux;
       ay += uy;
@@ -1194,27 +1222,30 @@ extern "C" __global__ __launch_bounds__(256) void bt_neighbor_mean_heavy(
       nbr += nm_finish(T, sx, sy, sn);
     }
   }
-  __shared__ unsigned long long work[NM_WAVES][2];
-  nm_store_work(split.wg_hsteps, split.wg_hnbr, steps, nbr, work);
+  __shared__ unsigned long long work[C::WAVES][2];
+  nm_store_work<C>(split.wg_hsteps, split.wg_hnbr, steps, nbr, work);
 }
 
 // ------------------------------------------------------------------- blur
 // blur.wgsl:13-65 generalized to any sigma: row pass from the u32 counts.
-// A 64 x 4 tile plus a taps-wide halo on each side sits in LDS; zero padding
-// outside the image; v = t[c]*w0, then v += (t[c-k] + t[c+k]) * w[k].
-extern "C" __global__ __launch_bounds__(256) void bt_blur_rows(const BtJob* __restrict__ jobs,
-                                                               const uint32_t* __restrict__ base,
-                                                               int njobs) {
+// A TW x TH tile (TW * TH = 256 threads, one output each) plus a taps-wide
+// halo on each side sits in LDS; zero padding outside the image;
+// v = t[c]*w0, then v += (t[c-k] + t[c+k]) * w[k]. The tile shape is a
+// template parameter the autotuner picks per iteration.
+template <int TW, int TH>
+__global__ __launch_bounds__(256) void bt_blur_rows(const BtJob* __restrict__ jobs,
+                                                    const uint32_t* __restrict__ base, int njobs) {
+  static_assert(TW * TH == 256, "one output per thread");
   JOB_PROLOGUE(BT_K_ROW);
-  __shared__ float tile[BT_ROW_TILE_H][BT_ROW_TILE_W + 2 * BT_MAX_TAPS];
+  __shared__ float tile[TH][TW + 2 * BT_MAX_TAPS];
   const uint32_t W = J.dw, H = J.dh, K = J.taps;
-  const uint32_t tiles_x = (W + BT_ROW_TILE_W - 1u) / BT_ROW_TILE_W;
-  const uint32_t x0 = (lb % tiles_x) * BT_ROW_TILE_W, y0 = (lb / tiles_x) * BT_ROW_TILE_H;
-  const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
+  const uint32_t tiles_x = (W + TW - 1u) / TW;
+  const uint32_t x0 = (lb % tiles_x) * TW, y0 = (lb / tiles_x) * TH;
+  const uint32_t tx = threadIdx.x % TW, ty = threadIdx.x / TW;
   const uint32_t y = y0 + ty;
-  const uint32_t span = BT_ROW_TILE_W + 2u * K;
+  const uint32_t span = TW + 2u * K;
   const uint32_t* src = J.dens_cnt + (size_t)y * W;
-  for (uint32_t e = tx; e < span; e += 64u) {
+  for (uint32_t e = tx; e < span; e += TW) {
     const int gx = (int)x0 - (int)K + (int)e;
     float v = 0.0f;
     if (y < H && gx >= 0 && gx < (int)W) v = (float)src[gx];
@@ -1232,19 +1263,21 @@ extern "C" __global__ __launch_bounds__(256) void bt_blur_rows(const BtJob* __re
 }
 
 // blur.wgsl:13-83 column pass plus the peak (atomicMax on the f32 bits of a
-// non-negative value, as the reference does).
-extern "C" __global__ __launch_bounds__(256) void bt_blur_cols(const BtJob* __restrict__ jobs,
-                                                               const uint32_t* __restrict__ base,
-                                                               int njobs) {
+// non-negative value, as the reference does). Tiles of 64 columns x TH rows
+// (each thread TH / 4 outputs of one column).
+template <int TH>
+__global__ __launch_bounds__(256) void bt_blur_cols(const BtJob* __restrict__ jobs,
+                                                    const uint32_t* __restrict__ base, int njobs) {
   JOB_PROLOGUE(BT_K_COL);
-  __shared__ float tile[BT_COL_TILE_H + 2 * BT_MAX_TAPS][BT_COL_TILE_W];
+  constexpr uint32_t TW = 64;
+  __shared__ float tile[TH + 2 * BT_MAX_TAPS][TW];
   __shared__ float wmax[4];
   const uint32_t W = J.dw, H = J.dh, K = J.taps;
-  const uint32_t tiles_x = (W + BT_COL_TILE_W - 1u) / BT_COL_TILE_W;
-  const uint32_t x0 = (lb % tiles_x) * BT_COL_TILE_W, y0 = (lb / tiles_x) * BT_COL_TILE_H;
+  const uint32_t tiles_x = (W + TW - 1u) / TW;
+  const uint32_t x0 = (lb % tiles_x) * TW, y0 = (lb / tiles_x) * TH;
   const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
   const uint32_t x = x0 + tx;
-  const uint32_t span = BT_COL_TILE_H + 2u * K;
+  const uint32_t span = TH + 2u * K;
   const float* src = J.dens_row;
   // eight independent loads in flight per thread before the LDS stores
   constexpr uint32_t B = 8;
@@ -1265,7 +1298,7 @@ extern "C" __global__ __launch_bounds__(256) void bt_blur_cols(const BtJob* __re
   __syncthreads();
   const float* wt = J.wt;
   float peak = 0.0f;
-  for (uint32_t r = ty; r < BT_COL_TILE_H; r += 4u) {
+  for (uint32_t r = ty; r < TH; r += 4u) {
     const uint32_t y = y0 + r;
     if (x < W && y < H) {
       const uint32_t c = K + r;
@@ -1672,25 +1705,83 @@ void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base
 void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_scatter, L256(grid), jobs, base, nj);
 }
-uint32_t neighbor_mean_blocks_per_cu() {
+template <int GL, int W>
+static uint32_t nm_blocks_per_cu() {
   int a = 0, b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, bt_neighbor_mean, 256, 0) != hipSuccess)
-    a = 1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bt_neighbor_mean_heavy, 256, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, bt_neighbor_mean<GL, W>, 64 * W, 0) !=
       hipSuccess)
+    a = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bt_neighbor_mean_heavy<GL, W>, 64 * W,
+                                                   0) != hipSuccess)
     b = 1;
   return (uint32_t)std::max(1, std::min(a, b));
 }
+template <int GL, int W>
+static void nm_launch(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
+                      BtControl* ctl, BtSplit split, unsigned long long samples) {
+  hipLaunchKernelGGL((bt_neighbor_mean<GL, W>), dim3(grid), dim3(64 * W), 0, s, jobs, tasks_in,
+                     ctl, split, samples);
+  hipLaunchKernelGGL((bt_neighbor_mean_heavy<GL, W>), dim3(grid), dim3(64 * W), 0, s, jobs,
+                     tasks_in, ctl, split, samples);
+}
+// Kernel shapes (lanes per group, waves per workgroup), by index.
+#define BT_NM_SHAPES(X) X(0, 16, 4) X(1, 32, 4) X(2, 16, 2) X(3, 32, 2) X(4, 16, 8) X(5, 32, 8)
+uint32_t neighbor_mean_blocks_per_cu(uint32_t shape) {
+  switch (shape) {
+#define BT_X(i, gl, w) \
+  case i:              \
+    return nm_blocks_per_cu<gl, w>();
+    BT_NM_SHAPES(BT_X)
+#undef BT_X
+  }
+  return 1;
+}
+uint32_t neighbor_mean_waves(uint32_t shape) {
+  switch (shape) {
+#define BT_X(i, gl, w) \
+  case i:              \
+    return w;
+    BT_NM_SHAPES(BT_X)
+#undef BT_X
+  }
+  return 4;
+}
 void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
-                   BtControl* ctl, BtSplit split, unsigned long long samples) {
-  hipLaunchKernelGGL(bt_neighbor_mean, L256(grid), jobs, tasks_in, ctl, split, samples);
-  hipLaunchKernelGGL(bt_neighbor_mean_heavy, L256(grid), jobs, tasks_in, ctl, split, samples);
+                   BtControl* ctl, BtSplit split, unsigned long long samples, uint32_t shape) {
+  switch (shape) {
+#define BT_X(i, gl, w)                                           \
+  case i:                                                        \
+    nm_launch<gl, w>(s, grid, jobs, tasks_in, ctl, split, samples); \
+    return;
+    BT_NM_SHAPES(BT_X)
+#undef BT_X
+  }
 }
-void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
-  hipLaunchKernelGGL(bt_blur_rows, L256(grid), jobs, base, nj);
+void row_tile(uint32_t shape, uint32_t* tw, uint32_t* th) {
+  static const uint32_t w[BT_ROW_SHAPE_COUNT] = {64, 128, 32, 256};
+  *tw = w[shape < BT_ROW_SHAPE_COUNT ? shape : 0];
+  *th = 256u / *tw;
 }
-void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
-  hipLaunchKernelGGL(bt_blur_cols, L256(grid), jobs, base, nj);
+uint32_t col_tile_h(uint32_t shape) {
+  static const uint32_t h[BT_COL_SHAPE_COUNT] = {32, 16, 64};
+  return h[shape < BT_COL_SHAPE_COUNT ? shape : 0];
+}
+void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+               uint32_t shape) {
+  switch (shape) {
+    case 1: hipLaunchKernelGGL((bt_blur_rows<128, 2>), L256(grid), jobs, base, nj); return;
+    case 2: hipLaunchKernelGGL((bt_blur_rows<32, 8>), L256(grid), jobs, base, nj); return;
+    case 3: hipLaunchKernelGGL((bt_blur_rows<256, 1>), L256(grid), jobs, base, nj); return;
+    default: hipLaunchKernelGGL((bt_blur_rows<64, 4>), L256(grid), jobs, base, nj); return;
+  }
+}
+void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+               uint32_t shape) {
+  switch (shape) {
+    case 1: hipLaunchKernelGGL((bt_blur_cols<16>), L256(grid), jobs, base, nj); return;
+    case 2: hipLaunchKernelGGL((bt_blur_cols<64>), L256(grid), jobs, base, nj); return;
+    default: hipLaunchKernelGGL((bt_blur_cols<32>), L256(grid), jobs, base, nj); return;
+  }
 }
 void density_clear(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   if (grid) hipLaunchKernelGGL(bt_density_clear, L256(grid), jobs, base, nj);

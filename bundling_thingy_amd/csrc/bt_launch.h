@@ -13,13 +13,27 @@ void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base
            BtControl* ctl, BtTask* tasks_out);
 void scatter(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void pre_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+// Kernel shapes of the neighbor mean (lanes per group x waves per
+// workgroup): 0 = 16 x 4 (the default), 1 = 32 x 4, 2 = 16 x 2, 3 = 32 x 2,
+// 4 = 16 x 8, 5 = 32 x 8. Results are the same under every shape.
+#define BT_NM_SHAPE_COUNT 6u
 // `samples`: the launch's samples, summed over its jobs.
 void neighbor_mean(hipStream_t s, uint32_t grid, const BtJob* jobs, const BtTask* tasks_in,
-                   BtControl* ctl, BtSplit split, unsigned long long samples);
-// Workgroups of the (persistent) neighbor-mean kernels resident per CU.
-uint32_t neighbor_mean_blocks_per_cu();
-void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
-void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
+                   BtControl* ctl, BtSplit split, unsigned long long samples, uint32_t shape);
+// Workgroups of a shape's (persistent) neighbor-mean kernels resident per CU.
+uint32_t neighbor_mean_blocks_per_cu(uint32_t shape);
+uint32_t neighbor_mean_waves(uint32_t shape);
+// Blur tile shapes: rows 0 = 64 x 4 (the default), 1 = 128 x 2, 2 = 32 x 8,
+// 3 = 256 x 1 (columns x rows); columns 0 = 64 x 32 (the default), 1 = 64 x
+// 16, 2 = 64 x 64. Results are the same under every shape.
+#define BT_ROW_SHAPE_COUNT 4u
+#define BT_COL_SHAPE_COUNT 3u
+void row_tile(uint32_t shape, uint32_t* tw, uint32_t* th);
+uint32_t col_tile_h(uint32_t shape);
+void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+               uint32_t shape = 0);
+void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
+               uint32_t shape = 0);
 void normalize(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 // Density of the current records outside an iteration: clear, then splat.
 void density_clear(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);

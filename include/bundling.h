@@ -239,6 +239,31 @@ int bt_profile(bt_ctx* ctx, uint32_t n, double* stage_ms, uint32_t n_stages, dou
 int bt_autotune(bt_ctx* ctx, uint32_t n, const uint32_t* candidates, uint32_t n_candidates,
                 uint32_t* chosen, double* best_ms);
 
+/* Kernel shapes (no counterpart in the reference: its workgroup sizes are
+ * fixed, neighbor_mean.rs:4, blur.wgsl:13-35, normalization.wgsl:5). Per
+ * schedule entry, three shape indices: the neighbor mean's lanes per group
+ * x waves per workgroup (0 = 16 x 4, 1 = 32 x 4, 2 = 16 x 2, 3 = 32 x 2,
+ * 4 = 16 x 8, 5 = 32 x 8), the row blur's tile (0 = 64 x 4, 1 = 128 x 2,
+ * 2 = 32 x 8, 3 = 256 x 1 pixels) and the column blur's (0 = 64 x 32,
+ * 1 = 64 x 16, 2 = 64 x 64). Results are bit-identical under every shape.
+ * A launch over several jobs runs the first job's shapes. */
+#define BT_NM_SHAPES 6u
+#define BT_ROW_SHAPES 4u
+#define BT_COL_SHAPES 3u
+/* Autotuner of the shapes: for each of the first n iterations, times each
+ * stage under every shape of that stage from the same state (the other
+ * stages at their current choice), keeps the fastest for every job
+ * (shapes[3 i + k], its stage time in stage_ms[3 i + k]; either may be
+ * NULL) and uses it from then on, including in hipGraphs. Leaves the jobs
+ * rewound. Run after bt_autotune (the grid split) when both are wanted. */
+int bt_autotune_shapes(bt_ctx* ctx, uint32_t n, uint32_t* shapes, double* stage_ms);
+/* Sets the shapes of schedule entry `iteration` of every job (all entries
+ * when iteration is 0xffffffff). */
+int bt_set_shapes(bt_ctx* ctx, uint32_t iteration, uint32_t nm_shape, uint32_t row_shape,
+                  uint32_t col_shape);
+/* The shapes job `job` runs iteration `iteration` with. */
+int bt_get_shapes(bt_ctx* ctx, uint32_t job, uint32_t iteration, uint32_t* shapes3);
+
 /* Measurement helper (no counterpart in the reference): copies `bytes` bytes
  * device-to-device on the context's stream (HIP's copy kernel), so profiler
  * byte counters can be scaled against a transfer of known size. *ms receives

@@ -584,3 +584,52 @@ def test_density_binary16_readback(bt, orc, case):
             np.testing.assert_array_equal(half, f32.astype(np.float16).view(np.uint16))
             assert np.any((half > 0) & (half < 0x0400)) or case == "reference"
             gpu.iterate()
+
+
+def _run_shapes(bt, g, opts, shapes, iters=3, graph=False):
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(g)
+        if shapes is not None:
+            gpu.set_shapes(*shapes)
+        gpu.iterate(iters, graph=graph)
+        info = gpu.info()
+        return (gpu.points()[0], gpu.density(0), gpu.density(2), gpu.density(3),
+                info.split_tasks)
+
+
+@pytest.mark.parametrize("shape", [(nm, 0, 0) for nm in range(6)] +
+                         [(0, row, 0) for row in range(1, 4)] + [(0, 0, col) for col in (1, 2)])
+def test_kernel_shapes_do_not_change_results(bt, cpu_case, shape):
+    """Every neighbor-mean shape (lanes per group x waves per workgroup) and
+    blur tile gives the default shapes' bits: positions, counts, blurred and
+    normalized density, on the plain CPU case (plain and grouped streams) and
+    on square cells as wide as the radius (tasks split across waves)."""
+    cases = [workloads.plain_cpu_options(),
+             bt.Options(spacing=0.005, radius=0.1, sigma=6.0, density=512, cell_split=1,
+                        x_split=1)]
+    for k, opts in enumerate(cases):
+        want = _run_shapes(bt, cpu_case, opts, None)
+        got = _run_shapes(bt, cpu_case, opts, shape)
+        for a, b in zip(got[:4], want[:4]):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32),
+                                          err_msg=f"shape {shape} case {k}")
+        if k == 1:
+            assert got[4] > 0  # the heavy-task kernel ran under this shape
+
+
+def test_autotune_shapes_keeps_results(bt, cpu_case):
+    """The shape autotuner picks one shape per stage per iteration; the
+    schedule replayed as a hipGraph under its choices gives the default
+    bits."""
+    opts = workloads.plain_cpu_options()
+    want = _run_shapes(bt, cpu_case, opts, None, iters=4)
+    with bt.Gpu(opts) as gpu:
+        gpu.upload(cpu_case)
+        shapes, ms = gpu.autotune_shapes(4)
+        assert shapes.shape == (4, 3) and np.all(shapes[:, 0] < 6) and np.all(ms[:, 0] > 0)
+        assert gpu.job_samples(0)[1] == 0  # rewound
+        for i in range(4):
+            assert gpu.shapes(i) == tuple(int(v) for v in shapes[i])
+        gpu.iterate(4, graph=True)
+        np.testing.assert_array_equal(gpu.points()[0].view(np.uint32), want[0].view(np.uint32))
+        np.testing.assert_array_equal(gpu.density(3).view(np.uint32), want[3].view(np.uint32))
