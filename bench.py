@@ -140,10 +140,20 @@ def batched_section(bt, workloads, n_graphs, iters=30):
     plain_s = time.perf_counter() - t0
     updates = iters * sum(samples)
     log(f"batched: plain per-graph {plain_s:.2f} s")
-    # batched form
+    # batched form: kernel shapes autotuned per iteration on the whole batch
+    # (outside the timed region, like the plain case's autotuning)
+    gpu.upload_batch(graphs, opts)
+    t = time.perf_counter()
+    shapes, _ = gpu.autotune_shapes(iters)
+    tune_s = time.perf_counter() - t
+    log(f"batched: shapes {shapes.tolist()} in {tune_s:.2f} s")
     t0 = time.perf_counter()
     s = gpu.upload_batch(graphs, opts)
     up_s = time.perf_counter() - t0
+    t = time.perf_counter()
+    for i in range(iters):
+        gpu.set_shapes(*(int(v) for v in shapes[i]), iteration=i)
+    up_s += time.perf_counter() - t
     t1 = time.perf_counter()
     gpu.iterate(iters, graph=True)  # first call captures + instantiates the hipGraph
     first_s = time.perf_counter() - t1
@@ -165,6 +175,7 @@ def batched_section(bt, workloads, n_graphs, iters=30):
                       "job_p50_ms": pct(lat, 50), "job_p99_ms": pct(lat, 99)},
             "batched": {"updates_per_s": updates / batch_s, "seconds": batch_s,
                         "upload_s": up_s, "schedule_s": run_s, "readback_s": read_s,
+                        "shapes": shapes.tolist(), "autotune_s": tune_s,
                         "graph_build_s": first_s - run_s,
                         "job_p50_ms": batch_s * 1e3, "job_p99_ms": batch_s * 1e3},
             "speedup": plain_s / batch_s}
@@ -183,6 +194,7 @@ def large_section(bt, workloads, steps=2, iters=50):
     upload_ms = (time.perf_counter() - t) * 1e3
     t = time.perf_counter()
     chosen, _ = gpu.autotune(iters, TUNE_CANDIDATES)
+    shapes, _ = gpu.autotune_shapes(iters)
     tune_s = time.perf_counter() - t
     gpu.rewind()
     gpu.iterate(iters, graph=True)
@@ -198,7 +210,7 @@ def large_section(bt, workloads, steps=2, iters=50):
     return {"edges": graph.edge_count, "samples": n, "iterations": iters,
             "density": [2048, 2048], "updates_per_s": iters * n / sec,
             "ms_per_schedule": sec * 1e3, "upload_ms": upload_ms, "autotune_s": tune_s,
-            "splits": chosen.tolist()}
+            "splits": chosen.tolist(), "shapes": shapes.tolist()}
 
 
 def render_section(bt, gpu, opts, reps=5):
@@ -446,8 +458,17 @@ def main():
     if not args.cell_split:
         t = time.perf_counter()
         chosen, best = gpu.autotune(ITERS, TUNE_CANDIDATES)
-        tune = {"splits": chosen.tolist(), "seconds": time.perf_counter() - t}
-        log(f"autotune: {chosen.tolist()} in {tune['seconds']:.2f} s")
+        t_split = time.perf_counter() - t
+        log(f"autotune: splits {chosen.tolist()} in {t_split:.2f} s")
+        t = time.perf_counter()
+        shapes, shape_ms = gpu.autotune_shapes(ITERS)
+        tune = {"splits": chosen.tolist(), "split_seconds": t_split,
+                "shapes": shapes.tolist(), "shape_seconds": time.perf_counter() - t,
+                "shapes_key": "per iteration: [neighbor-mean shape (0 = 16 lanes x 4 waves, "
+                              "1 = 32x4, 2 = 16x2, 3 = 32x2, 4 = 16x8, 5 = 32x8), row blur tile "
+                              "(0 = 64x4, 1 = 128x2, 2 = 32x8, 3 = 256x1), column blur tile "
+                              "(0 = 64x32, 1 = 64x16, 2 = 64x64)]"}
+        log(f"autotune: shapes {shapes.tolist()} in {tune['shape_seconds']:.2f} s")
 
     # warmup (also builds the hipGraph of the schedule)
     t = time.perf_counter()
