@@ -200,8 +200,8 @@ class Engine {
     bool pending = false;
   };
   std::vector<Pinned> pinned_;
-  // pinned staging of upload inputs: a ring, so an asynchronous upload never
-  // refills a buffer whose copy is still queued
+  // pinned staging of upload inputs (and of image readbacks): a ring, so an
+  // asynchronous upload never refills a buffer whose copy is still queued
   std::vector<Pinned> up_pinned_;
   uint32_t up_next_ = 0;
   void* upload_staging(size_t bytes, Pinned** slot);

@@ -2,10 +2,13 @@
 // and the endpoint dots, as compute kernels:
 //   count    one thread per sample: the quad of the segment (i, i+1), its
 //            MSAA coverage per pixel, and how many fragments it makes
-//   scan     exclusive sum of the counts (fragment offsets in draw order)
+//   scan     exclusive sum of the counts (fragment offsets in draw order):
+//            workgroup sums, a scan of those, then each tile's local scan
 //   emit     the same walk, writing {pixel | coverage << 24, segment}
-//   sort     stable radix sort on the pixel bits: each pixel's fragments
-//            end up together, still in draw (segment) order
+//   sort     stable LSD radix sort on the pixel bits, 10 bits per pass:
+//            per-tile digit histograms, their scan in digit-major order,
+//            then a scatter that ranks equal digits with wave ballots, so
+//            each pixel's fragments end up together, still in draw order
 //   counts   density colouring (one colour per pixel): fragments per sample,
 //            one atomic per run of a pixel's fragments in a wave
 //   runs     edge colouring: first and last fragment of every pixel
@@ -24,9 +27,8 @@
 // functions are positive, or zero on a top or left edge. Sample positions are
 // the standard 2x / 4x locations. Fragment shading happens once per pixel at
 // its center; endpoint dots are shaded per sample (interpolate(sample)).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
-
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -504,6 +506,174 @@ __global__ __launch_bounds__(256) void rs_resolve(Frame f, const uint32_t* keys,
 
 uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255u) / 256u); }
 
+// ------------------------------------------------------------------ scan
+// Exclusive scan of n u32 counts into u32 offsets, the total in 64 bits.
+constexpr uint32_t kScanTile = 4096;  // items per workgroup (256 threads x 16)
+
+__device__ __forceinline__ unsigned long long rs_wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long rs_wave_incl(unsigned long long v) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long t = __shfl_up(v, d, 64);
+    if ((int)lane >= d) v += t;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void rs_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                      unsigned long long* __restrict__ bsum) {
+  __shared__ unsigned long long part[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  unsigned long long s = 0;
+  for (uint32_t k = threadIdx.x; k < kScanTile; k += 256u)
+    if (base + k < n) s += in[base + k];
+  s = rs_wave_sum(s);
+  if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) bsum[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// One workgroup: exclusive scan of the nb tile sums in place, the total to
+// *total.
+__global__ __launch_bounds__(1024) void rs_scan_bsum(unsigned long long* __restrict__ bsum,
+                                                     uint32_t nb,
+                                                     unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long wtot[16];
+  __shared__ unsigned long long carry;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < nb; c0 += 1024u) {
+    const uint32_t i = c0 + threadIdx.x;
+    const unsigned long long v = i < nb ? bsum[i] : 0ull;
+    const unsigned long long incl = rs_wave_incl(v);
+    if (lane == 63u) wtot[w] = incl;
+    __syncthreads();
+    unsigned long long before = carry;
+    for (uint32_t k = 0; k < w; ++k) before += wtot[k];
+    if (i < nb) bsum[i] = before + incl - v;
+    __syncthreads();  // every thread has read carry and wtot
+    if (threadIdx.x == 0) {
+      unsigned long long t = carry;
+      for (uint32_t k = 0; k < 16u; ++k) t += wtot[k];
+      carry = t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// Each thread scans 16 consecutive items; the threads' sums are scanned
+// across the workgroup and offset by the tile's scanned sum.
+__global__ __launch_bounds__(256) void rs_scan_apply(const uint32_t* __restrict__ in,
+                                                     uint32_t* __restrict__ out, uint64_t n,
+                                                     const unsigned long long* __restrict__ bsum) {
+  __shared__ unsigned long long wtot[4];
+  constexpr uint32_t K = kScanTile / 256u;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint64_t first = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * K;
+  uint32_t v[K];
+  unsigned long long s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) {
+    v[k] = first + k < n ? in[first + k] : 0u;
+    s += v[k];
+  }
+  const unsigned long long incl = rs_wave_incl(s);
+  if (lane == 63u) wtot[w] = incl;
+  __syncthreads();
+  unsigned long long at = bsum[blockIdx.x] + incl - s;
+  for (uint32_t k = 0; k < w; ++k) at += wtot[k];
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) {
+    if (first + k < n) out[first + k] = (uint32_t)at;
+    at += v[k];
+  }
+}
+
+// ------------------------------------------------------------------ sort
+constexpr uint32_t kSortTile = 4096;   // keys per workgroup per pass (256 threads x 16)
+constexpr uint32_t kDigitBits = 10;    // bits per pass: 2 passes for a 2^20-pixel target
+constexpr uint32_t kDigits = 1u << kDigitBits;
+
+// Digit counts of each tile, written digit-major (hist[d * nb + tile]) so
+// that one exclusive scan gives every (digit, tile) its output offset; the
+// entry after the last is zeroed for the scan's total.
+__global__ __launch_bounds__(256) void rs_sort_hist(const uint32_t* __restrict__ keys, uint32_t m,
+                                                    uint32_t shift, uint32_t dmask, uint32_t nb,
+                                                    uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kDigits];
+  for (uint32_t d = threadIdx.x; d < kDigits; d += 256u) h[d] = 0u;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (uint32_t k = threadIdx.x; k < kSortTile; k += 256u) {
+    const uint32_t i = base + k;
+    if (i < m) atomicAdd(&h[(keys[i] >> shift) & dmask], 1u);
+  }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < kDigits; d += 256u) hist[d * nb + blockIdx.x] = h[d];
+  if (blockIdx.x == 0 && threadIdx.x == 0) hist[kDigits * nb] = 0u;
+}
+
+// Stable scatter of one pass: the tile in rounds of 256 keys (wave w takes
+// keys [64 w, 64 w + 64) of the round, in order). Within a wave the lanes
+// holding the same digit are found with one ballot per digit bit and ranked
+// by lane; the lowest of them (the leader) publishes the wave's count of
+// that digit; a key's position is its digit's base plus the counts of the
+// waves before its own plus its rank; then each leader advances its
+// digit's base and clears its count.
+__global__ __launch_bounds__(256) void rs_sort_scatter(const uint32_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin,
+                                                       uint32_t* __restrict__ kout,
+                                                       uint32_t* __restrict__ vout, uint32_t m,
+                                                       uint32_t shift, uint32_t dmask,
+                                                       uint32_t nb,
+                                                       const uint32_t* __restrict__ offs) {
+  __shared__ uint32_t base[kDigits];
+  __shared__ uint32_t cnt[4][kDigits];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  for (uint32_t d = threadIdx.x; d < kDigits; d += 256u) {
+    base[d] = offs[d * nb + blockIdx.x];
+    cnt[0][d] = cnt[1][d] = cnt[2][d] = cnt[3][d] = 0u;
+  }
+  __syncthreads();
+  const uint32_t tile = blockIdx.x * kSortTile;
+  for (uint32_t r = 0; r < kSortTile / 256u; ++r) {
+    const uint32_t i = tile + r * 256u + w * 64u + lane;
+    const bool valid = i < m;
+    const uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
+    const uint32_t d = (key >> shift) & dmask;
+    uint64_t peers = __builtin_amdgcn_ballot_w64(valid);
+#pragma unroll
+    for (uint32_t b = 0; b < kDigitBits; ++b) {
+      const uint64_t ones = __builtin_amdgcn_ballot_w64(((d >> b) & 1u) != 0u);
+      peers &= ((d >> b) & 1u) ? ones : ~ones;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    const bool leader = valid && rank == 0u;
+    if (leader) cnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = base[d] + rank;
+      for (uint32_t k = 0; k < w; ++k) pos += cnt[k][d];
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    if (leader) {
+      atomicAdd(&base[d], (uint32_t)__popcll(peers));
+      cnt[w][d] = 0u;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 void Engine::render_size(uint32_t* w, uint32_t* h, uint32_t job_i) {
@@ -569,15 +739,21 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
     DevBuf total;
     total.pool = pool_.get();
     total.alloc(8);
+    auto scan_launch = [&](const uint32_t* in, uint32_t* out, uint64_t items) {
+      const uint64_t nb = (items + kScanTile - 1) / kScanTile;
+      if (nb > 0xffffffffull) fail(BT_E_CAPACITY, "scan exceeds 2^32 tiles");
+      temp.reserve(std::max<size_t>(nb * 8, 256));
+      unsigned long long* bsum = temp.as<unsigned long long>();
+      hipLaunchKernelGGL(rs_scan_reduce, dim3((uint32_t)nb), dim3(256), 0, stream_, in, items,
+                         bsum);
+      hipLaunchKernelGGL(rs_scan_bsum, dim3(1), dim3(1024), 0, stream_, bsum, (uint32_t)nb,
+                         total.as<unsigned long long>());
+      hipLaunchKernelGGL(rs_scan_apply, dim3((uint32_t)nb), dim3(256), 0, stream_, in, out,
+                         items, bsum);
+      HIPCHK(hipGetLastError());
+    };
     auto scan = [&](const uint32_t* in, uint32_t* out, uint64_t count) -> uint64_t {
-      size_t tb = 0, rb = 0;
-      HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, count + 1, stream_));
-      HIPCHK(hipcub::DeviceReduce::Sum(nullptr, rb, in, total.as<uint64_t>(), count + 1,
-                                       stream_));
-      temp.reserve(std::max<size_t>(std::max(tb, rb), 256));
-      HIPCHK(hipcub::DeviceScan::ExclusiveSum(temp.p, tb, in, out, count + 1, stream_));
-      HIPCHK(hipcub::DeviceReduce::Sum(temp.p, rb, in, total.as<uint64_t>(), count + 1,
-                                       stream_));
+      scan_launch(in, out, count + 1);
       uint64_t t = 0;
       HIPCHK(hipMemcpyAsync(&t, total.p, 8, hipMemcpyDeviceToHost, stream_));
       sync();
@@ -617,16 +793,31 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
                          keys.as<uint32_t>(), vals.as<uint32_t>());
       HIPCHK(hipGetLastError());
       // stable sort on the pixel bits: per pixel, fragments stay in draw order
-      int bits = 1;
+      uint32_t bits = 1;
       while ((1ull << bits) < npix) ++bits;
-      hipcub::DoubleBuffer<uint32_t> dk(keys.as<uint32_t>(), keys2.as<uint32_t>());
-      hipcub::DoubleBuffer<uint32_t> dv(vals.as<uint32_t>(), vals2.as<uint32_t>());
-      size_t sb = 0;
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, dk, dv, (int)m, 0, bits, stream_));
-      temp.reserve(sb);
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(temp.p, sb, dk, dv, (int)m, 0, bits, stream_));
-      kkeys = dk.Current();
-      kvals = dv.Current();
+      const uint32_t nb = (m + kSortTile - 1) / kSortTile;
+      DevBuf hist, hoff;
+      hist.pool = pool_.get();
+      hoff.pool = pool_.get();
+      hist.alloc(((size_t)nb * kDigits + 1) * 4);
+      hoff.alloc(((size_t)nb * kDigits + 1) * 4);
+      uint32_t *ka = keys.as<uint32_t>(), *va = vals.as<uint32_t>();
+      uint32_t *kb = keys2.as<uint32_t>(), *vb = vals2.as<uint32_t>();
+      for (uint32_t shift = 0; shift < bits; shift += kDigitBits) {
+        // the last pass's digit stops at the pixel bits (the coverage mask above them is
+        // not part of the key order)
+        const uint32_t dmask = (1u << std::min(kDigitBits, bits - shift)) - 1u;
+        hipLaunchKernelGGL(rs_sort_hist, dim3(nb), dim3(256), 0, stream_, ka, m, shift, dmask,
+                           nb, hist.as<uint32_t>());
+        scan_launch(hist.as<uint32_t>(), hoff.as<uint32_t>(), (uint64_t)nb * kDigits + 1);
+        hipLaunchKernelGGL(rs_sort_scatter, dim3(nb), dim3(256), 0, stream_, ka, va, kb, vb, m,
+                           shift, dmask, nb, hoff.as<uint32_t>());
+        HIPCHK(hipGetLastError());
+        std::swap(ka, kb);
+        std::swap(va, vb);
+      }
+      kkeys = ka;
+      kvals = va;
       if (ro.color_by == BT_COLOR_DENSITY) {
         scount.alloc(npix * 16);
         HIPCHK(hipMemsetAsync(scount.p, 0, npix * 16, stream_));
@@ -658,8 +849,13 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
   hipLaunchKernelGGL(rs_resolve, dim3(blocks(npix)), dim3(256), 0, stream_, f, kkeys, kvals,
                      kfirst, klast, kscount, kdots, image.as<uint32_t>());
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(rgba, image.p, npix * 4, hipMemcpyDeviceToHost, stream_));
+  // one DMA into pinned memory, then a host copy (a pageable destination
+  // would be copied in staged chunks)
+  Pinned* slot = nullptr;
+  void* pin = upload_staging(npix * 4, &slot);
+  HIPCHK(hipMemcpyAsync(pin, image.p, npix * 4, hipMemcpyDeviceToHost, stream_));
   sync();
+  std::memcpy(rgba, pin, npix * 4);
 }
 
 }  // namespace bt
