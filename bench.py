@@ -376,8 +376,13 @@ def served_section(bt, workloads, n_jobs, rate_hz=200.0):
         out["updates"] = updates
         out["offered_updates_per_s"] = updates / arrivals[-1]
         plain_cap = updates / plain_s
+        # each job's service time alone on an idle GPU: a floor under any
+        # policy's latency for that job
+        iso = [service[k] * 1e3 for k in sorted(small)]
         sec = {"plain": {"capacity_updates_per_s": plain_cap, "seconds": plain_s,
-                         **lat_stats(enumerate(_fifo_queue_latency(arrivals, service)))}}
+                         **lat_stats(enumerate(_fifo_queue_latency(arrivals, service))),
+                         "isolated_p50_ms_le20k_edges": pct(iso, 50),
+                         "isolated_p99_ms_le20k_edges": pct(iso, 99)}}
         log(f"served [{dname}]: plain per-job {plain_s:.2f} s, {plain_cap:.3e} updates/s")
         for name, policy, tick in SERVER_POLICIES:
             cap_s, _, cap_ticks, _ = serve(bt, jobs, getattr(bt.Server, policy), tick, flags,
@@ -467,7 +472,8 @@ def main():
                 "shapes_key": "per iteration: [neighbor-mean shape (0 = 16 lanes x 4 waves, "
                               "1 = 32x4, 2 = 16x2, 3 = 32x2, 4 = 16x8, 5 = 32x8), row blur tile "
                               "(0 = 64x4, 1 = 128x2, 2 = 32x8, 3 = 256x1), column blur tile "
-                              "(0 = 64x32, 1 = 64x16, 2 = 64x64)]"}
+                              "(0 = 64x32, 1 = 64x16, 2 = 64x64; 3 = 64x32 and 4 = 64x16 fused "
+                              "row + column kernel)]"}
         log(f"autotune: shapes {shapes.tolist()} in {tune['shape_seconds']:.2f} s")
 
     # warmup (also builds the hipGraph of the schedule)
@@ -482,8 +488,10 @@ def main():
         gpu.rewind()
         t = time.perf_counter()
         gpu.iterate(ITERS, graph=True)   # synchronizes the stream before returning
-        times.append(time.perf_counter() - t)
-        log(f"step {s}: {times[-1] * 1e3:.2f} ms")
+        t1 = time.perf_counter()
+        times.append(t1 - t)
+        # the window on the monotonic clock, for matching profiler traces
+        log(f"step {s}: {times[-1] * 1e3:.2f} ms (window {t:.4f} {t1:.4f})")
     total = float(sum(times))
     ms = total / args.steps * 1e3
     value = ITERS * n * args.steps / total

@@ -10,6 +10,7 @@
 
 #define BT_FIXED_BIT 0x80000000u   // sample is a path endpoint (never moves)
 #define BT_MAX_TAPS 64             // blur half-width limit (sigma <= 21 px)
+#define BT_FUSED_TAPS 32           // widest blur the fused row + column kernel takes
 #define BT_FIX_ONE 16777216.0f     // 2^24: fixed-point quantum of neighbor sums
 #define BT_TASK_POINTS 64          // query points per neighbor-mean task (one wave)
 

@@ -307,8 +307,13 @@ Plan Engine::make_plan(const std::vector<Job*>& js, const std::vector<uint32_t>&
 Plan Engine::make_plan_of(const std::vector<Job*>& js, const std::vector<BtJob>& entries,
                           uint32_t kshape) const {
   Plan p;
-  p.kshape = kshape;
   const uint32_t nj = (uint32_t)js.size();
+  if (btk::col_fused((kshape >> 16) & 0xffu)) {
+    uint32_t taps = 0;
+    for (const BtJob& e : entries) taps = std::max(taps, e.taps);
+    if (taps > BT_FUSED_TAPS) kshape &= ~(0xffu << 16);  // the two-kernel blur
+  }
+  p.kshape = kshape;
   p.njobs = nj;
   p.base_off = align_up(nj * sizeof(BtJob), 256);
   p.host.assign(p.base_off + (size_t)BT_K_COUNT * (nj + 1) * 4, 0);
@@ -429,9 +434,11 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
                      shape);
   mark(6);
   if (density && p.grid[BT_K_ROW]) {
-    btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj, (p.kshape >> 8) & 0xffu);
+    const uint32_t col = (p.kshape >> 16) & 0xffu;
+    if (!btk::col_fused(col))
+      btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj, (p.kshape >> 8) & 0xffu);
     mark(7);
-    btk::blur_cols(stream_, p.grid[BT_K_COL], J, B(BT_K_COL), (int)nj, (p.kshape >> 16) & 0xffu);
+    btk::blur_cols(stream_, p.grid[BT_K_COL], J, B(BT_K_COL), (int)nj, col);
     mark(8);
     btk::normalize(stream_, p.grid[BT_K_NORM], J, B(BT_K_NORM), (int)nj);
     mark(9);
@@ -1020,8 +1027,9 @@ void Engine::autotune_shapes(uint32_t n, uint32_t* shapes, double* stage_ms) {
           for (Job* j : js) j->kshape[j->entry(i)] = v[0] | (v[1] << 8) | (v[2] << 16);
           double t3[3];
           run_one(t3);
-          if (t3[st] < best_t[st]) {
-            best_t[st] = t3[st];
+          const double t = st == 2 ? t3[1] + t3[2] : t3[st];
+          if (t < best_t[st]) {
+            best_t[st] = t;
             best[st] = c;
           }
         }
@@ -1033,7 +1041,7 @@ void Engine::autotune_shapes(uint32_t n, uint32_t* shapes, double* stage_ms) {
       for (Job* j : js) j->advanced(1);
       for (uint32_t k = 0; k < 3; ++k) {
         if (shapes) shapes[3 * i + k] = best[k];
-        if (stage_ms) stage_ms[3 * i + k] = best_t[k] < 1e300 ? best_t[k] : 0.0;
+        if (stage_ms) stage_ms[3 * i + k] = k == 0 ? best_t[0] : density ? t3[k] : 0.0;
       }
     }
   } catch (...) {

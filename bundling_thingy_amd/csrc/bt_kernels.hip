@@ -236,8 +236,9 @@ extern "C" __global__ __launch_bounds__(256) void bt_bin(const BtJob* __restrict
 }
 
 // -------------------------------------------------------------- cell scan
-// Three-phase exclusive scan of each job's n_cells + 1 counters (the last one
-// is zero, so it becomes the total). Replaces reserve_cells
+// Two-phase exclusive scan of each job's n_cells + 1 counters (the last one
+// is zero, so it becomes the total): block totals, then each block's local
+// scan offset by the sum of the totals before it. Replaces reserve_cells
 // (spatial_grid.wgsl:86-125), whose atomically ordered block bases are not
 // deterministic.
 template <typename T>
@@ -262,32 +263,25 @@ __device__ __forceinline__ void scan_reduce_body(const T* src, uint32_t n, uint3
   if (threadIdx.x == 0) bsum[lb] = total;
 }
 
+// Sum of the block totals bsum[0 .. lb) over a 256-thread block: the
+// block's offset (the totals are few, so each block adds its predecessors'
+// instead of a separate scan of the totals).
 template <typename T>
-__device__ __forceinline__ void scan_bsum_body(T* bsum, uint32_t nb) {
-  // nb <= BT_SCAN_ITEMS
-  __shared__ T lds[BT_SCAN_ITEMS];
-  __shared__ T part[4];
-  load_block(bsum, nb, 0u, lds);
-  T v[16];
+__device__ __forceinline__ T block_offset(const T* bsum, uint32_t lb, T* part) {
   T s = T(0);
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    v[k] = s;
-    s += lds[threadIdx.x * 16 + k];
-  }
+  for (uint32_t k = threadIdx.x; k < lb; k += 256u) s += bsum[k];
   T total;
-  T before = block_excl_scan(s, part, &total);
-#pragma unroll
-  for (int k = 0; k < 16; ++k) lds[threadIdx.x * 16 + k] = before + v[k];
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k < nb; k += 256u) bsum[k] = lds[k];
+  (void)block_excl_scan(s, part, &total);
+  return total;
 }
 
 template <typename T>
 __device__ __forceinline__ void scan_apply_body(T* data, uint32_t n, uint32_t lb, const T* bsum) {
   __shared__ T lds[BT_SCAN_ITEMS];
   __shared__ T part[4];
+  __shared__ T opart[4];
   const uint32_t first = lb * BT_SCAN_ITEMS;
+  const T offset = block_offset(bsum, lb, opart);
   load_block(data, n, first, lds);
   T v[16];
   T s = T(0);
@@ -297,7 +291,7 @@ __device__ __forceinline__ void scan_apply_body(T* data, uint32_t n, uint32_t lb
     s += lds[threadIdx.x * 16 + k];
   }
   T total;
-  T before = block_excl_scan(s, part, &total) + bsum[lb];
+  T before = block_excl_scan(s, part, &total) + offset;
 #pragma unroll
   for (int k = 0; k < 16; ++k) lds[threadIdx.x * 16 + k] = before + v[k];
   __syncthreads();
@@ -311,13 +305,6 @@ extern "C" __global__ __launch_bounds__(256) void bt_cell_scan_reduce(
     const BtJob* __restrict__ jobs, const uint32_t* __restrict__ base, int njobs) {
   JOB_PROLOGUE(BT_K_SCAN);
   scan_reduce_body<uint32_t>(J.cells, J.n_cells + 1u, lb, J.bsum);
-}
-
-extern "C" __global__ __launch_bounds__(256) void bt_cell_scan_bsum(
-    const BtJob* __restrict__ jobs, int njobs) {
-  const BtJob& J = jobs[blockIdx.x];
-  const uint32_t nb = (J.n_cells + 1u + BT_SCAN_ITEMS - 1u) / BT_SCAN_ITEMS;
-  scan_bsum_body<uint32_t>(J.bsum, nb);
 }
 
 extern "C" __global__ __launch_bounds__(256) void bt_cell_scan_apply(
@@ -415,9 +402,9 @@ extern "C" __global__ __launch_bounds__(256) void bt_scatter(const BtJob* __rest
 // Exclusive prefix sums of the sorted fixed-point coordinates, both axes in
 // one pass (two's complement int64 in u64 lanes), so any run of sorted
 // samples sums in O(1). The coordinates are read from the sorted records
-// (their upper 8 bytes); entry n is the total. Three phases over blocks of
-// BT_SCAN_ITEMS samples: block totals, a scan of the totals (interleaved x, y
-// in bsum), and the block-local scan plus the block's offset.
+// (their upper 8 bytes); entry n is the total. Two phases over blocks of
+// BT_SCAN_ITEMS samples: block totals (x, y interleaved in bsum), then the
+// block-local scan plus the sum of the totals before the block.
 __device__ __forceinline__ int2 qf_of(const BtJob& J, uint32_t i) {
   if (i >= J.n) return make_int2(0, 0);
   const int2* q = reinterpret_cast<const int2*>(J.sorted);
@@ -450,39 +437,6 @@ extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_reduce(
   }
 }
 
-extern "C" __global__ __launch_bounds__(256) void bt_pre_scan_bsum(const BtJob* __restrict__ jobs,
-                                                                   int njobs) {
-  const BtJob& J = jobs[blockIdx.x];
-  const uint32_t nb = (J.n + 1u + BT_SCAN_ITEMS - 1u) / BT_SCAN_ITEMS;
-  // nb <= BT_SCAN_ITEMS blocks, x and y interleaved in bsum
-  __shared__ long long part[8];
-  unsigned long long* bs = reinterpret_cast<unsigned long long*>(J.bsum);
-  constexpr uint32_t K = BT_SCAN_ITEMS / 256u;  // blocks per thread
-  long long vx[K], vy[K];
-  long long tx = 0, ty = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < K; ++k) {
-    const uint32_t blk = threadIdx.x * K + k;
-    vx[k] = tx;
-    vy[k] = ty;
-    if (blk < nb) {
-      tx += (long long)bs[2u * blk];
-      ty += (long long)bs[2u * blk + 1u];
-    }
-  }
-  long long totx, toty;
-  const long long bx = block_excl_scan(tx, part, &totx);
-  const long long by = block_excl_scan(ty, part + 4, &toty);
-#pragma unroll
-  for (uint32_t k = 0; k < K; ++k) {
-    const uint32_t blk = threadIdx.x * K + k;
-    if (blk < nb) {
-      bs[2u * blk] = (unsigned long long)(bx + vx[k]);
-      bs[2u * blk + 1u] = (unsigned long long)(by + vy[k]);
-    }
-  }
-}
-
 // 1024 threads per block: each of the 16 waves scans 256 consecutive samples
 // as 4 chunks of 64 (lane l holds samples c * 64 + l), so loads and the int64
 // stores are coalesced and enough waves are resident to keep HBM busy; the
@@ -494,8 +448,22 @@ extern "C" __global__ __launch_bounds__(1024) void bt_pre_scan_apply(
   constexpr uint32_t WAVES = BT_PRE_APPLY_THREADS / 64u;
   constexpr uint32_t PER_WAVE = BT_SCAN_ITEMS / WAVES, CHUNKS = PER_WAVE / 64u;
   __shared__ long long part[2 * WAVES];
+  __shared__ long long opart[2 * WAVES];
   const uint32_t lane = LANE(), wave = threadIdx.x >> 6;
   const uint32_t w0 = lb * BT_SCAN_ITEMS + wave * PER_WAVE;
+  // the block's offset: the totals of the blocks before it
+  const unsigned long long* bs = reinterpret_cast<const unsigned long long*>(J.bsum);
+  long long sx = 0, sy = 0;
+  for (uint32_t k = threadIdx.x; k < lb; k += BT_PRE_APPLY_THREADS) {
+    sx += (long long)bs[2u * k];
+    sy += (long long)bs[2u * k + 1u];
+  }
+  sx = wave_sum64(sx);
+  sy = wave_sum64(sy);
+  if (lane == 0) {
+    opart[wave] = sx;
+    opart[WAVES + wave] = sy;
+  }
   long long ex[CHUNKS], ey[CHUNKS];
   long long rx = 0, ry = 0;
 #pragma unroll
@@ -513,8 +481,11 @@ extern "C" __global__ __launch_bounds__(1024) void bt_pre_scan_apply(
     part[WAVES + wave] = ry;
   }
   __syncthreads();
-  const unsigned long long* bs = reinterpret_cast<const unsigned long long*>(J.bsum);
-  long long ox = (long long)bs[2u * lb], oy = (long long)bs[2u * lb + 1u];
+  long long ox = 0, oy = 0;
+  for (uint32_t w = 0; w < WAVES; ++w) {
+    ox += opart[w];
+    oy += opart[WAVES + w];
+  }
   for (uint32_t w = 0; w < wave; ++w) {
     ox += part[w];
     oy += part[WAVES + w];
@@ -1262,6 +1233,42 @@ __global__ __launch_bounds__(256) void bt_blur_rows(const BtJob* __restrict__ jo
   }
 }
 
+// The column pass over a tile of 64 columns in LDS whose row c holds image
+// row y0 - K + c: thread (tx, ty) computes rows ty, ty + 4, ... of column
+// tx. Its TH / 4 outputs advance together through the taps, so their LDS
+// reads overlap; each output still adds its taps in the order
+// v = t[c]*w0, v += (t[c-k] + t[c+k]) * w[k], k = 1..K. Returns the thread's
+// largest in-image value (0 if none).
+template <int TH, int ROWS, int COLS>
+__device__ __forceinline__ float col_pass(float (&tile)[ROWS][COLS], const BtJob& J, uint32_t x,
+                                          uint32_t y0, uint32_t tx, uint32_t ty) {
+  constexpr uint32_t Q = TH / 4;
+  const uint32_t W = J.dw, H = J.dh, K = J.taps;
+  const float* wt = J.wt;
+  float v[Q];
+  const float w0 = ld_uniform(wt, 0u);
+#pragma unroll
+  for (uint32_t q = 0; q < Q; ++q) v[q] = tile[K + ty + 4u * q][tx] * w0;
+  for (uint32_t k = 1; k <= K; ++k) {
+    const float wk = ld_uniform(wt, k);
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q) {
+      const uint32_t c = K + ty + 4u * q;
+      v[q] = v[q] + (tile[c - k][tx] + tile[c + k][tx]) * wk;
+    }
+  }
+  float peak = 0.0f;
+#pragma unroll
+  for (uint32_t q = 0; q < Q; ++q) {
+    const uint32_t y = y0 + ty + 4u * q;
+    if (x < W && y < H) {
+      J.dens_col[(size_t)y * W + x] = v[q];
+      peak = fmaxf(peak, v[q]);
+    }
+  }
+  return wave_max(peak);
+}
+
 // blur.wgsl:13-83 column pass plus the peak (atomicMax on the f32 bits of a
 // non-negative value, as the reference does). Tiles of 64 columns x TH rows
 // (each thread TH / 4 outputs of one column).
@@ -1296,19 +1303,103 @@ __global__ __launch_bounds__(256) void bt_blur_cols(const BtJob* __restrict__ jo
     }
   }
   __syncthreads();
-  const float* wt = J.wt;
-  float peak = 0.0f;
-  for (uint32_t r = ty; r < TH; r += 4u) {
-    const uint32_t y = y0 + r;
-    if (x < W && y < H) {
-      const uint32_t c = K + r;
-      float v = tile[c][tx] * ld_uniform(wt, 0u);
-      for (uint32_t k = 1; k <= K; ++k) v = v + (tile[c - k][tx] + tile[c + k][tx]) * ld_uniform(wt, k);
-      J.dens_col[(size_t)y * W + x] = v;
-      peak = fmaxf(peak, v);
+  const float peak = col_pass<TH>(tile, J, x, y0, tx, ty);
+  if ((threadIdx.x & 63u) == 0) wmax[ty] = peak;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    atomicMax(J.peak, __float_as_uint(m));
+  }
+}
+
+// Both blur passes in one kernel, for blurs of at most BT_FUSED_TAPS taps:
+// the counts of a (TH + 2K) x (64 + 2K) window go to LDS, its rows are
+// blurred (each row value computed as bt_blur_rows computes it, so the row
+// plane is the same), the tile's own rows are written to the row plane, and
+// the column pass runs on the row values still in LDS. Halo rows outside the
+// image hold zero counts and so blur to zero, the value the column pass of
+// the two-kernel form reads there.
+template <int TH>
+__global__ __launch_bounds__(256) void bt_blur_fused(const BtJob* __restrict__ jobs,
+                                                     const uint32_t* __restrict__ base, int njobs) {
+  JOB_PROLOGUE(BT_K_COL);
+  constexpr uint32_t TW = 64, KMAX = BT_FUSED_TAPS;
+  constexpr uint32_t RMAX = TH + 2u * KMAX, PER = RMAX / 4u;
+  __shared__ float tile[RMAX][TW + 2u * KMAX];
+  __shared__ float wmax[4];
+  const uint32_t W = J.dw, H = J.dh, K = J.taps;
+  const uint32_t tiles_x = (W + TW - 1u) / TW;
+  const uint32_t x0 = (lb % tiles_x) * TW, y0 = (lb / tiles_x) * TH;
+  const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
+  const uint32_t R = TH + 2u * K, S = TW + 2u * K;
+  const uint32_t* src = J.dens_cnt;
+  // four rows x two column halves: eight independent loads in flight per
+  // thread before the LDS stores
+  for (uint32_t r0 = ty; r0 < R; r0 += 16u) {
+    float v[4][2];
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) {
+      const uint32_t r = r0 + 4u * b;
+      const int gy = (int)y0 - (int)K + (int)r;
+#pragma unroll
+      for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t c = tx + h * TW;
+        const int gx = (int)x0 - (int)K + (int)c;
+        v[b][h] = (r < R && c < S && gy >= 0 && gy < (int)H && gx >= 0 && gx < (int)W)
+                      ? (float)src[(size_t)gy * W + gx] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) {
+      const uint32_t r = r0 + 4u * b;
+#pragma unroll
+      for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t c = tx + h * TW;
+        if (r < R && c < S) tile[r][c] = v[b][h];
+      }
     }
   }
-  peak = wave_max(peak);
+  __syncthreads();
+  // row pass over rows ty, ty + 4, ... of the window, four rows at a time
+  // advancing together through the taps (wave-uniform guard per group)
+  const float* wt = J.wt;
+  const uint32_t x = x0 + tx;
+  const float w0 = ld_uniform(wt, 0u);
+  float rv[PER];
+#pragma unroll
+  for (uint32_t q0 = 0; q0 < PER; q0 += 4u) {
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) rv[q0 + b] = 0.0f;
+    if (ty + 4u * q0 < R) {
+      const uint32_t c = K + tx;
+      float v[4];
+#pragma unroll
+      for (uint32_t b = 0; b < 4; ++b) v[b] = tile[ty + 4u * (q0 + b)][c] * w0;
+      for (uint32_t k = 1; k <= K; ++k) {
+        const float wk = ld_uniform(wt, k);
+#pragma unroll
+        for (uint32_t b = 0; b < 4; ++b) {
+          const uint32_t r = ty + 4u * (q0 + b);
+          v[b] = v[b] + (tile[r][c - k] + tile[r][c + k]) * wk;
+        }
+      }
+#pragma unroll
+      for (uint32_t b = 0; b < 4; ++b) {
+        const uint32_t r = ty + 4u * (q0 + b);
+        rv[q0 + b] = v[b];
+        const uint32_t y = y0 + r - K;
+        if (r >= K && r < K + TH && x < W && y < H) J.dens_row[(size_t)y * W + x] = v[b];
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t q = 0; q < PER; ++q) {
+    const uint32_t r = ty + 4u * q;
+    if (r < R) tile[r][tx] = rv[q];
+  }
+  __syncthreads();
+  const float peak = col_pass<TH>(tile, J, x, y0, tx, ty);
   if ((threadIdx.x & 63u) == 0) wmax[ty] = peak;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1689,12 +1780,10 @@ void bin(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, 
 }
 void cell_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_cell_scan_reduce, L256(grid), jobs, base, nj);
-  hipLaunchKernelGGL(bt_cell_scan_bsum, L256((uint32_t)nj), jobs, nj);
   hipLaunchKernelGGL(bt_cell_scan_apply, L256(grid), jobs, base, nj);
 }
 void pre_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_pre_scan_reduce, L256(grid), jobs, base, nj);
-  hipLaunchKernelGGL(bt_pre_scan_bsum, L256((uint32_t)nj), jobs, nj);
   hipLaunchKernelGGL(bt_pre_scan_apply, dim3(grid), dim3(BT_PRE_APPLY_THREADS), 0, s, jobs, base,
                      nj);
 }
@@ -1763,9 +1852,10 @@ void row_tile(uint32_t shape, uint32_t* tw, uint32_t* th) {
   *th = 256u / *tw;
 }
 uint32_t col_tile_h(uint32_t shape) {
-  static const uint32_t h[BT_COL_SHAPE_COUNT] = {32, 16, 64};
+  static const uint32_t h[BT_COL_SHAPE_COUNT] = {32, 16, 64, 32, 16};
   return h[shape < BT_COL_SHAPE_COUNT ? shape : 0];
 }
+bool col_fused(uint32_t shape) { return shape == 3u || shape == 4u; }
 void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                uint32_t shape) {
   switch (shape) {
@@ -1780,6 +1870,8 @@ void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* 
   switch (shape) {
     case 1: hipLaunchKernelGGL((bt_blur_cols<16>), L256(grid), jobs, base, nj); return;
     case 2: hipLaunchKernelGGL((bt_blur_cols<64>), L256(grid), jobs, base, nj); return;
+    case 3: hipLaunchKernelGGL((bt_blur_fused<32>), L256(grid), jobs, base, nj); return;
+    case 4: hipLaunchKernelGGL((bt_blur_fused<16>), L256(grid), jobs, base, nj); return;
     default: hipLaunchKernelGGL((bt_blur_cols<32>), L256(grid), jobs, base, nj); return;
   }
 }

@@ -25,11 +25,15 @@ uint32_t neighbor_mean_blocks_per_cu(uint32_t shape);
 uint32_t neighbor_mean_waves(uint32_t shape);
 // Blur tile shapes: rows 0 = 64 x 4 (the default), 1 = 128 x 2, 2 = 32 x 8,
 // 3 = 256 x 1 (columns x rows); columns 0 = 64 x 32 (the default), 1 = 64 x
-// 16, 2 = 64 x 64. Results are the same under every shape.
+// 16, 2 = 64 x 64, and the fused row + column kernel 3 = 64 x 32, 4 = 64 x
+// 16 (blur_cols then does both passes and blur_rows is not launched; only
+// for blurs of at most BT_FUSED_TAPS taps). Results are the same under every
+// shape.
 #define BT_ROW_SHAPE_COUNT 4u
-#define BT_COL_SHAPE_COUNT 3u
+#define BT_COL_SHAPE_COUNT 5u
 void row_tile(uint32_t shape, uint32_t* tw, uint32_t* th);
 uint32_t col_tile_h(uint32_t shape);
+bool col_fused(uint32_t shape);
 void blur_rows(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
                uint32_t shape = 0);
 void blur_cols(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,

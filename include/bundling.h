@@ -245,17 +245,23 @@ int bt_autotune(bt_ctx* ctx, uint32_t n, const uint32_t* candidates, uint32_t n_
  * x waves per workgroup (0 = 16 x 4, 1 = 32 x 4, 2 = 16 x 2, 3 = 32 x 2,
  * 4 = 16 x 8, 5 = 32 x 8), the row blur's tile (0 = 64 x 4, 1 = 128 x 2,
  * 2 = 32 x 8, 3 = 256 x 1 pixels) and the column blur's (0 = 64 x 32,
- * 1 = 64 x 16, 2 = 64 x 64). Results are bit-identical under every shape.
- * A launch over several jobs runs the first job's shapes. */
+ * 1 = 64 x 16, 2 = 64 x 64; 3 = 64 x 32 and 4 = 64 x 16 run both blur
+ * passes in one kernel, which ignores the row shape, for blurs of at most
+ * 32 taps, sigma <= 10.67 px; wider blurs fall back to column shape 0).
+ * Results are bit-identical under every shape. A launch over several jobs
+ * runs the first job's shapes. */
 #define BT_NM_SHAPES 6u
 #define BT_ROW_SHAPES 4u
-#define BT_COL_SHAPES 3u
+#define BT_COL_SHAPES 5u
 /* Autotuner of the shapes: for each of the first n iterations, times each
  * stage under every shape of that stage from the same state (the other
- * stages at their current choice), keeps the fastest for every job
- * (shapes[3 i + k], its stage time in stage_ms[3 i + k]; either may be
- * NULL) and uses it from then on, including in hipGraphs. Leaves the jobs
- * rewound. Run after bt_autotune (the grid split) when both are wanted. */
+ * stages at their current choice; the column shapes by the time of both
+ * blur stages, as the fused shapes do both), keeps the fastest for every
+ * job (shapes[3 i + k]; either output may be NULL) and uses it from then
+ * on, including in hipGraphs. stage_ms[3 i + k] is the stage time of the
+ * chosen shapes: neighbor mean, row blur (0 under a fused column shape),
+ * column blur. Leaves the jobs rewound. Run after bt_autotune (the grid
+ * split) when both are wanted. */
 int bt_autotune_shapes(bt_ctx* ctx, uint32_t n, uint32_t* shapes, double* stage_ms);
 /* Sets the shapes of schedule entry `iteration` of every job (all entries
  * when iteration is 0xffffffff). */

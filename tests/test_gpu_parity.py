@@ -594,27 +594,44 @@ def _run_shapes(bt, g, opts, shapes, iters=3, graph=False):
         gpu.iterate(iters, graph=graph)
         info = gpu.info()
         return (gpu.points()[0], gpu.density(0), gpu.density(2), gpu.density(3),
-                info.split_tasks)
+                gpu.density(1), info.split_tasks)
 
 
 @pytest.mark.parametrize("shape", [(nm, 0, 0) for nm in range(6)] +
-                         [(0, row, 0) for row in range(1, 4)] + [(0, 0, col) for col in (1, 2)])
+                         [(0, row, 0) for row in range(1, 4)] +
+                         [(0, 0, col) for col in (1, 2, 3, 4)])
 def test_kernel_shapes_do_not_change_results(bt, cpu_case, shape):
     """Every neighbor-mean shape (lanes per group x waves per workgroup) and
     blur tile gives the default shapes' bits: positions, counts, blurred and
-    normalized density, on the plain CPU case (plain and grouped streams) and
-    on square cells as wide as the radius (tasks split across waves)."""
+    normalized density and the row-blur plane, on the plain CPU case (plain
+    and grouped streams) and on square cells as wide as the radius (tasks
+    split across waves)."""
     cases = [workloads.plain_cpu_options(),
              bt.Options(spacing=0.005, radius=0.1, sigma=6.0, density=512, cell_split=1,
                         x_split=1)]
     for k, opts in enumerate(cases):
         want = _run_shapes(bt, cpu_case, opts, None)
         got = _run_shapes(bt, cpu_case, opts, shape)
-        for a, b in zip(got[:4], want[:4]):
+        for a, b in zip(got[:5], want[:5]):
             np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32),
                                           err_msg=f"shape {shape} case {k}")
         if k == 1:
-            assert got[4] > 0  # the heavy-task kernel ran under this shape
+            assert got[5] > 0  # the heavy-task kernel ran under this shape
+
+
+@pytest.mark.parametrize("sigma", [10.5, 12.0])
+def test_fused_blur_edges(bt, cpu_case, sigma):
+    """The fused row + column blur at its widest blur (sigma 10.5: 32 taps,
+    the halo of a 64 x 16 tile twice its height) and past it (sigma 12: 36
+    taps, where the two-kernel blur runs instead) on a 200 x 200 map (tiles
+    cut by both image edges) gives the default shapes' bits."""
+    opts = bt.Options(spacing=0.005, radius=0.05, sigma=sigma, density=200)
+    want = _run_shapes(bt, cpu_case, opts, None, iters=2)
+    for col in (3, 4):
+        got = _run_shapes(bt, cpu_case, opts, (0, 0, col), iters=2)
+        for a, b in zip(got[:5], want[:5]):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32),
+                                          err_msg=f"col shape {col}")
 
 
 def test_autotune_shapes_keeps_results(bt, cpu_case):
