@@ -236,8 +236,15 @@ def render_section(bt, gpu, opts, reps=5):
 
 
 # FIFO for throughput; shortest-first with 1M sample-updates per tick for
-# latency (shorter ticks let small jobs through sooner, at some throughput)
-SERVER_POLICIES = (("fifo", "FIFO", 0), ("shortest_1M", "SHORTEST", 1 << 20))
+# latency (shorter ticks let small jobs through sooner, at some throughput);
+# two lanes (jobs of at most 256k samples tick on their own stream, larger
+# ones on another, overlapping on the GPU). The number is the tick budget
+# (SHORTEST) or the small-lane job limit (LANES).
+SERVER_POLICIES = (("fifo", "FIFO", 0), ("shortest_1M", "SHORTEST", 1 << 20),
+                   ("lanes_256k", "LANES", 1 << 18))
+# A second, lighter arrival stream (the same jobs at half the rate) for the
+# latency-load relation, under the final-only density policy.
+LIGHT_RATE_FACTOR = 0.5
 # Density policies, each applied to both forms: the density of every
 # iteration (as Gpu::iterate computes it), or only the one that is returned,
 # the last iteration's (BT_FLAG_DENSITY_ON_READ).
@@ -254,12 +261,15 @@ def _flagged(bt, opts, flags):
 
 def serve(bt, jobs, policy, tick, flags=0, realtime=True):
     """Feeds the jobs to a server: in real time at their arrival times
-    (realtime) or all at once (saturated: the server's capacity). Returns
-    (makespan s, [(job index, latency ms)], ticks, (start, end) on the
-    monotonic clock); latency = completion - arrival (saturated: - start)."""
+    (realtime) or all at once (saturated: the server's capacity). Each
+    finished job's samples and density are read into host arrays, as the
+    plain per-job form reads them. Returns (makespan s, [(job index, latency
+    ms)], ticks, (start, end) on the monotonic clock); latency = completion -
+    arrival (saturated: - start)."""
     n_jobs = len(jobs)
     srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy,
-                    tick_samples=tick)
+                    tick_samples=tick if policy == bt.Server.SHORTEST else 0,
+                    lane_samples=tick if policy == bt.Server.LANES else 0)
     clk0 = srv.clock_ms()
     t0 = time.perf_counter()
     pending, slat, i, done = {}, [], 0, 0
@@ -279,6 +289,9 @@ def serve(bt, jobs, policy, tick, flags=0, realtime=True):
             if r.state != 2:
                 raise RuntimeError(f"served job {k} failed")
             slat.append((k, r.done_ms - (clk0 + ta * 1e3)))
+            oc = jobs[k][2].c
+            srv.points(tk, r.samples)
+            srv.density(tk, oc.density_w, oc.density_h)
             srv.release(tk)
             done += 1
         if running == 0 and queued == 0 and i < n_jobs:
@@ -393,10 +406,18 @@ def served_section(bt, workloads, n_jobs, rate_hz=200.0):
                          "stream_updates_per_s": updates / rt_s, "stream_makespan_s": rt_s,
                          "stream_ticks": rt_ticks, **lat_stats(rt_lat)}
             if tick:
-                sec[name]["tick_samples"] = tick
+                sec[name]["tick_samples" if policy == "SHORTEST" else "lane_samples"] = tick
             log(f"served [{dname}] {name}: capacity {updates / cap_s:.3e} updates/s "
                 f"({plain_s / cap_s:.2f}x plain); stream p99 {sec[name]['p99_ms']:.1f} ms, "
                 f"jobs <= 20k edges p99 {sec[name]['p99_ms_le20k_edges']:.1f} ms")
+            if flags & 4:
+                light = [(ta / LIGHT_RATE_FACTOR, g, o, it) for (ta, g, o, it) in jobs]
+                l_s, l_lat, _, _ = serve(bt, light, getattr(bt.Server, policy), tick, flags)
+                sec[name]["light_stream"] = {"rate_hz": rate_hz * LIGHT_RATE_FACTOR,
+                                             "makespan_s": l_s, **lat_stats(l_lat)}
+                log(f"served [{dname}] {name} at {rate_hz * LIGHT_RATE_FACTOR:.0f} jobs/s: "
+                    f"p99 {sec[name]['light_stream']['p99_ms']:.1f} ms, jobs <= 20k edges p99 "
+                    f"{sec[name]['light_stream']['p99_ms_le20k_edges']:.1f} ms")
         out[dname] = sec
     return out
 

@@ -83,7 +83,7 @@ class bt_graph_desc(C.Structure):
 class bt_server_options(C.Structure):
     _fields_ = [("defaults", bt_options), ("memory_budget", C.c_uint64),
                 ("max_inflight", C.c_uint32), ("policy", C.c_uint32),
-                ("tick_samples", C.c_uint64)]
+                ("tick_samples", C.c_uint64), ("lane_samples", C.c_uint64)]
 
 
 class bt_server_info(C.Structure):
@@ -655,18 +655,21 @@ def blur_weights(sigma, max_taps=64):
 class Server:
     """Continuous-batching server: memory-based admission, one batched launch
     per stage per tick over the advanced jobs (policy FIFO: every in-flight
-    job; SHORTEST: least remaining work first, tick_samples per tick)."""
+    job; SHORTEST: least remaining work first, tick_samples per tick; LANES:
+    jobs of at most lane_samples samples tick on one stream, larger ones on
+    another, the two overlapping on the GPU)."""
 
-    FIFO, SHORTEST = 0, 1
+    FIFO, SHORTEST, LANES = 0, 1, 2
 
     def __init__(self, options=None, memory_budget=64 << 30, max_inflight=64, policy=0,
-                 tick_samples=0):
+                 tick_samples=0, lane_samples=0):
         so = bt_server_options()
         so.defaults = (options or Options()).c
         so.memory_budget = memory_budget
         so.max_inflight = max_inflight
         so.policy = policy
         so.tick_samples = tick_samples
+        so.lane_samples = lane_samples
         h = _P()
         rc = lib().bt_server_create(C.byref(so), C.byref(h))
         if rc:

@@ -402,6 +402,7 @@ namespace {
 struct ServedJob {
   uint64_t ticket = 0;
   uint32_t state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  uint32_t lane = 0;   // the engine (stream) that runs it
   std::vector<float> xy;
   std::vector<uint32_t> starts;
   bt_options opts{};
@@ -411,14 +412,14 @@ struct ServedJob {
   uint64_t est_samples = 0;  // samples of the count pass (exact)
   std::vector<uint64_t> path_scan;  // the count pass's path offsets, for the upload
   double last_ms = 0;     // server clock of admission or of the last iteration
-  // retirement in flight: results copied into a pinned buffer, `done` marks
-  // the end of the copies
+  // retirement: the results are copied into a pinned host buffer (`done`
+  // marks the end of the copies), which holds them until release: records
+  // (12 B each), then the normalized density (dw x dh floats), then the
+  // upload's status word
   hipEvent_t done = nullptr;
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
   std::unique_ptr<bt::Job> job;
-  std::vector<unsigned char> points;  // result records (12 B each)
-  std::vector<float> density;         // normalized density at completion
   uint32_t samples = 0, dw = 0, dh = 0;
   double submit_ms = 0, admit_ms = 0, done_ms = 0;
   std::string err;
@@ -427,12 +428,15 @@ struct ServedJob {
 }  // namespace
 
 static const double kStarveMs = 1000.0;       // SHORTEST: a job waiting longer goes first
-static const size_t kTicksAhead = 4;          // ticks queued on the GPU at most
+static const size_t kTicksAhead = 4;          // ticks per lane queued on the GPU at most
 static const uint64_t kTickSamples = 4ull << 20;  // SHORTEST: default sample-updates per tick
+static const uint64_t kLaneSamples = 1ull << 20;  // LANES: default small-lane job limit
 
 struct bt_server {
   bt_server_options o{};
-  std::unique_ptr<bt::Engine> engine;
+  // one engine (its own stream and launch state) per lane: BT_POLICY_LANES
+  // runs small and large jobs on two streams whose ticks overlap on the GPU
+  std::vector<std::unique_ptr<bt::Engine>> engines;
   std::chrono::steady_clock::time_point t0;
   uint64_t next_ticket = 1;
   uint64_t used = 0;   // admitted jobs' footprints (estimate, or held if larger)
@@ -442,7 +446,7 @@ struct bt_server {
   std::vector<ServedJob*> running;
   std::vector<ServedJob*> finishing;          // results being copied out
   std::multimap<size_t, void*> pinned_free;   // pinned result buffers for reuse
-  std::deque<hipEvent_t> ticks;               // ends of the ticks still on the GPU
+  std::vector<std::deque<hipEvent_t>> ticks;  // per lane: ends of its ticks still on the GPU
   std::deque<uint64_t> finished;              // tickets done or failed, not yet taken
   std::map<uint64_t, std::unique_ptr<ServedJob>> all;
   std::string err;
@@ -488,8 +492,15 @@ uint64_t estimate_bytes(ServedJob& s, uint64_t* samples_out) {
 }
 
 // Finishes retirements whose copies are done (all of them if `block`):
-// results move to the job's host vectors, device state and pinned buffers
-// are released.
+// the job's device state is released; its results stay in the pinned
+// buffer until the job is released.
+// Returns a job's result buffer to the free list.
+void put_pinned(bt_server* srv, ServedJob& s) {
+  if (!s.pinned) return;
+  srv->pinned_free.emplace(s.pinned_bytes, s.pinned);
+  s.pinned = nullptr;
+}
+
 void reap(bt_server* srv, bool block) {
   // Each job leaves `finishing` before anything can throw: on an error the
   // list holds exactly the jobs not handled yet, so a later release of a
@@ -510,20 +521,15 @@ void reap(bt_server* srv, bool block) {
     const size_t pb = (size_t)s->samples * 12;
     const unsigned char* pin = (const unsigned char*)s->pinned;
     uint32_t status = 0;
-    std::memcpy(&status, pin + pb + s->density.size() * 4, 4);
+    std::memcpy(&status, pin + pb + (size_t)s->dw * s->dh * 4, 4);
     if (status & 2u) {
       // the emit pass disagreed with the count pass (resampling.rs:296-299)
-      s->points.clear();
-      s->density.clear();
       s->state = 3;
       s->err = "GPU resampling count and write passes disagree";
+      put_pinned(srv, *s);
     } else {
-      s->points.assign(pin, pin + pb);
-      if (!s->density.empty()) std::memcpy(s->density.data(), pin + pb, s->density.size() * 4);
-      s->state = 2;
+      s->state = 2;  // the results stay in the pinned buffer until release
     }
-    srv->pinned_free.emplace(s->pinned_bytes, s->pinned);
-    s->pinned = nullptr;
     (void)hipEventDestroy(s->done);
     s->done = nullptr;
     s->job.reset();
@@ -535,17 +541,19 @@ void reap(bt_server* srv, bool block) {
   }
 }
 
-// A pinned host buffer of at least `bytes` (reused when one is free).
+// A pinned host buffer of at least `bytes`: a free one of the same size
+// class (powers of two from 1 MiB) when there is one, else a new one.
 void* get_pinned(bt_server* srv, size_t bytes, size_t* got) {
-  auto it = srv->pinned_free.lower_bound(bytes);
-  if (it != srv->pinned_free.end() && it->first <= 2 * bytes + (1 << 20)) {
+  size_t b = 1 << 20;
+  while (b < bytes) b <<= 1;
+  auto it = srv->pinned_free.find(b);
+  if (it != srv->pinned_free.end()) {
     void* p = it->second;
     *got = it->first;
     srv->pinned_free.erase(it);
     return p;
   }
   void* p = nullptr;
-  const size_t b = (bytes + (1 << 20) - 1) / (1 << 20) * (1 << 20);
   HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
   *got = b;
   return p;
@@ -564,7 +572,10 @@ int bt_server_create(const bt_server_options* opts, bt_server** out) {
   if (s->o.memory_budget == 0) s->o.memory_budget = 64ull << 30;
   std::string err;
   int rc = guarded(&err, [&] {
-    s->engine = std::make_unique<bt::Engine>(bt::resolve_options(&s->o.defaults));
+    const int lanes = s->o.policy == BT_POLICY_LANES ? 2 : 1;
+    for (int l = 0; l < lanes; ++l)
+      s->engines.push_back(std::make_unique<bt::Engine>(bt::resolve_options(&s->o.defaults)));
+    s->ticks.resize(lanes);
   });
   if (rc != BT_OK) return rc;
   s->t0 = std::chrono::steady_clock::now();
@@ -575,15 +586,18 @@ int bt_server_create(const bt_server_options* opts, bt_server** out) {
 void bt_server_destroy(bt_server* srv) {
   if (!srv) return;
   try {
-    if (srv->engine) srv->engine->sync();
-    for (hipEvent_t ev : srv->ticks) (void)hipEventDestroy(ev);
+    for (auto& e : srv->engines) e->sync();
+    for (auto& q : srv->ticks)
+      for (hipEvent_t ev : q) (void)hipEventDestroy(ev);
     for (ServedJob* s : srv->finishing) {
       if (s->done) (void)hipEventDestroy(s->done);
       if (s->pinned) (void)hipHostFree(s->pinned);
     }
+    for (auto& kv : srv->all)
+      if (kv.second->state >= 2 && kv.second->pinned) (void)hipHostFree(kv.second->pinned);
     for (auto& kv : srv->pinned_free) (void)hipHostFree(kv.second);
     srv->all.clear();
-    srv->engine.reset();
+    srv->engines.clear();
   } catch (...) {
   }
   delete srv;
@@ -620,6 +634,8 @@ int bt_server_submit(bt_server* srv, const float* xy, uint32_t n_points, const u
     srv->err = "job footprint exceeds the server memory budget";
     return BT_E_CAPACITY;
   }
+  if (srv->o.policy == BT_POLICY_LANES)
+    s->lane = s->est_samples > (srv->o.lane_samples ? srv->o.lane_samples : kLaneSamples);
   *ticket = s->ticket;
   srv->queue.push_back(s.get());
   srv->all.emplace(s->ticket, std::move(s));
@@ -630,15 +646,36 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
   if (!srv) return BT_E_INVALID;
   srv->err.clear();
   return guarded(&srv->err, [&] {
-    bt::Engine& e = *srv->engine;
-    // at most kTicksAhead ticks queued on the GPU: the host prepares the next
-    // tick while the previous one runs, and a loop of steps keeps pace
-    while (srv->ticks.size() >= kTicksAhead) {
-      const hipEvent_t ev = srv->ticks.front();
-      srv->ticks.pop_front();
-      const hipError_t q = hipEventSynchronize(ev);
-      (void)hipEventDestroy(ev);
-      HIPCHK(q);
+    // At most kTicksAhead ticks per lane queued on the GPU: the host prepares
+    // the next tick while the previous one runs, and a loop of steps keeps
+    // pace. Each lane is paced by its own ticks (a full lane skips this step);
+    // the host blocks only when every lane with running jobs is full, on the
+    // first such lane's oldest tick (lane 0 holds the small jobs).
+    const uint32_t lanes = (uint32_t)srv->engines.size();
+    auto pop_tick = [&](uint32_t lane, bool block) -> bool {
+      std::deque<hipEvent_t>& q = srv->ticks[lane];
+      if (q.empty()) return false;
+      const hipError_t r = block ? hipEventSynchronize(q.front()) : hipEventQuery(q.front());
+      if (r == hipErrorNotReady) return false;
+      (void)hipEventDestroy(q.front());
+      q.pop_front();
+      HIPCHK(r);
+      return true;
+    };
+    for (uint32_t lane = 0; lane < lanes; ++lane)
+      while (pop_tick(lane, false)) {
+      }
+    {
+      std::vector<bool> active(lanes, false);
+      for (const ServedJob* s : srv->running) active[s->lane] = true;
+      int wait_lane = -1;
+      bool room = false;
+      for (uint32_t lane = 0; lane < lanes; ++lane) {
+        if (!active[lane]) continue;
+        if (srv->ticks[lane].size() < kTicksAhead) room = true;
+        else if (wait_lane < 0) wait_lane = (int)lane;
+      }
+      if (!room && wait_lane >= 0) pop_tick((uint32_t)wait_lane, true);
     }
     // nothing left to launch: wait for the results being copied out
     reap(srv, srv->running.empty() && srv->queue.empty());
@@ -673,31 +710,42 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
       srv->queue.pop_front();
     }
     if (!admit.empty()) {
-      std::vector<bt::GraphIn> gs;
-      for (ServedJob* s : admit)
-        gs.push_back({s->xy.data(), (uint32_t)(s->xy.size() / 2), s->starts.data(),
-                      (uint32_t)s->starts.size(), s->opts, s->path_scan.data()});
-      std::vector<std::unique_ptr<bt::Job>> jobs;
-      try {
-        // queued behind the ticks on the GPU; the host does not wait
-        e.upload_jobs(gs, &jobs, true, false);
-      } catch (const bt::Error& err) {
-        // Isolate failures: retry one by one so one bad graph fails alone.
-        jobs.clear();
+      // each lane's admitted jobs in one upload on its engine, queued behind
+      // the ticks on the GPU (the host does not wait)
+      std::vector<std::unique_ptr<bt::Job>> jobs(admit.size());
+      for (uint32_t lane = 0; lane < srv->engines.size(); ++lane) {
+        bt::Engine& e = *srv->engines[lane];
+        std::vector<size_t> idx;
+        std::vector<bt::GraphIn> gs;
         for (size_t i = 0; i < admit.size(); ++i) {
-          std::vector<std::unique_ptr<bt::Job>> one;
-          try {
-            e.upload_jobs({gs[i]}, &one, true, false);
-            jobs.push_back(std::move(one[0]));
-          } catch (const bt::Error& err1) {
-            admit[i]->state = 3;
-            admit[i]->err = err1.msg;
-            admit[i]->done_ms = srv->now();
-            srv->finished.push_back(admit[i]->ticket);
-            jobs.push_back(nullptr);
-          }
+          ServedJob* s = admit[i];
+          if (s->lane != lane) continue;
+          idx.push_back(i);
+          gs.push_back({s->xy.data(), (uint32_t)(s->xy.size() / 2), s->starts.data(),
+                        (uint32_t)s->starts.size(), s->opts, s->path_scan.data()});
         }
-        (void)err;
+        if (gs.empty()) continue;
+        std::vector<std::unique_ptr<bt::Job>> up;
+        try {
+          e.upload_jobs(gs, &up, true, false);
+          for (size_t k = 0; k < idx.size(); ++k) jobs[idx[k]] = std::move(up[k]);
+        } catch (const bt::Error& err) {
+          // Isolate failures: retry one by one so one bad graph fails alone.
+          for (size_t k = 0; k < idx.size(); ++k) {
+            std::vector<std::unique_ptr<bt::Job>> one;
+            try {
+              e.upload_jobs({gs[k]}, &one, true, false);
+              jobs[idx[k]] = std::move(one[0]);
+            } catch (const bt::Error& err1) {
+              ServedJob* s = admit[idx[k]];
+              s->state = 3;
+              s->err = err1.msg;
+              s->done_ms = srv->now();
+              srv->finished.push_back(s->ticket);
+            }
+          }
+          (void)err;
+        }
       }
       const double t = srv->now();
       for (size_t i = 0; i < admit.size(); ++i) {
@@ -722,7 +770,8 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
         srv->running.push_back(s);
       }
     }
-    // One iteration of the advanced jobs in one batched launch per stage.
+    // One iteration of the advanced jobs in one batched launch per stage
+    // (per lane).
     if (!srv->running.empty()) {
       std::vector<ServedJob*> adv;
       if (shortest) {
@@ -743,18 +792,24 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
       } else {
         adv = srv->running;
       }
-      std::vector<bt::Job*> js;
-      for (ServedJob* s : adv) js.push_back(s->job.get());
-      e.iterate_jobs(js, false);  // queued; the next tick's host work overlaps it
-
-      hipEvent_t ev = nullptr;
-      HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      const hipError_t rq = hipEventRecord(ev, e.stream());
-      if (rq != hipSuccess) (void)hipEventDestroy(ev);
-      HIPCHK(rq);
-      srv->ticks.push_back(ev);
       const double t = srv->now();
-      for (ServedJob* s : adv) s->last_ms = t;
+      for (uint32_t lane = 0; lane < lanes; ++lane) {
+        if (srv->ticks[lane].size() >= kTicksAhead) continue;  // this lane waits
+        bt::Engine& e = *srv->engines[lane];
+        std::vector<bt::Job*> js;
+        for (ServedJob* s : adv)
+          if (s->lane == lane) js.push_back(s->job.get());
+        if (js.empty()) continue;
+        e.iterate_jobs(js, false);  // queued; the next tick's host work overlaps it
+        for (ServedJob* s : adv)
+          if (s->lane == lane) s->last_ms = t;
+        hipEvent_t ev = nullptr;
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        const hipError_t rq = hipEventRecord(ev, e.stream());
+        if (rq != hipSuccess) (void)hipEventDestroy(ev);
+        HIPCHK(rq);
+        srv->ticks[lane].push_back(ev);
+      }
       std::vector<ServedJob*> still;
       for (ServedJob* s : srv->running) {
         if (s->job->iteration < s->iterations) {
@@ -765,12 +820,11 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
         const bool dens = s->job->dens_cnt.p != nullptr;
         s->dw = dens ? s->job->dw : 0;
         s->dh = dens ? s->job->dh : 0;
-        s->density.assign((size_t)s->dw * s->dh, 0.0f);
         const size_t pb = (size_t)s->samples * 12;
         // records, density, then the upload's status word
-        s->pinned = get_pinned(srv, pb + s->density.size() * 4 + 4, &s->pinned_bytes);
+        s->pinned = get_pinned(srv, pb + (size_t)s->dw * s->dh * 4 + 4, &s->pinned_bytes);
         HIPCHK(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
-        e.queue_readback(*s->job, s->pinned,
+        srv->engines[s->lane]->queue_readback(*s->job, s->pinned,
                          dens ? (float*)((unsigned char*)s->pinned + pb) : nullptr, s->done);
         srv->finishing.push_back(s);
       }
@@ -834,8 +888,8 @@ int bt_server_read_points(bt_server* srv, uint64_t ticket, void* rec12, size_t n
   if (it == srv->all.end()) return BT_E_INVALID;
   const ServedJob& s = *it->second;
   if (s.state != 2) return BT_E_BUSY;
-  if (n_samples * 12 != s.points.size()) return BT_E_INVALID;
-  std::memcpy(rec12, s.points.data(), s.points.size());
+  if (n_samples != s.samples) return BT_E_INVALID;
+  std::memcpy(rec12, s.pinned, n_samples * 12);
   return BT_OK;
 }
 
@@ -846,8 +900,8 @@ int bt_server_read_density(bt_server* srv, uint64_t ticket, uint32_t kind, void*
   if (it == srv->all.end()) return BT_E_INVALID;
   const ServedJob& s = *it->second;
   if (s.state != 2) return BT_E_BUSY;
-  if (n_pixels != s.density.size()) return BT_E_INVALID;
-  std::memcpy(out, s.density.data(), n_pixels * 4);
+  if (n_pixels != (size_t)s.dw * s.dh) return BT_E_INVALID;
+  std::memcpy(out, (const unsigned char*)s.pinned + (size_t)s.samples * 12, n_pixels * 4);
   return BT_OK;
 }
 
@@ -856,6 +910,7 @@ int bt_server_release(bt_server* srv, uint64_t ticket) {
   auto it = srv->all.find(ticket);
   if (it == srv->all.end()) return BT_E_INVALID;
   if (it->second->state < 2) return BT_E_BUSY;
+  put_pinned(srv, *it->second);
   srv->all.erase(it);
   return BT_OK;
 }

@@ -358,6 +358,11 @@ typedef struct bt_server bt_server;
 #define BT_POLICY_FIFO 0      /* admit in arrival order; every running job advances each tick */
 #define BT_POLICY_SHORTEST 1  /* admit and advance the least remaining work first, up to
                                  tick_samples sample-updates per tick */
+#define BT_POLICY_LANES 2     /* admit in arrival order; jobs of at most lane_samples
+                                 samples run in a small-job lane, the others in a large-job
+                                 lane, each lane ticking all its running jobs on its own
+                                 stream, so the two lanes' ticks overlap on the GPU and small
+                                 jobs do not wait behind large jobs' iterations */
 
 typedef struct bt_server_options {
   bt_options defaults;       /* device, flags */
@@ -365,6 +370,8 @@ typedef struct bt_server_options {
   uint32_t max_inflight;     /* jobs holding device state at once */
   uint32_t policy;           /* BT_POLICY_* */
   uint64_t tick_samples;     /* BT_POLICY_SHORTEST: sample-updates per tick (0 = 4M) */
+  uint64_t lane_samples;     /* BT_POLICY_LANES: largest job (samples, from the count pass
+                                at submission) of the small-job lane (0 = 1M) */
 } bt_server_options;
 
 typedef struct bt_job_result {

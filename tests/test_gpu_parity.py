@@ -411,11 +411,12 @@ def test_rejected_upload_drops_the_graph(bt, cpu_case, points, starts, msg):
         np.testing.assert_array_equal(gpu.points()[0].view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("policy,tick", [(0, 0), (1, 0), (1, 1)])
+@pytest.mark.parametrize("policy,tick", [(0, 0), (1, 0), (1, 1), (2, 20000)])
 def test_server_results_match_alone(bt, policy, tick):
     """Co-scheduled jobs give the bits of each job alone, points and final
-    density, under both policies (tick 1: the shortest-first policy advances
-    one job per tick)."""
+    density, under every policy (tick 1: the shortest-first policy advances
+    one job per tick; policy 2 with jobs of at most 20,000 samples in the
+    small-job lane: both lanes hold jobs)."""
     graphs = workloads.batched_graphs(count=4, lo=60, hi=400)
     opts = bt.Options(spacing=0.004, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=256,
                       schedule_len=6)
@@ -428,12 +429,16 @@ def test_server_results_match_alone(bt, policy, tick):
             alone.append(gpu.points()[0])
             alone_d.append(gpu.density())
     srv = bt.Server(opts, memory_budget=1 << 30, max_inflight=3, policy=policy,
-                    tick_samples=tick)
+                    tick_samples=tick if policy == 1 else 0,
+                    lane_samples=tick if policy == 2 else 0)
     tickets = [srv.submit(g, opts, it) for g, it in zip(graphs, iters)]
     for _ in range(256):
         r, q = srv.step()
         if r == 0 and q == 0:
             break
+    if policy == 2:
+        small = [len(a) <= tick for a in alone]
+        assert any(small) and not all(small), [len(a) for a in alone]
     for t, ref, ref_d in zip(tickets, alone, alone_d):
         res = srv.poll(t)
         assert res.state == 2 and res.iterations_done == res.iterations_total
