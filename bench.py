@@ -436,7 +436,8 @@ def main():
                     help="time the export image of the bundled plain GPU case (0: skip)")
     ap.add_argument("--profile-only", action="store_true",
                     help="for rocprofv3 counter runs: a 256 MiB device copy (calibration), "
-                         "then upload + autotune + one eager 30-iteration schedule")
+                         "then upload + autotune (splits, then shapes) + one eager "
+                         "30-iteration schedule")
     ap.add_argument("--profile-batched", action="store_true",
                     help="for rocprofv3 kernel traces: the batched schedule (capture, rewind, "
                          "replay), printing the replay's window on the monotonic clock")
@@ -473,6 +474,7 @@ def main():
     if args.profile_only:
         if not args.cell_split:
             gpu.autotune(ITERS, TUNE_CANDIDATES)
+            gpu.autotune_shapes(ITERS)
         gpu.iterate(ITERS)
         log("profile-only schedule done")
         gpu.close()

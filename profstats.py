@@ -1,19 +1,24 @@
-"""Summaries of rocprofv3 output for the profiles/ notes.
+"""Summaries of rocprofv3 output for the profiles/ notes (measurement
+tooling beside bench.py; nothing in the library imports it).
 
-  python -m bundling_thingy_amd.profstats stats <kernel_trace.csv | results.db>
+Kernel names are shown without "void " and the parameter list, so template
+instances keep their arguments ("bt_neighbor_mean<16, 4>"); the <kernels>
+arguments below match the base name, every template instance of it.
+
+  python profstats.py stats <kernel_trace.csv | results.db>
       per-kernel calls / average / total / min / max durations
-  python -m bundling_thingy_amd.profstats last <kernel_trace.csv> <kernels> <n>
+  python profstats.py last <kernel_trace.csv> <kernels> <n>
       average duration of the last n dispatches; <kernels> is one name or a
       comma-separated list whose averages are added (one stage made of
       several launches, each dispatched once per stage)
-  python -m bundling_thingy_amd.profstats series <kernel_trace.csv> <kernel> <n>
+  python profstats.py series <kernel_trace.csv> <kernel> <n>
       durations (ms) of the last n dispatches of one kernel, in order
-  python -m bundling_thingy_amd.profstats window <kernel_trace.csv> <t0> <t1>
+  python profstats.py window <kernel_trace.csv> <t0> <t1>
       the dispatches that start between t0 and t1 (seconds of the monotonic
       clock, as bench.py prints them): span from the first start to the
       last end, busy time (union of the dispatch intervals), idle gaps, and
-      per-kernel totals
-  python -m bundling_thingy_amd.profstats traffic <fetch.csv> <write.csv> <kernels> <n> <copy_bytes>
+      per-kernel totals, also summed over each kernel's template instances
+  python profstats.py traffic <fetch.csv> <write.csv> <kernels> <n> <copy_bytes>
       HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
       each scaled by what the counter reads for HIP's device copy of
       copy_bytes bytes (read once, written once) in the same run; a kernel
@@ -25,14 +30,35 @@ import sqlite3
 import sys
 
 
+def _short(name):
+    """'void bt_neighbor_mean<16, 4>(BtJob const*, ...)' -> 'bt_neighbor_mean<16, 4>'."""
+    name = name[5:] if name.startswith("void ") else name
+    name = name.replace("(anonymous namespace)::", "")
+    depth = 0
+    for i, c in enumerate(name):
+        if c == "<":
+            depth += 1
+        elif c == ">":
+            depth -= 1
+        elif c == "(" and depth == 0:
+            return name[:i]
+    return name
+
+
+def _base(name):
+    """The kernel's base name, without template arguments."""
+    return _short(name).split("<")[0]
+
+
 def _trace_rows(path):
     if path.endswith(".db"):
         db = sqlite3.connect(path)
-        return [(n, float(d)) for n, d in db.execute("select name, duration from kernels")]
+        return [(_short(n), float(d)) for n, d in db.execute("select name, duration from kernels")]
     out = []
     with open(path) as f:
         for r in csv.DictReader(f):
-            out.append((r["Kernel_Name"], float(r["End_Timestamp"]) - float(r["Start_Timestamp"])))
+            out.append((_short(r["Kernel_Name"]),
+                        float(r["End_Timestamp"]) - float(r["Start_Timestamp"])))
     return out
 
 
@@ -55,7 +81,7 @@ def last(path, kernels, n):
     rows = _trace_rows(path)
     total = 0.0
     for k in kernels.split(","):
-        d = [t for name, t in rows if name == k][-n:]
+        d = [t for name, t in rows if _base(name) == k][-n:]
         total += sum(d) / len(d) / 1e6
     return total
 
@@ -66,7 +92,7 @@ def window(path, t0, t1):
         for r in csv.DictReader(f):
             a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
             if t0 * 1e9 <= a <= t1 * 1e9:
-                iv.append((a, b, r["Kernel_Name"]))
+                iv.append((a, b, _short(r["Kernel_Name"])))
     if not iv:
         return "no dispatches in the window"
     iv.sort()
@@ -87,6 +113,12 @@ def window(path, t0, t1):
     agg = {}
     for a, b, name in iv:
         agg[name] = agg.get(name, 0) + (b - a)
+    base = {}
+    for a, b, name in iv:
+        k = _base(name)
+        c = base.setdefault(k, [0, 0])
+        c[0] += 1
+        c[1] += b - a
     lines = [f"dispatches {len(iv)}, span {span / 1e6:.1f} ms, busy {busy / 1e6:.1f} ms "
              f"({100.0 * busy / span:.0f} %)",
              f"idle gaps: {len(gaps)}, {sum(gaps) / 1e6:.1f} ms in all, median {med / 1e3:.1f} us; "
@@ -94,6 +126,11 @@ def window(path, t0, t1):
              "| kernel | total ms | share of busy |", "|---|---|---|"]
     for name, t in sorted(agg.items(), key=lambda kv: -kv[1]):
         lines.append(f"| {name[:48]} | {t / 1e6:.2f} | {100.0 * t / busy:.1f} % |")
+    lines += ["", "By base name (template instances together):", "",
+              "| kernel | dispatches | total ms | avg us | share of busy |", "|---|---|---|---|---|"]
+    for name, (c, t) in sorted(base.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"| {name[:48]} | {c} | {t / 1e6:.2f} | {t / c / 1e3:.1f} | "
+                     f"{100.0 * t / busy:.1f} % |")
     return "\n".join(lines)
 
 
@@ -112,7 +149,7 @@ def traffic(fetch_csv, write_csv, kernel, n, copy_bytes):
         factor = (copy_bytes / 1024.0) / calib_kib
         kib = 0.0
         for k in kernel.split(","):
-            vals = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"] == k][-n:]
+            vals = [float(r["Counter_Value"]) for r in rows if _base(r["Kernel_Name"]) == k][-n:]
             kib += sum(vals) / len(vals)
         out[key] = {"kib_per_dispatch": kib, "calibration_kib": calib_kib, "factor": factor,
                     "bytes_per_dispatch": kib * 1024.0 * factor}
@@ -128,7 +165,8 @@ if __name__ == "__main__":
     elif cmd == "last":
         print(f"{last(sys.argv[2], sys.argv[3], int(sys.argv[4])):.4f} ms")
     elif cmd == "series":
-        d = [t for name, t in _trace_rows(sys.argv[2]) if name == sys.argv[3]][-int(sys.argv[4]):]
+        d = [t for name, t in _trace_rows(sys.argv[2]) if _base(name) == sys.argv[3]]
+        d = d[-int(sys.argv[4]):]
         print(" ".join(f"{t / 1e6:.2f}" for t in d))
     elif cmd == "window":
         print(window(sys.argv[2], float(sys.argv[3]), float(sys.argv[4])))
