@@ -42,7 +42,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(graph, opts, positions, target_s=15.0):
+def cpu_baseline(graph, opts, positions, target_s=12.0):
     """The CPU oracle on all host threads, following the schedule: iteration
     i's neighbor means are computed from the positions the GPU run fed that
     iteration (`positions[i]`, read back after each eager iteration), with
@@ -85,8 +85,10 @@ def cpu_baseline(graph, opts, positions, target_s=15.0):
 
     res = {}
     for name, fn in fns.items():
-        # a coarse pass over the whole schedule sets the stride of the timed pass
-        coarse = 4096
+        # a coarse pass over the whole schedule sets the stride of the timed
+        # pass (one sample in 64: enough that the means' time stands clear of
+        # the per-call time subtracted from it)
+        coarse = 64
         _, tm, other = schedule(fn, coarse)
         avail = max(target_s / 2.0 - other, 1.0)  # the timed pass takes about target_s / 2
         stride = int(max(1, min(coarse, np.ceil(coarse * tm / avail))))
