@@ -18,6 +18,10 @@ arguments below match the base name, every template instance of it.
       clock, as bench.py prints them): span from the first start to the
       last end, busy time (union of the dispatch intervals), idle gaps, and
       per-kernel totals, also summed over each kernel's template instances
+  python profstats.py queues <kernel_trace.csv> <t0> <t1> <marker>
+      per hardware queue: busy time and the spans of the ticks that begin at
+      each dispatch of <marker> (a kernel run once per iteration), with the
+      gaps between them; then the time two or more queues run kernels
   python profstats.py traffic <fetch.csv> <write.csv> <kernels> <n> <copy_bytes>
       HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
       each scaled by what the counter reads for HIP's device copy of
@@ -134,6 +138,54 @@ def window(path, t0, t1):
     return "\n".join(lines)
 
 
+def queues(path, t0, t1, marker):
+    """Per hardware queue, over the dispatches starting in [t0, t1]: busy
+    time, and the spans of the stretches that begin at each dispatch of
+    `marker` (one per tick when the marker runs once per iteration): from
+    the marker's start to the last end before the next marker, and the gap
+    to the next marker. Last, the time at least two queues run kernels."""
+    by = {}
+    allv = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            if t0 * 1e9 <= a <= t1 * 1e9:
+                v = (a, b, _base(r["Kernel_Name"]), r["Queue_Id"])
+                by.setdefault(r["Queue_Id"], []).append(v)
+                allv.append(v)
+
+    def pct(xs, q):
+        xs = sorted(xs)
+        return xs[min(len(xs) - 1, int(q / 100.0 * len(xs)))] if xs else 0.0
+
+    lines = ["| queue | dispatches | busy ms | ticks | span p50 / p90 / p99 ms | gap p50 / p99 ms |",
+             "|---|---|---|---|---|---|"]
+    for qid, iv in sorted(by.items()):
+        iv.sort()
+        busy = sum(b - a for a, b, _, _ in iv)
+        marks = [k for k, v in enumerate(iv) if v[2] == marker]
+        spans, gaps = [], []
+        for j in range(len(marks) - 1):
+            seg = iv[marks[j]:marks[j + 1]]
+            end = max(b for _, b, _, _ in seg)
+            spans.append((end - seg[0][0]) / 1e6)
+            gaps.append((iv[marks[j + 1]][0] - end) / 1e6)
+        if not spans:
+            continue
+        lines.append(f"| {qid} | {len(iv)} | {busy / 1e6:.1f} | {len(spans)} | "
+                     f"{pct(spans, 50):.2f} / {pct(spans, 90):.2f} / {pct(spans, 99):.2f} | "
+                     f"{pct(gaps, 50):.3f} / {pct(gaps, 99):.3f} |")
+    ev = sorted([(a, 1, q) for a, _, _, q in allv] + [(b, -1, q) for _, b, _, q in allv])
+    active, last, two = {}, None, 0
+    for t, d, q in ev:
+        if last is not None and sum(1 for c in active.values() if c > 0) >= 2:
+            two += t - last
+        active[q] = active.get(q, 0) + d
+        last = t
+    lines.append(f"two or more queues running kernels: {two / 1e6:.1f} ms")
+    return "\n".join(lines)
+
+
 def _counter(path, name):
     with open(path) as f:
         return [r for r in csv.DictReader(f) if r["Counter_Name"] == name]
@@ -170,6 +222,8 @@ if __name__ == "__main__":
         print(" ".join(f"{t / 1e6:.2f}" for t in d))
     elif cmd == "window":
         print(window(sys.argv[2], float(sys.argv[3]), float(sys.argv[4])))
+    elif cmd == "queues":
+        print(queues(sys.argv[2], float(sys.argv[3]), float(sys.argv[4]), sys.argv[5]))
     elif cmd == "traffic":
         print(json.dumps(traffic(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]),
                                  int(sys.argv[6])), indent=1))
