@@ -219,10 +219,12 @@ def render_section(bt, gpu, opts, reps=5):
     """Export image of the bundled graph (render_paths.wgsl, gpu.rs:572-651):
     density colouring, 4x MSAA, on the 1024^2 map; wall time of the whole
     call (rasterize, sort, resolve, 4 MiB readback) and of the noisiness
-    measurement on the host."""
+    measurement on the host; the median of `reps` calls after three
+    untimed ones."""
     with bt.Gpu(opts) as blank:
         rgb = bt.noisiness_background(blank.render())
-    gpu.render()
+    for _ in range(3):  # warm-up: first-call allocations, fresh host pages
+        gpu.render()
     times = []
     for _ in range(reps):
         t = time.perf_counter()

@@ -206,6 +206,7 @@ Engine::~Engine() {
     if (s.ev) (void)hipEventDestroy(s.ev);
     if (s.p) (void)hipHostFree(s.p);
   }
+  if (rd_pinned_.p) (void)hipHostFree(rd_pinned_.p);
 }
 
 void Engine::sync() { HIPCHK(hipStreamSynchronize(stream_)); }
@@ -482,6 +483,18 @@ void* Engine::upload_staging(size_t bytes, Pinned** slot) {
   }
   *slot = &s;
   return s.p;
+}
+
+void* Engine::readback_staging(size_t bytes) {
+  if (rd_pinned_.bytes < bytes) {
+    if (rd_pinned_.p) HIPCHK(hipHostFree(rd_pinned_.p));
+    rd_pinned_.p = nullptr;
+    rd_pinned_.bytes = 0;
+    const size_t b = align_up(std::max<size_t>(bytes, 1 << 20), 1 << 20);
+    HIPCHK(hipHostMalloc(&rd_pinned_.p, b, hipHostMallocDefault));
+    rd_pinned_.bytes = b;
+  }
+  return rd_pinned_.p;
 }
 
 void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique_ptr<Job>>* out,

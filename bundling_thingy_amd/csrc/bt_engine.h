@@ -200,11 +200,15 @@ class Engine {
     bool pending = false;
   };
   std::vector<Pinned> pinned_;
-  // pinned staging of upload inputs (and of image readbacks): a ring, so an
-  // asynchronous upload never refills a buffer whose copy is still queued
+  // pinned staging of upload inputs: a ring, so an asynchronous upload
+  // never refills a buffer whose copy is still queued
   std::vector<Pinned> up_pinned_;
   uint32_t up_next_ = 0;
   void* upload_staging(size_t bytes, Pinned** slot);
+  // pinned buffer of blocking readbacks (images): one, grown when needed,
+  // kept for the next call (the caller waits for its copy before reuse)
+  Pinned rd_pinned_;
+  void* readback_staging(size_t bytes);
   uint32_t pinned_next_ = 0;
   uint32_t retire_next_ = 0;
   size_t slot_bytes_ = 0;

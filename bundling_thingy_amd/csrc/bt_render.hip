@@ -851,8 +851,7 @@ void Engine::render(const bt_render_options& ro, uint8_t* rgba, size_t n_bytes, 
   HIPCHK(hipGetLastError());
   // one DMA into pinned memory, then a host copy (a pageable destination
   // would be copied in staged chunks)
-  Pinned* slot = nullptr;
-  void* pin = upload_staging(npix * 4, &slot);
+  void* pin = readback_staging(npix * 4);
   HIPCHK(hipMemcpyAsync(pin, image.p, npix * 4, hipMemcpyDeviceToHost, stream_));
   sync();
   std::memcpy(rgba, pin, npix * 4);
