@@ -239,13 +239,15 @@ def render_section(bt, gpu, opts, reps=5):
             "ms": ms, "noisiness": m["noisiness"], "noisiness_ms": noise_ms}
 
 
-# FIFO for throughput; shortest-first with 1M sample-updates per tick for
-# latency (shorter ticks let small jobs through sooner, at some throughput);
-# two lanes (jobs of at most 256k samples tick on their own stream, larger
-# ones on another, overlapping on the GPU). The number is the tick budget
-# (SHORTEST) or the small-lane job limit (LANES).
-SERVER_POLICIES = (("fifo", "FIFO", 0), ("shortest_1M", "SHORTEST", 1 << 20),
-                   ("lanes_256k", "LANES", 1 << 18))
+# (name, policy, tick_samples, lane_samples). FIFO for throughput;
+# shortest-first with 1M sample-updates per tick for latency (shorter ticks
+# let small jobs through sooner, at some throughput); two lanes (jobs of at
+# most 256k samples tick on their own stream, larger ones on another,
+# overlapping on the GPU), each lane's tick taking the least remaining work
+# first up to 1M or 2M sample-updates while no job waits for admission.
+SERVER_POLICIES = (("fifo", "FIFO", 0, 0), ("shortest_1M", "SHORTEST", 1 << 20, 0),
+                   ("lanes_256k_1M", "LANES", 1 << 20, 1 << 18),
+                   ("lanes_256k_2M", "LANES", 2 << 20, 1 << 18))
 # A second, lighter arrival stream (the same jobs at half the rate) for the
 # latency-load relation, under the final-only density policy.
 LIGHT_RATE_FACTOR = 0.5
@@ -263,7 +265,7 @@ def _flagged(bt, opts, flags):
     return o
 
 
-def serve(bt, jobs, policy, tick, flags=0, realtime=True):
+def serve(bt, jobs, policy, tick, flags=0, realtime=True, lane=0):
     """Feeds the jobs to a server: in real time at their arrival times
     (realtime) or all at once (saturated: the server's capacity). Each
     finished job's samples and density are read into host arrays, as the
@@ -272,8 +274,7 @@ def serve(bt, jobs, policy, tick, flags=0, realtime=True):
     arrival (saturated: - start)."""
     n_jobs = len(jobs)
     srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy,
-                    tick_samples=tick if policy == bt.Server.SHORTEST else 0,
-                    lane_samples=tick if policy == bt.Server.LANES else 0)
+                    tick_samples=tick, lane_samples=lane)
     clk0 = srv.clock_ms()
     t0 = time.perf_counter()
     pending, slat, i, done = {}, [], 0, 0
@@ -326,8 +327,9 @@ def profile_served(bt, workloads, n_jobs=1000, rate_hz=200.0):
     server policy, in real time; prints each run's window."""
     jobs = workloads.served_jobs(n_jobs, rate_hz)
     serve(bt, jobs[:3], bt.Server.FIFO, 0, 4)  # warm the code paths
-    for name, policy, tick in SERVER_POLICIES:
-        srv_s, _, ticks, (t0, t1) = serve(bt, jobs, getattr(bt.Server, policy), tick, 4)
+    for name, policy, tick, lane in SERVER_POLICIES:
+        srv_s, _, ticks, (t0, t1) = serve(bt, jobs, getattr(bt.Server, policy), tick, 4,
+                                          lane=lane)
         log(f"served {name}: makespan {srv_s:.3f} s, {ticks} ticks, window {t0:.4f} {t1:.4f}")
 
 
@@ -401,22 +403,24 @@ def served_section(bt, workloads, n_jobs, rate_hz=200.0):
                          "isolated_p50_ms_le20k_edges": pct(iso, 50),
                          "isolated_p99_ms_le20k_edges": pct(iso, 99)}}
         log(f"served [{dname}]: plain per-job {plain_s:.2f} s, {plain_cap:.3e} updates/s")
-        for name, policy, tick in SERVER_POLICIES:
-            cap_s, _, cap_ticks, _ = serve(bt, jobs, getattr(bt.Server, policy), tick, flags,
-                                           realtime=False)
-            rt_s, rt_lat, rt_ticks, _ = serve(bt, jobs, getattr(bt.Server, policy), tick, flags)
+        for name, policy, tick, lane in SERVER_POLICIES:
+            pol = getattr(bt.Server, policy)
+            cap_s, _, cap_ticks, _ = serve(bt, jobs, pol, tick, flags, realtime=False, lane=lane)
+            rt_s, rt_lat, rt_ticks, _ = serve(bt, jobs, pol, tick, flags, lane=lane)
             sec[name] = {"capacity_updates_per_s": updates / cap_s, "capacity_seconds": cap_s,
                          "capacity_vs_plain": plain_s / cap_s, "capacity_ticks": cap_ticks,
                          "stream_updates_per_s": updates / rt_s, "stream_makespan_s": rt_s,
                          "stream_ticks": rt_ticks, **lat_stats(rt_lat)}
             if tick:
-                sec[name]["tick_samples" if policy == "SHORTEST" else "lane_samples"] = tick
+                sec[name]["tick_samples"] = tick
+            if lane:
+                sec[name]["lane_samples"] = lane
             log(f"served [{dname}] {name}: capacity {updates / cap_s:.3e} updates/s "
                 f"({plain_s / cap_s:.2f}x plain); stream p99 {sec[name]['p99_ms']:.1f} ms, "
                 f"jobs <= 20k edges p99 {sec[name]['p99_ms_le20k_edges']:.1f} ms")
             if flags & 4:
                 light = [(ta / LIGHT_RATE_FACTOR, g, o, it) for (ta, g, o, it) in jobs]
-                l_s, l_lat, _, _ = serve(bt, light, getattr(bt.Server, policy), tick, flags)
+                l_s, l_lat, _, _ = serve(bt, light, pol, tick, flags, lane=lane)
                 sec[name]["light_stream"] = {"rate_hz": rate_hz * LIGHT_RATE_FACTOR,
                                              "makespan_s": l_s, **lat_stats(l_lat)}
                 log(f"served [{dname}] {name} at {rate_hz * LIGHT_RATE_FACTOR:.0f} jobs/s: "

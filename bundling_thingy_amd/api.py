@@ -657,7 +657,9 @@ class Server:
     per stage per tick over the advanced jobs (policy FIFO: every in-flight
     job; SHORTEST: least remaining work first, tick_samples per tick; LANES:
     jobs of at most lane_samples samples tick on one stream, larger ones on
-    another, the two overlapping on the GPU)."""
+    another, the two overlapping on the GPU; with tick_samples, each lane's
+    tick takes the least remaining work first up to tick_samples while no
+    job waits for admission)."""
 
     FIFO, SHORTEST, LANES = 0, 1, 2
 

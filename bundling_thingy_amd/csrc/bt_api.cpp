@@ -774,7 +774,13 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
     // (per lane).
     if (!srv->running.empty()) {
       std::vector<ServedJob*> adv;
-      if (shortest) {
+      // SHORTEST, and LANES with a tick budget while no job waits for
+      // admission: each lane's tick takes its jobs with the least remaining
+      // work first, up to the budget (with a backlog, LANES ticks every
+      // running job: throughput first)
+      const bool lane_budget = srv->o.policy == BT_POLICY_LANES && srv->o.tick_samples &&
+                               srv->queue.empty();
+      if (shortest || lane_budget) {
         const double now = srv->now();
         std::vector<ServedJob*> order(srv->running);
         std::stable_sort(order.begin(), order.end(), [&](const ServedJob* a, const ServedJob* b) {
@@ -783,11 +789,11 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
           return remaining(a) < remaining(b);
         });
         const uint64_t budget = srv->o.tick_samples ? srv->o.tick_samples : kTickSamples;
-        uint64_t taken = 0;
+        std::vector<uint64_t> taken(lanes, 0);
         for (ServedJob* s : order) {
-          if (!adv.empty() && taken + s->job->n > budget) continue;
+          if (taken[s->lane] && taken[s->lane] + s->job->n > budget) continue;
           adv.push_back(s);
-          taken += s->job->n;
+          taken[s->lane] += s->job->n;
         }
       } else {
         adv = srv->running;

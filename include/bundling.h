@@ -360,16 +360,22 @@ typedef struct bt_server bt_server;
                                  tick_samples sample-updates per tick */
 #define BT_POLICY_LANES 2     /* admit in arrival order; jobs of at most lane_samples
                                  samples run in a small-job lane, the others in a large-job
-                                 lane, each lane ticking all its running jobs on its own
-                                 stream, so the two lanes' ticks overlap on the GPU and small
-                                 jobs do not wait behind large jobs' iterations */
+                                 lane, each lane ticking on its own stream, so the two lanes'
+                                 ticks overlap on the GPU and small jobs do not wait behind
+                                 large jobs' iterations. With tick_samples = 0 each tick takes
+                                 all of its lane's running jobs; otherwise, while no job waits
+                                 for admission, it takes the lane's jobs with the least
+                                 remaining work first up to tick_samples sample-updates, and
+                                 all of them when a backlog forms (throughput first) */
 
 typedef struct bt_server_options {
   bt_options defaults;       /* device, flags */
   uint64_t memory_budget;    /* device bytes the in-flight jobs may hold */
   uint32_t max_inflight;     /* jobs holding device state at once */
   uint32_t policy;           /* BT_POLICY_* */
-  uint64_t tick_samples;     /* BT_POLICY_SHORTEST: sample-updates per tick (0 = 4M) */
+  uint64_t tick_samples;     /* BT_POLICY_SHORTEST: sample-updates per tick (0 = 4M);
+                                BT_POLICY_LANES: per lane and tick while no job waits
+                                (0 = every running job) */
   uint64_t lane_samples;     /* BT_POLICY_LANES: largest job (samples, from the count pass
                                 at submission) of the small-job lane (0 = 1M) */
 } bt_server_options;

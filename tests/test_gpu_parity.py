@@ -411,12 +411,14 @@ def test_rejected_upload_drops_the_graph(bt, cpu_case, points, starts, msg):
         np.testing.assert_array_equal(gpu.points()[0].view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("policy,tick", [(0, 0), (1, 0), (1, 1), (2, 20000)])
-def test_server_results_match_alone(bt, policy, tick):
+@pytest.mark.parametrize("policy,tick,lane", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 0, 20000),
+                                              (2, 1, 20000)])
+def test_server_results_match_alone(bt, policy, tick, lane):
     """Co-scheduled jobs give the bits of each job alone, points and final
     density, under every policy (tick 1: the shortest-first policy advances
     one job per tick; policy 2 with jobs of at most 20,000 samples in the
-    small-job lane: both lanes hold jobs)."""
+    small-job lane: both lanes hold jobs; with tick 1 each lane advances one
+    job per tick once no job waits for admission, all of them before)."""
     graphs = workloads.batched_graphs(count=4, lo=60, hi=400)
     opts = bt.Options(spacing=0.004, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=256,
                       schedule_len=6)
@@ -429,15 +431,14 @@ def test_server_results_match_alone(bt, policy, tick):
             alone.append(gpu.points()[0])
             alone_d.append(gpu.density())
     srv = bt.Server(opts, memory_budget=1 << 30, max_inflight=3, policy=policy,
-                    tick_samples=tick if policy == 1 else 0,
-                    lane_samples=tick if policy == 2 else 0)
+                    tick_samples=tick, lane_samples=lane)
     tickets = [srv.submit(g, opts, it) for g, it in zip(graphs, iters)]
     for _ in range(256):
         r, q = srv.step()
         if r == 0 and q == 0:
             break
     if policy == 2:
-        small = [len(a) <= tick for a in alone]
+        small = [len(a) <= lane for a in alone]
         assert any(small) and not all(small), [len(a) for a in alone]
     for t, ref, ref_d in zip(tickets, alone, alone_d):
         res = srv.poll(t)

@@ -168,14 +168,13 @@ def served_case(bt):
     return jobs, alone
 
 
-def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=(), flags=0):
+def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=(), flags=0, lane=0):
     """Submits `per_step` arrivals before each tick (about one tick's worth of
     the 200 jobs/s stream at full load), so admissions keep arriving while
     earlier jobs run. Returns {job index: (positions hash, density hash)} and
     {job index: (positions, density)} for the jobs in `keep`."""
     srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy,
-                    tick_samples=tick if policy == 1 else 0,
-                    lane_samples=tick if policy == 2 else 0)
+                    tick_samples=tick, lane_samples=lane)
     try:
         pending, out, raw, i = {}, {}, {}, 0
         for _ in range(100000):
@@ -211,16 +210,18 @@ def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=(), flags=0):
         srv.close()
 
 
-@pytest.mark.parametrize("policy,tick,flags", [(0, 0, 0), (1, 1 << 20, 0), (0, 0, 4),
-                                               (1, 1 << 20, 4), (2, 1 << 18, 0), (2, 1 << 18, 4)])
-def test_served_stream_equals_alone(bt, served_case, policy, tick, flags):
+@pytest.mark.parametrize("policy,tick,lane,flags", [(0, 0, 0, 0), (1, 1 << 20, 0, 0),
+                                                    (0, 0, 0, 4), (1, 1 << 20, 0, 4),
+                                                    (2, 0, 1 << 18, 0), (2, 1 << 20, 1 << 18, 4)])
+def test_served_stream_equals_alone(bt, served_case, policy, tick, lane, flags):
     """Every job of the 100-job served stream, co-scheduled with the others
     under continuous batching (FIFO, shortest-first with 1M sample-updates
-    per tick, and two lanes split at 256k samples), ends with the bits of the
-    job alone, whether the jobs compute their density every iteration or only
-    at the last one (flags 4 = BT_FLAG_DENSITY_ON_READ)."""
+    per tick, and two lanes split at 256k samples, without and with a 1M
+    per-lane tick budget), ends with the bits of the job alone, whether the
+    jobs compute their density every iteration or only at the last one
+    (flags 4 = BT_FLAG_DENSITY_ON_READ)."""
     jobs, alone = served_case
-    out, _ = _serve_compressed(bt, jobs, policy, tick, flags=flags)
+    out, _ = _serve_compressed(bt, jobs, policy, tick, flags=flags, lane=lane)
     for k in range(len(jobs)):
         assert out[k] == alone[k], f"job {k}"
 
