@@ -501,6 +501,22 @@ void put_pinned(bt_server* srv, ServedJob& s) {
   s.pinned = nullptr;
 }
 
+// A job whose results could not be queued for copying: it fails alone and
+// gives back its device state and its share of the budget.
+void retire_failed(bt_server* srv, ServedJob& s, const std::string& msg) {
+  if (s.done) (void)hipEventDestroy(s.done);
+  s.done = nullptr;
+  put_pinned(srv, s);
+  s.job.reset();
+  srv->used -= std::min(srv->used, s.estimate);
+  srv->held -= std::min(srv->held, s.held);
+  s.held = 0;
+  s.state = 3;
+  s.err = msg;
+  s.done_ms = srv->now();
+  srv->finished.push_back(s.ticket);
+}
+
 void reap(bt_server* srv, bool block) {
   // Each job leaves `finishing` before anything can throw: on an error the
   // list holds exactly the jobs not handled yet, so a later release of a
@@ -816,25 +832,40 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
         HIPCHK(rq);
         srv->ticks[lane].push_back(ev);
       }
-      std::vector<ServedJob*> still;
-      for (ServedJob* s : srv->running) {
-        if (s->job->iteration < s->iterations) {
-          still.push_back(s);
-          continue;
-        }
-        // retire: queue the copies of the results; reap() completes them
-        const bool dens = s->job->dens_cnt.p != nullptr;
-        s->dw = dens ? s->job->dw : 0;
-        s->dh = dens ? s->job->dh : 0;
-        const size_t pb = (size_t)s->samples * 12;
-        // records, density, then the upload's status word
-        s->pinned = get_pinned(srv, pb + (size_t)s->dw * s->dh * 4 + 4, &s->pinned_bytes);
-        HIPCHK(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
-        srv->engines[s->lane]->queue_readback(*s->job, s->pinned,
-                         dens ? (float*)((unsigned char*)s->pinned + pb) : nullptr, s->done);
-        srv->finishing.push_back(s);
-      }
+      // Retire the jobs that ran their last iteration: they leave `running`
+      // first, then each either joins `finishing` (copies queued; reap()
+      // completes them) or, if queuing fails, fails alone; the first error
+      // is reported after every retiring job has been handled.
+      std::vector<ServedJob*> still, retire;
+      for (ServedJob* s : srv->running)
+        (s->job->iteration < s->iterations ? still : retire).push_back(s);
       srv->running.swap(still);
+      int err_code = BT_OK;
+      std::string err_msg;
+      for (ServedJob* s : retire) {
+        try {
+          const bool dens = s->job->dens_cnt.p != nullptr;
+          s->dw = dens ? s->job->dw : 0;
+          s->dh = dens ? s->job->dh : 0;
+          const size_t pb = (size_t)s->samples * 12;
+          // records, density, then the upload's status word
+          s->pinned = get_pinned(srv, pb + (size_t)s->dw * s->dh * 4 + 4, &s->pinned_bytes);
+          HIPCHK(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
+          srv->engines[s->lane]->queue_readback(
+              *s->job, s->pinned, dens ? (float*)((unsigned char*)s->pinned + pb) : nullptr,
+              s->done);
+          srv->finishing.push_back(s);
+        } catch (const bt::Error& e) {
+          // copies queued before the error must end before the buffer is reused
+          (void)hipStreamSynchronize(srv->engines[s->lane]->stream());
+          retire_failed(srv, *s, e.msg);
+          if (err_code == BT_OK) {
+            err_code = e.code;
+            err_msg = e.msg;
+          }
+        }
+      }
+      if (err_code != BT_OK) bt::fail(err_code, err_msg);
     }
     if (running) *running = (uint32_t)(srv->running.size() + srv->finishing.size());
     if (queued) *queued = (uint32_t)srv->queue.size();
