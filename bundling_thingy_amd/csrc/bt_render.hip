@@ -147,16 +147,6 @@ __device__ void tri_setup(int ax, int ay, int bx, int by, int cx, int cy, Tri& t
   }
 }
 
-__device__ __forceinline__ bool tri_in(const Tri& t, int sx, int sy) {
-  bool in = t.live;
-#pragma unroll
-  for (int e = 0; e < 3; ++e) {
-    const long long E = (long long)t.dx[e] * (sy - t.ay[e]) - (long long)t.dy[e] * (sx - t.ax[e]);
-    in = in && E >= t.bias[e];
-  }
-  return in;
-}
-
 struct Span {
   int x0, x1, y0, y1;  // inclusive pixel range (empty when x0 > x1 or y0 > y1)
 };
@@ -177,20 +167,6 @@ __device__ Span quad_span(const Frame& f, const Quad& q) {
   s.y0 = max(0, -((224 - mny) >> 8));
   s.y1 = min((int)f.h - 1, (mxy - 32) >> 8);
   return s;
-}
-
-__device__ __forceinline__ void pixel_masks(const Frame& f, const Tri& t0, const Tri& t1, int px,
-                                            int py, uint32_t& m0, uint32_t& m1) {
-  m0 = 0u;
-  m1 = 0u;
-  for (uint32_t k = 0; k < f.msaa; ++k) {
-    int sx, sy;
-    sample_pos(f.msaa, k, sx, sy);
-    sx += px * 256;
-    sy += py * 256;
-    if (tri_in(t0, sx, sy)) m0 |= 1u << k;
-    if (tri_in(t1, sx, sy)) m1 |= 1u << k;
-  }
 }
 
 // x-extent of edge (a, b) inside the horizontal band [yl, yh] (fixed point,
@@ -255,9 +231,47 @@ __device__ void walk_row(const Frame& f, const SegSetup& g, int py, F&& emit) {
   if (lo > hi) return;
   const int x0 = max(g.s.x0, (int)ceil((lo - 4.0 - 224.0) / 256.0));
   const int x1 = min(g.s.x1, (int)floor((hi + 4.0 - 32.0) / 256.0));
+  if (x0 > x1) return;
+  // Edge functions E = dx * (sy - ay) - dy * (sx - ax) of both triangles at
+  // the (up to 4) samples of pixel x0, stepped one pixel right by
+  // subtracting 256 * dy: exact integers, so a sample is inside (E >= bias
+  // on all three edges of a live triangle) exactly when the direct
+  // evaluation says so.
+  long long E[2][3][4], step[2][3];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const Tri& T = t ? g.t1 : g.t0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      step[t][e] = 256ll * T.dy[e];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        int sx = 0, sy = 0;
+        if ((uint32_t)k < f.msaa) sample_pos(f.msaa, (uint32_t)k, sx, sy);
+        sx += x0 * 256;
+        sy += py * 256;
+        E[t][e][k] = (long long)T.dx[e] * (sy - T.ay[e]) - (long long)T.dy[e] * (sx - T.ax[e]);
+      }
+    }
+  }
   for (int px = x0; px <= x1; ++px) {
-    uint32_t m0, m1;
-    pixel_masks(f, g.t0, g.t1, px, py, m0, m1);
+    uint32_t m0 = 0u, m1 = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if ((uint32_t)k >= f.msaa) break;
+      const bool in0 = g.t0.live && E[0][0][k] >= g.t0.bias[0] && E[0][1][k] >= g.t0.bias[1] &&
+                       E[0][2][k] >= g.t0.bias[2];
+      const bool in1 = g.t1.live && E[1][0][k] >= g.t1.bias[0] && E[1][1][k] >= g.t1.bias[1] &&
+                       E[1][2][k] >= g.t1.bias[2];
+      m0 |= in0 ? 1u << k : 0u;
+      m1 |= in1 ? 1u << k : 0u;
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) E[t][e][k] -= step[t][e];
     const uint32_t pix = (uint32_t)py * f.w + (uint32_t)px;
     if (m0 & m1) {
       emit(pix, m0);
