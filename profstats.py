@@ -22,6 +22,9 @@ arguments below match the base name, every template instance of it.
       per hardware queue: busy time and the spans of the ticks that begin at
       each dispatch of <marker> (a kernel run once per iteration), with the
       gaps between them; then the time two or more queues run kernels
+  python profstats.py traffic_series <fetch.csv> <write.csv> <kernels> <n> <copy_bytes>
+      the same counters per dispatch (MB fetched and written per kernel, the
+      last n dispatches in order), as a table
   python profstats.py traffic <fetch.csv> <write.csv> <kernels> <n> <copy_bytes>
       HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
       each scaled by what the counter reads for HIP's device copy of
@@ -186,6 +189,26 @@ def queues(path, t0, t1, marker):
     return "\n".join(lines)
 
 
+def traffic_series(fetch_csv, write_csv, kernels, n, copy_bytes):
+    """Per dispatch (the last n of each kernel, in order): MB fetched and
+    written, each counter scaled as in `traffic`."""
+    cols, heads = [], []
+    for path, counter in ((fetch_csv, "FETCH_SIZE"), (write_csv, "WRITE_SIZE")):
+        rows = _counter(path, counter)
+        calib_kib = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
+        factor = (copy_bytes / 1024.0) / calib_kib
+        for k in kernels.split(","):
+            v = [float(r["Counter_Value"]) for r in rows if _base(r["Kernel_Name"]) == k][-n:]
+            cols.append([x * 1024.0 * factor / 1e6 for x in v])
+            heads.append(f"{k} {'fetch' if counter == 'FETCH_SIZE' else 'write'} MB")
+    lines = ["| dispatch | " + " | ".join(heads) + " | total MB |",
+             "|---|" + "---|" * (len(heads) + 1)]
+    for i in range(min(len(c) for c in cols)):
+        vals = [c[i] for c in cols]
+        lines.append(f"| {i} | " + " | ".join(f"{x:.1f}" for x in vals) + f" | {sum(vals):.1f} |")
+    return "\n".join(lines)
+
+
 def _counter(path, name):
     with open(path) as f:
         return [r for r in csv.DictReader(f) if r["Counter_Name"] == name]
@@ -224,6 +247,9 @@ if __name__ == "__main__":
         print(window(sys.argv[2], float(sys.argv[3]), float(sys.argv[4])))
     elif cmd == "queues":
         print(queues(sys.argv[2], float(sys.argv[3]), float(sys.argv[4]), sys.argv[5]))
+    elif cmd == "traffic_series":
+        print(traffic_series(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]),
+                             int(sys.argv[6])))
     elif cmd == "traffic":
         print(json.dumps(traffic(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]),
                                  int(sys.argv[6])), indent=1))
