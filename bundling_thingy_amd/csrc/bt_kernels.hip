@@ -6,7 +6,8 @@
 //   bin / cell scan / tasks / scatter <- spatial_grid.wgsl:72-138 (count_cells,
 //                                      reserve_cells, scatter_points, prepare_dispatch)
 //   neighbor_mean                    <- neighbor_mean.wgsl:75-203 (tiled + sparse)
-//   blur_rows / blur_cols_peak       <- blur.wgsl:13-83
+//   blur_rows / blur_cols (+ peak),  <- blur.wgsl:13-83
+//   or both in blur_fused
 //   normalize                        <- normalization.wgsl:5-17
 //
 // Arithmetic contract (identical to the CPU oracle): FP32 without contraction
