@@ -529,6 +529,55 @@ def test_edge_case_graph_bitexact(bt, orc, radius):
         np.testing.assert_array_equal(gpu.endpoints(), orc.endpoints(pl.points, paths))
 
 
+def _random_polylines(bt, seed, n_paths=300):
+    """Seeded random polylines: 2-12 control points each, steps from a
+    hundredth of a texel to a third of the map, a fifth of them with a
+    repeated point, some closed (last point = first)."""
+    rng = np.random.default_rng(seed)
+    polys = []
+    for _ in range(n_paths):
+        k = int(rng.integers(2, 13))
+        p = [rng.uniform(0.0, 1.0, 2)]
+        for _ in range(k - 1):
+            step = 10.0 ** rng.uniform(-4.5, -0.5)
+            p.append(p[-1] + rng.normal(0.0, 1.0, 2) * step)
+        if rng.random() < 0.2:
+            j = int(rng.integers(0, k))
+            p.insert(j, p[j].copy())
+        if rng.random() < 0.1:
+            p.append(p[0].copy())
+        polys.append(np.array(p, np.float32))
+    pts = np.concatenate(polys).astype(np.float32)
+    starts = np.cumsum([0] + [len(q) for q in polys]).astype(np.uint32)
+    return bt.Graph(pts, starts)
+
+
+@pytest.mark.parametrize("seed,mode", [(1, "reference"), (2, "reference"), (3, "generalized")])
+def test_random_polylines_bitexact(bt, orc, seed, mode):
+    """Seeded random polylines (joints carrying the resampling remainder,
+    repeated points, closed paths, sub-texel segments): samples, path
+    counts, two iterations and the density planes bit-identical to the
+    oracle, in reference mode (S = 301, radius 12 texels) and with the
+    generalized parameters (spacing 0.004, radius 0.03 -> 0.02, 256^2)."""
+    g = _random_polylines(bt, seed)
+    if mode == "reference":
+        opts = bt.Options(map_size=301, neighbor_radius=12)
+    else:
+        opts = bt.Options(spacing=0.004, radius=(0.03, 0.02), sigma=(3.0, 2.0), density=256,
+                          schedule_len=2)
+    pl = _oracle_pipeline(orc, g, opts)
+    with bt.Gpu(opts) as gpu:
+        n = gpu.upload(g)
+        assert n == len(pl.points)
+        np.testing.assert_array_equal(gpu.paths()[:, 2], pl.counts)
+        _assert_points(gpu.points()[0], pl.points, f"seed {seed}: upload")
+        for it in range(2):
+            gpu.iterate()
+            _assert_points(gpu.points()[0], pl.iterate(opts.iter_params(it)),
+                           f"seed {seed}: iteration {it + 1}")
+            _assert_density(gpu, pl, f"seed {seed}: iteration {it + 1}")
+
+
 def test_second_upload_replaces_the_graph(bt, cpu_case):
     """gpu.rs:494-497: a new upload drops the previous graph (and its
     iteration count); the results equal those of a fresh context."""
