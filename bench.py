@@ -239,15 +239,26 @@ def render_section(bt, gpu, opts, reps=5):
             "ms": ms, "noisiness": m["noisiness"], "noisiness_ms": noise_ms}
 
 
-# (name, policy, tick_samples, lane_samples). FIFO for throughput;
-# shortest-first with 1M sample-updates per tick for latency (shorter ticks
-# let small jobs through sooner, at some throughput); two lanes (jobs of at
-# most 256k samples tick on their own stream, larger ones on another,
-# overlapping on the GPU), each lane's tick taking the least remaining work
-# first up to 1M or 2M sample-updates while no job waits for admission.
-SERVER_POLICIES = (("fifo", "FIFO", 0, 0), ("shortest_1M", "SHORTEST", 1 << 20, 0),
-                   ("lanes_256k_1M", "LANES", 1 << 20, 1 << 18),
-                   ("lanes_256k_2M", "LANES", 2 << 20, 1 << 18))
+# (name, policy, tick_samples, lane_samples, small_cu_mask, large_cu_mask).
+# FIFO for throughput; shortest-first with 1M sample-updates per tick for
+# latency (shorter ticks let small jobs through sooner, at some throughput);
+# two lanes (jobs of at most 256k samples tick on their own stream, larger
+# ones on another, overlapping on the GPU), each lane's tick taking the least
+# remaining work first up to 1M sample-updates while no job waits for
+# admission: on any CU, or with the small lane on shader engine 3 of every
+# XCD and the large lane on shader engines 0-2 while both have work (each
+# lane on every CU while the other has none).
+SE3, SE012 = 0xFF000000, 0x00FFFFFF
+SERVER_POLICIES = (("fifo", "FIFO", 0, 0, 0, 0), ("shortest_1M", "SHORTEST", 1 << 20, 0, 0, 0),
+                   ("lanes_256k_1M", "LANES", 1 << 20, 1 << 18, 0, 0),
+                   ("lanes_256k_1M_se", "LANES", 1 << 20, 1 << 18, SE3, SE012))
+# Each server is warmed before its measured run with jobs of another seed (a
+# long-running server has its allocator slabs and pinned result buffers in
+# place); finished jobs are read into two reused host buffers.
+WARM_JOBS = 64
+_REC_BUF = None
+_DENS_BUF = None
+_WARM = {}
 # A second, lighter arrival stream (the same jobs at half the rate) for the
 # latency-load relation, under the final-only density policy.
 LIGHT_RATE_FACTOR = 0.5
@@ -265,16 +276,14 @@ def _flagged(bt, opts, flags):
     return o
 
 
-def serve(bt, jobs, policy, tick, flags=0, realtime=True, lane=0):
-    """Feeds the jobs to a server: in real time at their arrival times
-    (realtime) or all at once (saturated: the server's capacity). Each
-    finished job's samples and density are read into host arrays, as the
-    plain per-job form reads them. Returns (makespan s, [(job index, latency
-    ms)], ticks, (start, end) on the monotonic clock); latency = completion -
-    arrival (saturated: - start)."""
+def _drive(bt, srv, jobs, flags, realtime):
+    """Runs the jobs through the server (see serve); returns (makespan s,
+    [(job index, latency ms)], ticks, (start, end))."""
+    global _REC_BUF, _DENS_BUF
+    if _REC_BUF is None:
+        _REC_BUF = np.zeros(4 << 20, dtype=np.uint8)
+        _DENS_BUF = np.zeros(2048 * 2048, dtype=np.float32)
     n_jobs = len(jobs)
-    srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy,
-                    tick_samples=tick, lane_samples=lane)
     clk0 = srv.clock_ms()
     t0 = time.perf_counter()
     pending, slat, i, done = {}, [], 0, 0
@@ -295,15 +304,37 @@ def serve(bt, jobs, policy, tick, flags=0, realtime=True, lane=0):
                 raise RuntimeError(f"served job {k} failed")
             slat.append((k, r.done_ms - (clk0 + ta * 1e3)))
             oc = jobs[k][2].c
-            srv.points(tk, r.samples)
-            srv.density(tk, oc.density_w, oc.density_h)
+            if 12 * r.samples > _REC_BUF.size:
+                _REC_BUF = np.zeros(12 * r.samples, dtype=np.uint8)
+            srv.points_raw(tk, r.samples, _REC_BUF)
+            srv.density(tk, oc.density_w, oc.density_h, _DENS_BUF)
             srv.release(tk)
             done += 1
         if running == 0 and queued == 0 and i < n_jobs:
             time.sleep(max(0.0, jobs[i][0] - (time.perf_counter() - t0)))
     t1 = time.perf_counter()
-    srv.close()
     return t1 - t0, slat, ticks, (t0, t1)
+
+
+def serve(bt, workloads, jobs, policy, tick, flags=0, realtime=True, lane=0, masks=(0, 0)):
+    """Feeds the jobs to a server, warmed first with WARM_JOBS jobs of
+    another seed (all at once, not measured): in real time at their arrival
+    times (realtime) or all at once (saturated: the server's capacity). Each
+    finished job's samples and density are read into host arrays, as the
+    plain per-job form reads them. Returns (makespan s, [(job index, latency
+    ms)], ticks, (start, end) on the monotonic clock); latency = completion -
+    arrival (saturated: - start)."""
+    srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy,
+                    tick_samples=tick, lane_samples=lane, small_cu_mask=masks[0],
+                    large_cu_mask=masks[1])
+    if WARM_JOBS:
+        if _WARM.get("jobs") is None:
+            _WARM["jobs"] = workloads.served_jobs(WARM_JOBS, seed=77)
+        _drive(bt, srv, _WARM["jobs"], flags, False)
+    try:
+        return _drive(bt, srv, jobs, flags, realtime)
+    finally:
+        srv.close()
 
 
 def profile_batched(bt, workloads, n_graphs=256, iters=30):
@@ -326,18 +357,21 @@ def profile_served(bt, workloads, n_jobs=1000, rate_hz=200.0):
     """For rocprofv3 runs: the served stream (final-only density) under each
     server policy, in real time; prints each run's window."""
     jobs = workloads.served_jobs(n_jobs, rate_hz)
-    serve(bt, jobs[:3], bt.Server.FIFO, 0, 4)  # warm the code paths
-    for name, policy, tick, lane in SERVER_POLICIES:
-        srv_s, _, ticks, (t0, t1) = serve(bt, jobs, getattr(bt.Server, policy), tick, 4,
-                                          lane=lane)
+    serve(bt, workloads, jobs[:3], bt.Server.FIFO, 0, 4)  # warm the code paths
+    for name, policy, tick, lane, m_small, m_large in SERVER_POLICIES:
+        srv_s, _, ticks, (t0, t1) = serve(bt, workloads, jobs, getattr(bt.Server, policy), tick,
+                                          4, lane=lane, masks=(m_small, m_large))
         log(f"served {name}: makespan {srv_s:.3f} s, {ticks} ticks, window {t0:.4f} {t1:.4f}")
 
 
 def _plain_per_job(bt, jobs, flags):
     """The plain per-job drop-in over the jobs, one after another with no
     waiting (its capacity): upload, one blocking bt_iterate per iteration,
-    read the samples and the density. Returns (seconds, [service seconds],
+    read the samples and the density (into two reused host buffers, as the
+    server's results are read). Returns (seconds, [service seconds],
     sample-updates)."""
+    rec = np.zeros(12 * 4 << 20, dtype=np.uint8)
+    dens = np.zeros(2048 * 2048, dtype=np.float32)
     gpu = bt.Gpu()
     gpu.upload_batch([jobs[0][1]], [_flagged(bt, jobs[0][2], flags)])
     gpu.iterate(2)
@@ -348,8 +382,8 @@ def _plain_per_job(bt, jobs, flags):
         ts = time.perf_counter()
         n = gpu.upload_batch([g], [_flagged(bt, o, flags)])
         gpu.iterate(it)
-        gpu.points(0)
-        gpu.density(3, 0)
+        gpu.points_raw(0, rec)
+        gpu.density(3, 0, dens)
         service.append(time.perf_counter() - ts)
         updates += it * int(n[0])
     total = time.perf_counter() - t0
@@ -389,7 +423,7 @@ def served_section(bt, workloads, n_jobs, rate_hz=200.0):
 
     out = {"jobs": n_jobs, "rate_hz": rate_hz, "jobs_le20k_edges": len(small),
            "spacing": 0.01, "arrivals_s": arrivals[-1]}
-    serve(bt, jobs[:3], bt.Server.FIFO, 0, 4)  # warm the code paths
+    serve(bt, workloads, jobs[:3], bt.Server.FIFO, 0, 4)  # warm the code paths
     for dname, flags in DENSITY_POLICIES:
         plain_s, service, updates = _plain_per_job(bt, jobs, flags)
         out["updates"] = updates
@@ -403,10 +437,12 @@ def served_section(bt, workloads, n_jobs, rate_hz=200.0):
                          "isolated_p50_ms_le20k_edges": pct(iso, 50),
                          "isolated_p99_ms_le20k_edges": pct(iso, 99)}}
         log(f"served [{dname}]: plain per-job {plain_s:.2f} s, {plain_cap:.3e} updates/s")
-        for name, policy, tick, lane in SERVER_POLICIES:
+        for name, policy, tick, lane, m_small, m_large in SERVER_POLICIES:
             pol = getattr(bt.Server, policy)
-            cap_s, _, cap_ticks, _ = serve(bt, jobs, pol, tick, flags, realtime=False, lane=lane)
-            rt_s, rt_lat, rt_ticks, _ = serve(bt, jobs, pol, tick, flags, lane=lane)
+            kw = dict(lane=lane, masks=(m_small, m_large))
+            cap_s, _, cap_ticks, _ = serve(bt, workloads, jobs, pol, tick, flags, realtime=False,
+                                           **kw)
+            rt_s, rt_lat, rt_ticks, _ = serve(bt, workloads, jobs, pol, tick, flags, **kw)
             sec[name] = {"capacity_updates_per_s": updates / cap_s, "capacity_seconds": cap_s,
                          "capacity_vs_plain": plain_s / cap_s, "capacity_ticks": cap_ticks,
                          "stream_updates_per_s": updates / rt_s, "stream_makespan_s": rt_s,
@@ -415,12 +451,14 @@ def served_section(bt, workloads, n_jobs, rate_hz=200.0):
                 sec[name]["tick_samples"] = tick
             if lane:
                 sec[name]["lane_samples"] = lane
+            if m_small or m_large:
+                sec[name]["cu_masks"] = [hex(m_small), hex(m_large)]
             log(f"served [{dname}] {name}: capacity {updates / cap_s:.3e} updates/s "
                 f"({plain_s / cap_s:.2f}x plain); stream p99 {sec[name]['p99_ms']:.1f} ms, "
                 f"jobs <= 20k edges p99 {sec[name]['p99_ms_le20k_edges']:.1f} ms")
             if flags & 4:
                 light = [(ta / LIGHT_RATE_FACTOR, g, o, it) for (ta, g, o, it) in jobs]
-                l_s, l_lat, _, _ = serve(bt, light, pol, tick, flags, lane=lane)
+                l_s, l_lat, _, _ = serve(bt, workloads, light, pol, tick, flags, **kw)
                 sec[name]["light_stream"] = {"rate_hz": rate_hz * LIGHT_RATE_FACTOR,
                                              "makespan_s": l_s, **lat_stats(l_lat)}
                 log(f"served [{dname}] {name} at {rate_hz * LIGHT_RATE_FACTOR:.0f} jobs/s: "

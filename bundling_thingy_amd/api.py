@@ -83,7 +83,8 @@ class bt_graph_desc(C.Structure):
 class bt_server_options(C.Structure):
     _fields_ = [("defaults", bt_options), ("memory_budget", C.c_uint64),
                 ("max_inflight", C.c_uint32), ("policy", C.c_uint32),
-                ("tick_samples", C.c_uint64), ("lane_samples", C.c_uint64)]
+                ("tick_samples", C.c_uint64), ("lane_samples", C.c_uint64),
+                ("small_cu_mask", C.c_uint32), ("large_cu_mask", C.c_uint32)]
 
 
 class bt_server_info(C.Structure):
@@ -514,9 +515,12 @@ class Gpu:
         self._check(lib().bt_read_paths_job(self._h, job, _u32p(out), n))
         return out
 
-    def points_raw(self, job=0):
+    def points_raw(self, job=0, out=None):
+        """The job's 12-byte sample records, into `out` (a contiguous buffer of
+        at least 12 N bytes, reused by a serving loop) or a new array."""
         n, _ = self.job_samples(job)
-        buf = np.empty(n * 12, dtype=np.uint8)
+        buf = np.empty(n * 12, dtype=np.uint8) if out is None else \
+            np.frombuffer(out, dtype=np.uint8, count=n * 12)
         self._check(lib().bt_read_points_job(self._h, job, buf.ctypes.data_as(_P), n))
         return buf
 
@@ -536,13 +540,16 @@ class Gpu:
     def distortion_snapshot(self, layout, job=0):
         return layout.decode(self.points_raw(job).tobytes())
 
-    def density(self, kind=DENSITY_NORMALIZED, job=0):
+    def density(self, kind=DENSITY_NORMALIZED, job=0, out=None):
+        """The job's density plane `kind` (h, w), into `out` (a contiguous
+        buffer of at least w * h values of the kind's type) or a new array."""
         p = self._opts[job].iter_params(0)
         w, h = p.density_w, p.density_h
         dt = {DENSITY_COUNTS: np.uint32, DENSITY_HALF: np.uint16}.get(kind, np.float32)
-        out = np.zeros(w * h, dtype=dt)
-        self._check(lib().bt_read_density_job(self._h, job, kind, out.ctypes.data_as(_P), w * h))
-        return out.reshape(h, w)
+        buf = np.zeros(w * h, dtype=dt) if out is None else \
+            np.frombuffer(out, dtype=dt, count=w * h)
+        self._check(lib().bt_read_density_job(self._h, job, kind, buf.ctypes.data_as(_P), w * h))
+        return buf.reshape(h, w)
 
     def copy_probe(self, nbytes):
         """Device-to-device copy of nbytes (profiler calibration); returns ms."""
@@ -659,12 +666,14 @@ class Server:
     jobs of at most lane_samples samples tick on one stream, larger ones on
     another, the two overlapping on the GPU; with tick_samples, each lane's
     tick takes the least remaining work first up to tick_samples while no
-    job waits for admission)."""
+    job waits for admission; small_cu_mask / large_cu_mask keep the lanes on
+    the CUs they select (one bit per CU of every group of 32) while both
+    lanes have work)."""
 
     FIFO, SHORTEST, LANES = 0, 1, 2
 
     def __init__(self, options=None, memory_budget=64 << 30, max_inflight=64, policy=0,
-                 tick_samples=0, lane_samples=0):
+                 tick_samples=0, lane_samples=0, small_cu_mask=0, large_cu_mask=0):
         so = bt_server_options()
         so.defaults = (options or Options()).c
         so.memory_budget = memory_budget
@@ -672,6 +681,8 @@ class Server:
         so.policy = policy
         so.tick_samples = tick_samples
         so.lane_samples = lane_samples
+        so.small_cu_mask = small_cu_mask
+        so.large_cu_mask = large_cu_mask
         h = _P()
         rc = lib().bt_server_create(C.byref(so), C.byref(h))
         if rc:
@@ -708,16 +719,27 @@ class Server:
         return res
 
     def points(self, ticket, samples):
-        buf = np.zeros(samples * 12, dtype=np.uint8)
-        self._check(lib().bt_server_read_points(self._h, ticket, buf.ctypes.data_as(_P), samples))
-        r = buf.view(np.dtype([("x", "<f4"), ("y", "<f4"), ("id", "<u4")]))
+        r = self.points_raw(ticket, samples)
         return np.stack([r["x"], r["y"]], 1), r["id"].copy()
 
-    def density(self, ticket, w, h):
-        out = np.zeros(w * h, dtype=np.float32)
+    def points_raw(self, ticket, samples, out=None):
+        """The job's 12-byte sample records {x, y, edge id}, copied into `out`
+        (a contiguous buffer of at least samples * 12 bytes, reused by a
+        serving loop) or a new array."""
+        rec = np.dtype([("x", "<f4"), ("y", "<f4"), ("id", "<u4")])
+        buf = np.empty(samples * 12, dtype=np.uint8) if out is None else \
+            np.frombuffer(out, dtype=np.uint8, count=samples * 12)
+        self._check(lib().bt_server_read_points(self._h, ticket, buf.ctypes.data_as(_P), samples))
+        return buf.view(rec)
+
+    def density(self, ticket, w, h, out=None):
+        """The job's normalized density (h, w), copied into `out` (a
+        contiguous float32 buffer of at least w * h values) or a new array."""
+        buf = np.empty(w * h, dtype=np.float32) if out is None else \
+            np.frombuffer(out, dtype=np.float32, count=w * h)
         self._check(lib().bt_server_read_density(self._h, ticket, DENSITY_NORMALIZED,
-                                                 out.ctypes.data_as(_P), w * h))
-        return out.reshape(h, w)
+                                                 buf.ctypes.data_as(_P), w * h))
+        return buf.reshape(h, w)
 
     def release(self, ticket):
         self._check(lib().bt_server_release(self._h, ticket))

@@ -586,11 +586,25 @@ int bt_server_create(const bt_server_options* opts, bt_server** out) {
   s->o = *opts;
   if (s->o.max_inflight == 0) s->o.max_inflight = 64;
   if (s->o.memory_budget == 0) s->o.memory_budget = 64ull << 30;
+  // A CU mask must leave CUs on every XCD: the hardware deals workgroups to
+  // the XCDs round-robin, and bit b of each 32-bit group selects a CU of XCD
+  // b mod 8, so every residue mod 8 needs a set bit (workgroups dealt to an
+  // XCD without CUs would never run).
+  for (uint32_t m : {s->o.small_cu_mask, s->o.large_cu_mask}) {
+    if (!m) continue;
+    uint32_t seen = 0;
+    for (uint32_t b = 0; b < 32; ++b)
+      if (m >> b & 1u) seen |= 1u << (b % BT_XCDS);
+    if (seen != (1u << BT_XCDS) - 1u) return BT_E_INVALID;
+  }
   std::string err;
   int rc = guarded(&err, [&] {
     const int lanes = s->o.policy == BT_POLICY_LANES ? 2 : 1;
-    for (int l = 0; l < lanes; ++l)
-      s->engines.push_back(std::make_unique<bt::Engine>(bt::resolve_options(&s->o.defaults)));
+    for (int l = 0; l < lanes; ++l) {
+      const uint32_t mask = lanes == 1 ? 0u : (l == 0 ? s->o.small_cu_mask : s->o.large_cu_mask);
+      s->engines.push_back(
+          std::make_unique<bt::Engine>(bt::resolve_options(&s->o.defaults), mask));
+    }
     s->ticks.resize(lanes);
   });
   if (rc != BT_OK) return rc;
@@ -785,6 +799,15 @@ int bt_server_step(bt_server* srv, uint32_t* running, uint32_t* queued) {
         srv->used += s->estimate;
         srv->running.push_back(s);
       }
+    }
+    // Lanes with CU masks: a lane runs on every CU while the other lane has
+    // no job running or queued (its masked CUs would idle), on its mask
+    // otherwise.
+    if (lanes == 2 && (srv->engines[0]->masked() || srv->engines[1]->masked())) {
+      bool busy[2] = {false, false};
+      for (const ServedJob* s : srv->running) busy[s->lane] = true;
+      for (const ServedJob* s : srv->queue) busy[s->lane] = true;
+      for (uint32_t lane = 0; lane < 2; ++lane) srv->engines[lane]->set_wide(!busy[1 - lane]);
     }
     // One iteration of the advanced jobs in one batched launch per stage
     // (per lane).

@@ -171,7 +171,7 @@ size_t Job::footprint(const bt_options& o, uint64_t n, uint32_t n_points, uint32
   return s;
 }
 
-Engine::Engine(const bt_options& o) : opts_(o), pool_(new DevPool) {
+Engine::Engine(const bt_options& o, uint32_t cu_mask) : opts_(o), pool_(new DevPool) {
   device_ = o.device;
   HIPCHK(hipSetDevice(device_));
   hipDeviceProp_t prop;
@@ -180,7 +180,26 @@ Engine::Engine(const bt_options& o) : opts_(o), pool_(new DevPool) {
   for (uint32_t k = 0; k < BT_NM_SHAPE_COUNT; ++k)
     nm_per_cu_[k] = btk::neighbor_mean_blocks_per_cu(k);
   noalloc_setup(device_);  // the allocation-free entry point's scratch on this device
-  HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (cu_mask && cu_mask != 0xffffffffu) {
+    // Every group of 32 CUs gets the same pattern, so every XCD keeps CUs
+    // (workgroups are dealt to the XCDs round-robin); persistent grids are
+    // sized to the CUs the stream may use.
+    uint32_t xcds = 0;
+    for (uint32_t b = 0; b < 32; ++b)
+      if (cu_mask >> b & 1u) xcds |= 1u << (b % BT_XCDS);
+    if (xcds != (1u << BT_XCDS) - 1u) fail(BT_E_INVALID, "CU mask leaves an XCD without CUs");
+    const uint32_t words = ((uint32_t)cus_ + 31u) / 32u;
+    std::vector<uint32_t> m(words, cu_mask);
+    HIPCHK(hipExtStreamCreateWithCUMask(&streams_[0], words, m.data()));
+    HIPCHK(hipStreamCreateWithFlags(&streams_[1], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&switch_ev_, hipEventDisableTiming));
+    stream_cus_[1] = cus_;
+    cus_ = std::max(1, (int)(words * (uint32_t)__builtin_popcount(cu_mask)));
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&streams_[0], hipStreamNonBlocking));
+  }
+  stream_cus_[0] = cus_;
+  stream_ = streams_[0];
   ctl_.alloc(sizeof(BtControl));
   HIPCHK(hipMemset(ctl_.p, 0, sizeof(BtControl)));
   split_.alloc(kSplitBytes);
@@ -192,12 +211,28 @@ Engine::Engine(const bt_options& o) : opts_(o), pool_(new DevPool) {
   retire_plans_.alloc(kRetireSlots * kRetireSlotBytes);
 }
 
+void Engine::set_wide(bool wide) {
+  const int want = wide && streams_[1] ? 1 : 0;
+  if (want == active_) return;
+  HIPCHK(hipEventRecord(switch_ev_, streams_[active_]));
+  HIPCHK(hipStreamWaitEvent(streams_[want], switch_ev_, 0));
+  active_ = want;
+  stream_ = streams_[active_];
+  cus_ = stream_cus_[active_];
+}
+
 Engine::~Engine() {
   drop_graphs();
-  if (stream_) {
-    (void)hipStreamSynchronize(stream_);
-    (void)hipStreamDestroy(stream_);
-  }
+  // the active stream is ordered after everything queued on the other
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (hipStream_t& st : streams_)
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+      st = nullptr;
+    }
+  stream_ = nullptr;
+  if (switch_ev_) (void)hipEventDestroy(switch_ev_);
   for (Pinned& s : pinned_) {
     if (s.ev) (void)hipEventDestroy(s.ev);
     if (s.p) (void)hipHostFree(s.p);

@@ -91,7 +91,10 @@ struct GraphIn {
 
 class Engine {
  public:
-  explicit Engine(const bt_options& o);
+  // cu_mask: 0 for every CU, else the CUs the engine's stream may use, one
+  // bit per CU of each group of 32 (the same pattern in every group; on
+  // MI355X each byte of it is one shader engine of every XCD).
+  explicit Engine(const bt_options& o, uint32_t cu_mask = 0);
   ~Engine();
 
   const bt_options& options() const { return opts_; }
@@ -163,6 +166,11 @@ class Engine {
   Job& job(uint32_t i);
   size_t device_bytes() const;
   hipStream_t stream() const { return stream_; }
+  // With a CU mask: move the engine's work to the stream over every CU
+  // (wide) or back to the masked one. The new stream first waits for all
+  // work queued on the old one, so the two act as one ordered stream.
+  void set_wide(bool wide);
+  bool masked() const { return streams_[1] != nullptr; }
   void sync();
 
   std::string err;
@@ -191,6 +199,10 @@ class Engine {
   int cus_ = 256;
   uint32_t nm_per_cu_[BT_NM_SHAPE_COUNT] = {};  // resident neighbor-mean workgroups per CU
   hipStream_t stream_ = nullptr;
+  hipStream_t streams_[2] = {nullptr, nullptr};  // [0] masked (or the only one), [1] every CU
+  int stream_cus_[2] = {0, 0};
+  int active_ = 0;
+  hipEvent_t switch_ev_ = nullptr;
   std::vector<std::unique_ptr<Job>> jobs_;
   DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_, retire_plans_;
   struct Pinned {

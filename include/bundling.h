@@ -366,7 +366,9 @@ typedef struct bt_server bt_server;
                                  all of its lane's running jobs; otherwise, while no job waits
                                  for admission, it takes the lane's jobs with the least
                                  remaining work first up to tick_samples sample-updates, and
-                                 all of them when a backlog forms (throughput first) */
+                                 all of them when a backlog forms (throughput first). With
+                                 CU masks (small_cu_mask, large_cu_mask) the lanes run on
+                                 disjoint CUs while both have work */
 
 typedef struct bt_server_options {
   bt_options defaults;       /* device, flags */
@@ -378,6 +380,13 @@ typedef struct bt_server_options {
                                 (0 = every running job) */
   uint64_t lane_samples;     /* BT_POLICY_LANES: largest job (samples, from the count pass
                                 at submission) of the small-job lane (0 = 1M) */
+  uint32_t small_cu_mask;    /* BT_POLICY_LANES: CUs the small-job lane may use while the
+                                large-job lane has work, one bit per CU of every group of
+                                32 (0 = all). Bit b selects a CU of XCD b mod 8, so every
+                                residue mod 8 needs a set bit (else BT_E_INVALID); on
+                                MI355X each byte is one shader engine of every XCD */
+  uint32_t large_cu_mask;    /* the same for the large-job lane. A lane runs on every CU
+                                while the other lane has no job running or queued */
 } bt_server_options;
 
 typedef struct bt_job_result {

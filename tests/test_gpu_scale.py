@@ -168,13 +168,15 @@ def served_case(bt):
     return jobs, alone
 
 
-def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=(), flags=0, lane=0):
+def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=(), flags=0, lane=0,
+                      masks=(0, 0)):
     """Submits `per_step` arrivals before each tick (about one tick's worth of
     the 200 jobs/s stream at full load), so admissions keep arriving while
     earlier jobs run. Returns {job index: (positions hash, density hash)} and
     {job index: (positions, density)} for the jobs in `keep`."""
     srv = bt.Server(bt.Options(), memory_budget=96 << 30, max_inflight=128, policy=policy,
-                    tick_samples=tick, lane_samples=lane)
+                    tick_samples=tick, lane_samples=lane, small_cu_mask=masks[0],
+                    large_cu_mask=masks[1])
     try:
         pending, out, raw, i = {}, {}, {}, 0
         for _ in range(100000):
@@ -210,18 +212,23 @@ def _serve_compressed(bt, jobs, policy, tick, per_step=3, keep=(), flags=0, lane
         srv.close()
 
 
-@pytest.mark.parametrize("policy,tick,lane,flags", [(0, 0, 0, 0), (1, 1 << 20, 0, 0),
-                                                    (0, 0, 0, 4), (1, 1 << 20, 0, 4),
-                                                    (2, 0, 1 << 18, 0), (2, 1 << 20, 1 << 18, 4)])
-def test_served_stream_equals_alone(bt, served_case, policy, tick, lane, flags):
+SE3, SE012 = 0xFF000000, 0x00FFFFFF  # shader engine 3 / 0-2 of every XCD
+
+
+@pytest.mark.parametrize("policy,tick,lane,flags,masks", [
+    (0, 0, 0, 0, (0, 0)), (1, 1 << 20, 0, 0, (0, 0)), (0, 0, 0, 4, (0, 0)),
+    (1, 1 << 20, 0, 4, (0, 0)), (2, 0, 1 << 18, 0, (0, 0)), (2, 1 << 20, 1 << 18, 4, (0, 0)),
+    (2, 1 << 20, 1 << 18, 4, (SE3, SE012)), (2, 0, 1 << 18, 0, (SE012, SE3))])
+def test_served_stream_equals_alone(bt, served_case, policy, tick, lane, flags, masks):
     """Every job of the 100-job served stream, co-scheduled with the others
     under continuous batching (FIFO, shortest-first with 1M sample-updates
     per tick, and two lanes split at 256k samples, without and with a 1M
-    per-lane tick budget), ends with the bits of the job alone, whether the
-    jobs compute their density every iteration or only at the last one
-    (flags 4 = BT_FLAG_DENSITY_ON_READ)."""
+    per-lane tick budget, on any CU or with the lanes on disjoint shader
+    engines while both have work), ends with the bits of the job alone,
+    whether the jobs compute their density every iteration or only at the
+    last one (flags 4 = BT_FLAG_DENSITY_ON_READ)."""
     jobs, alone = served_case
-    out, _ = _serve_compressed(bt, jobs, policy, tick, flags=flags, lane=lane)
+    out, _ = _serve_compressed(bt, jobs, policy, tick, flags=flags, lane=lane, masks=masks)
     for k in range(len(jobs)):
         assert out[k] == alone[k], f"job {k}"
 

@@ -252,3 +252,16 @@ def test_layout_rejects_bad_reservations(bt):
     rec["id"][4] = 0
     with pytest.raises(bt.BtError):
         L.decode(rec.tobytes())
+
+
+@pytest.mark.parametrize("small,large", [(0x0F000000, 0), (0, 0xF0F0F0F0), (0x01010101, 0)])
+def test_server_refuses_cu_masks_that_leave_an_xcd_empty(bt, small, large):
+    """A lane's CU mask selects, with bit b of every 32-bit group, a CU of
+    XCD b mod 8 (the hardware deals workgroups to the 8 XCDs round-robin),
+    so a mask without a set bit in some residue class mod 8 would leave
+    that XCD's workgroups nowhere to run: creation refuses it before any
+    device call."""
+    from bundling_thingy_amd import api
+    with pytest.raises(api.BtError) as e:
+        bt.Server(policy=bt.Server.LANES, small_cu_mask=small, large_cu_mask=large)
+    assert e.value.code == api.BT_E_INVALID
