@@ -60,7 +60,7 @@ class bt_info(C.Structure):
                 ("reserved_bytes", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("split_tasks", C.c_uint32), ("split_parts", C.c_uint32),
                 ("test_steps", C.c_uint64), ("neighbors", C.c_uint64),
-                ("candidates", C.c_uint64)]
+                ("candidates", C.c_uint64), ("group_steps", C.c_uint64)]
 
 
 class bt_render_options(C.Structure):

@@ -261,7 +261,7 @@ int bt_get_info(bt_ctx* ctx, bt_info* info) {
                  [&] {
                    ctx->engine->split_counts(&info->split_tasks, &info->split_parts);
                    ctx->engine->work_counts(&info->test_steps, &info->neighbors,
-                                            &info->candidates);
+                                            &info->candidates, &info->group_steps);
                  });
 }
 

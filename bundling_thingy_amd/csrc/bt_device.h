@@ -151,6 +151,7 @@ struct BtSplit {          // scratch of the heavy-task split (one per launch)
   unsigned long long* wg_nbr;     // [BT_MAX_WG]
   unsigned long long* wg_hsteps;  // [BT_MAX_WG]
   unsigned long long* wg_hnbr;    // [BT_MAX_WG]
+  unsigned long long* wg_gsteps;  // [BT_MAX_WG]: of wg_steps, those of lane-group streams
 };
 
 struct __attribute__((aligned(16))) BtTask {

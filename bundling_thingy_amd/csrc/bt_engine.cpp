@@ -32,7 +32,7 @@ static const size_t kSplitAccOff = kSplitItemsOff + (size_t)BT_MAX_ITEMS * sizeo
 static const size_t kSplitCandOff = kSplitAccOff + (size_t)BT_MAX_HEAVY * 192 * sizeof(long long);
 static const size_t kSplitRecOff = kSplitCandOff + (size_t)BT_MAX_WG * sizeof(unsigned long long);
 static const size_t kSplitWorkOff = kSplitRecOff + (size_t)BT_MAX_HEAVY * BT_REC_WORDS * 4;
-static const size_t kSplitBytes = kSplitWorkOff + 4 * (size_t)BT_MAX_WG * sizeof(unsigned long long);
+static const size_t kSplitBytes = kSplitWorkOff + 5 * (size_t)BT_MAX_WG * sizeof(unsigned long long);
 
 BtJob Job::dev(uint32_t it, uint32_t parity_at, bool force_density) const {
   BtJob d;
@@ -254,26 +254,28 @@ void Engine::split_counts(uint32_t* tasks, uint32_t* parts) {
   *parts = std::min(w[2], (uint32_t)BT_MAX_ITEMS);
 }
 
-void Engine::work_counts(uint64_t* steps, uint64_t* neighbors, uint64_t* candidates) {
+void Engine::work_counts(uint64_t* steps, uint64_t* neighbors, uint64_t* candidates,
+                         uint64_t* group_steps) {
   const size_t wg = std::min<size_t>(last_nm_grid_, BT_MAX_WG);
   if (wg == 0) {
-    *steps = *neighbors = *candidates = 0;
+    *steps = *neighbors = *candidates = *group_steps = 0;
     return;
   }
-  std::vector<unsigned long long> h(5 * wg);
+  std::vector<unsigned long long> h(6 * wg);
   const char* base = split_.as<char>();
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 5; ++k)
     HIPCHK(hipMemcpyAsync(h.data() + k * wg, base + kSplitWorkOff + (size_t)k * BT_MAX_WG * 8,
                           wg * 8, hipMemcpyDeviceToHost, stream_));
-  HIPCHK(hipMemcpyAsync(h.data() + 4 * wg, base + kSplitCandOff, wg * 8, hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(h.data() + 5 * wg, base + kSplitCandOff, wg * 8, hipMemcpyDeviceToHost,
                         stream_));
   sync();
-  uint64_t a[5] = {0, 0, 0, 0, 0};
-  for (int k = 0; k < 5; ++k)
+  uint64_t a[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < 6; ++k)
     for (size_t b = 0; b < wg; ++b) a[k] += h[k * wg + b];
   *steps = a[0] + a[2];
   *neighbors = a[1] + a[3];
-  *candidates = a[4];
+  *candidates = a[5];
+  *group_steps = a[4] + a[2];  // the heavy kernel's parts stream per lane group
 }
 
 void Engine::drop_graphs() {
@@ -464,6 +466,7 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   split.wg_nbr = work + BT_MAX_WG;
   split.wg_hsteps = work + 2 * BT_MAX_WG;
   split.wg_hnbr = work + 3 * BT_MAX_WG;
+  split.wg_gsteps = work + 4 * BT_MAX_WG;
   const uint32_t shape = p.kshape & 0xffu;
   last_nm_grid_ = (uint32_t)cus_ * nm_per_cu_[shape];
   btk::neighbor_mean(stream_, last_nm_grid_, J, tasks_.as<BtTask>(), ctl, split, p.samples,

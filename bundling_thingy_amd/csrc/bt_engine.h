@@ -160,7 +160,8 @@ class Engine {
   // Executed work of the last neighbor-mean launch: wave-steps (one
   // candidate tested against the 64 lanes' samples), neighbors found for
   // the movable samples, and the candidates of the tasks' streams.
-  void work_counts(uint64_t* steps, uint64_t* neighbors, uint64_t* candidates);
+  void work_counts(uint64_t* steps, uint64_t* neighbors, uint64_t* candidates,
+                   uint64_t* group_steps);
 
   size_t njobs() const { return jobs_.size(); }
   Job& job(uint32_t i);

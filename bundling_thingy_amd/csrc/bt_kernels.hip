@@ -565,16 +565,29 @@ __device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total
 #define NM_GCHUNK 64        // candidates per group per chunk of the grouped stream
 #define NM_GSTRIDE 65       // float4 per lane group in a tile (one of padding: no bank conflicts)
 #define NM_GROUP_MIN 2048u  // candidates above which a task streams per lane group
+#define NM_MF_ROWS 9        // B columns of the matrix-instruction sums: 4 + 4 digits, count
 
 // Kernel shape, a template parameter the autotuner chooses per iteration:
-// lanes per group of the grouped stream (16: 4 groups, 32: 2 groups) and
-// waves per workgroup. Results do not depend on it (exact sums).
-template <int GL, int W>
+// lanes per group of the grouped stream (16: 4 groups, 32: 2 groups), waves
+// per workgroup, and whether the grouped stream adds its accepted
+// candidates with matrix instructions (MF, 16-lane groups only). Results do
+// not depend on it (exact sums).
+template <int GL, int W, bool MF = false>
 struct NmCfg {
   static constexpr uint32_t GLANES = GL;
   static constexpr uint32_t GROUPS = 64 / GL;
   static constexpr uint32_t WAVES = W;
+  static constexpr bool MFMA = MF;
   static constexpr uint32_t TILE = GROUPS * NM_GSTRIDE > 128 ? GROUPS * NM_GSTRIDE : 128;
+  // MF stream, per wave: the digit planes of B (GROUPS x NM_MF_ROWS x 64
+  // bytes), the candidates' positions (GROUPS x 64 float2), and (over both,
+  // between chunks) the 4 x 16 x 16 int32 results of the matrix products
+  static constexpr uint32_t MF_BT = GROUPS * NM_MF_ROWS * NM_GCHUNK;
+  static constexpr uint32_t MF_BYTES = MF_BT + GROUPS * NM_GCHUNK * 8u;
+  static constexpr uint32_t WAVE_BYTES_0 = TILE * 16u;
+  static constexpr uint32_t WAVE_BYTES_1 = MF ? (MF_BYTES > 4096u ? MF_BYTES : 4096u) : 0u;
+  static constexpr uint32_t WAVE_BYTES = WAVE_BYTES_0 > WAVE_BYTES_1 ? WAVE_BYTES_0 : WAVE_BYTES_1;
+  static_assert(!MF || GL == 16, "matrix-instruction sums need 16-lane groups");
   // record of a heavy task's set-up (BT_REC_WORDS words): per lane group its
   // ranges (start[128], pref[129]), then stream lengths m[GROUPS], range
   // counts n[GROUPS], then per group the inside sums x, y, count (int64)
@@ -587,13 +600,15 @@ struct NmCfg {
   static_assert(GL == 16 || GL == 32, "lane groups of 16 or 32");
 };
 
-// The tile holds either two chunks of 64 candidates (plain stream, double
-// buffered) or one chunk of 64 candidates per lane group (grouped stream).
+// Per wave, `raw` holds either two chunks of 64 candidates (plain stream,
+// double buffered) or one chunk of 64 candidates per lane group (grouped
+// stream), as float4 records, or the matrix-instruction stream's layout.
 template <class C>
 struct NmLds {
-  float4 tile[C::WAVES][C::TILE];
+  alignas(16) unsigned char raw[C::WAVES][C::WAVE_BYTES];
   uint32_t start[C::WAVES][C::GROUPS][NM_RANGES];
   uint32_t pref[C::WAVES][C::GROUPS][NM_RANGES + 1];
+  __device__ float4* tile(uint32_t w) { return reinterpret_cast<float4*>(raw[w]); }
 };
 
 // Per-wave state of one task.
@@ -768,7 +783,7 @@ __device__ __forceinline__ void nm_stream(const NmTask& T, NmLds<C>& L, uint32_t
   float4 nxt = first < nch ? fetch(first) : make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t b = 0;
   for (uint32_t ch = first; ch < nch; ch += step) {
-    float4* tile = &L.tile[w][b * 64u];
+    float4* tile = L.tile(w) + b * 64u;
     tile[lane] = nxt;
     if (ch + step < nch) nxt = fetch(ch + step);
     __builtin_amdgcn_wave_barrier();
@@ -859,12 +874,150 @@ __device__ __forceinline__ NmGroups nm_group_rows(const NmTask& T, NmLds<C>& L) 
   return G;
 }
 
+typedef int nm_v4i __attribute__((ext_vector_type(4)));
+
+// Folds the four 16 x 16 matrix-product accumulators of the grouped MF
+// stream into each lane's exact sums. Accumulator m (one per lane group)
+// holds, in lane l = 16 q + c, register r, the column-c total of sample row
+// 4 q + r of group m (the sample of lane 16 m + 4 q + r): columns 0-3 the
+// 7-bit digits of (x fixed point + 2^24), 4-7 those of y, 8 the count. The
+// products pass through LDS (red: 4 x 16 x 16 int32, over the stream's
+// staging area, free between chunks).
+__device__ __forceinline__ void nm_mf_fold(nm_v4i (&D)[4], int* red, long long& ax,
+                                           long long& ay, long long& n) {
+  const uint32_t lane = LANE(), q = lane >> 4, c = lane & 15u;
+#pragma unroll
+  for (uint32_t m = 0; m < 4u; ++m)
+#pragma unroll
+    for (uint32_t r = 0; r < 4u; ++r) red[(m * 16u + 4u * q + r) * 16u + c] = D[m][r];
+  __builtin_amdgcn_wave_barrier();
+  const int* row = red + lane * 16u;  // this lane's sample: group lane / 16, row lane % 16
+  const long long sx = (long long)row[0] + ((long long)row[1] << 7) + ((long long)row[2] << 14) +
+                       ((long long)row[3] << 21);
+  const long long sy = (long long)row[4] + ((long long)row[5] << 7) + ((long long)row[6] << 14) +
+                       ((long long)row[7] << 21);
+  const long long cnt = row[8];
+  ax += sx - (cnt << 24);
+  ay += sy - (cnt << 24);
+  n += cnt;
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (uint32_t m = 0; m < 4u; ++m) D[m] = nm_v4i{0, 0, 0, 0};
+}
+
+// The grouped stream with the accepted sums as matrix products (16-lane
+// groups). Per chunk, group g's 64 candidates are staged as positions and as
+// the 7-bit digits of their fixed-point coordinates plus 2^24 (B, 64 x 9:
+// x digits, y digits, a column of ones; |coordinate| <= 1, so the biased
+// value fits 25 bits); lane l = 16 q + i tests candidates 16 q .. 16 q + 15
+// of every group against the sample of row i of that group (lane 16 g + i),
+// giving the 0/1 bytes of A (16 x 64), and v_mfma_i32_16x16x64_i8 adds
+// A x B to group g's accumulator. Integer products: the sums are exact and
+// equal the VALU stream's. Accumulators fold into the lane sums at the end
+// (and every 2^16 chunks, far below int32 overflow: a chunk adds at most
+// 64 x 127 per entry).
+template <class C>
+__device__ __forceinline__ void nm_group_stream_mf(const NmTask& T, NmLds<C>& L,
+                                                   const NmGroups& G, uint32_t first,
+                                                   uint32_t step, long long& ax, long long& ay,
+                                                   long long& n, uint64_t& steps) {
+  static_assert(C::GLANES == 16, "16-lane groups");
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane >> 4, sub = lane & 15u;
+  const BtJob& J = *T.J;
+  const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2;
+  const float4* S = J.sorted;
+  const uint32_t* start = L.start[w][grp];
+  const uint32_t* pref = L.pref[w][grp];
+  const float inf = __int_as_float(0x7f800000);
+  const uint32_t nch = (G.mmax + NM_GCHUNK - 1u) / NM_GCHUNK;
+  uint32_t cur = 0;
+  auto fetch = [&](uint32_t ch, uint32_t i) -> float4 {
+    const uint32_t v = ch * NM_GCHUNK + i * 16u + sub;
+    return v < G.my_m ? nm_candidate(S, start, pref, G.my_n, v, cur)
+                      : make_float4(inf, inf, 0.f, 0.f);
+  };
+  unsigned char* bt = L.raw[w];                                     // [g][NM_MF_ROWS][64]
+  float2* xy = reinterpret_cast<float2*>(L.raw[w] + C::MF_BT);      // [g][64]
+  int* red = reinterpret_cast<int*>(L.raw[w]);
+  auto ones = [&]() {  // the count column: row 8 of each group's B
+    reinterpret_cast<uint32_t*>(bt + (grp * NM_MF_ROWS + 8u) * NM_GCHUNK)[sub] = 0x01010101u;
+  };
+  ones();
+  // the samples of this lane's rows (row `sub` of each group)
+  float qx[4], qy[4];
+#pragma unroll
+  for (uint32_t m = 0; m < 4u; ++m) {
+    qx[m] = __shfl(T.px, (int)(16u * m + sub), 64);
+    qy[m] = __shfl(T.py, (int)(16u * m + sub), 64);
+  }
+  nm_v4i D[4];
+#pragma unroll
+  for (uint32_t m = 0; m < 4u; ++m) D[m] = nm_v4i{0, 0, 0, 0};
+  float4 nxt[4];
+  if (first < nch) {
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) nxt[i] = fetch(first, i);
+  }
+  uint32_t since = 0;
+  for (uint32_t ch = first; ch < nch; ch += step) {
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+      const uint32_t k = i * 16u + sub;
+      xy[grp * NM_GCHUNK + k] = make_float2(nxt[i].x, nxt[i].y);
+      const uint32_t ux = (uint32_t)(__float_as_int(nxt[i].z) + (1 << 24));
+      const uint32_t uy = (uint32_t)(__float_as_int(nxt[i].w) + (1 << 24));
+      unsigned char* col = bt + grp * NM_MF_ROWS * NM_GCHUNK + k;
+      col[0] = (unsigned char)(ux & 127u);
+      col[NM_GCHUNK] = (unsigned char)((ux >> 7) & 127u);
+      col[2 * NM_GCHUNK] = (unsigned char)((ux >> 14) & 127u);
+      col[3 * NM_GCHUNK] = (unsigned char)(ux >> 21);
+      col[4 * NM_GCHUNK] = (unsigned char)(uy & 127u);
+      col[5 * NM_GCHUNK] = (unsigned char)((uy >> 7) & 127u);
+      col[6 * NM_GCHUNK] = (unsigned char)((uy >> 14) & 127u);
+      col[7 * NM_GCHUNK] = (unsigned char)(uy >> 21);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (ch + step < nch) {
+#pragma unroll
+      for (uint32_t i = 0; i < 4u; ++i) nxt[i] = fetch(ch + step, i);
+    }
+    steps += min((uint32_t)NM_GCHUNK, G.mmax - ch * (uint32_t)NM_GCHUNK);
+#pragma unroll
+    for (uint32_t m = 0; m < 4u; ++m) {
+      const float2* cand = xy + m * NM_GCHUNK + 16u * grp;
+      nm_v4i a = nm_v4i{0, 0, 0, 0};
+#pragma unroll
+      for (uint32_t j = 0; j < 16u; ++j) {
+        const float2 c = cand[j];
+        const float dx = (c.x - qx[m]) * tsx;
+        const float dy = (c.y - qy[m]) * tsy;
+        const bool ok = fmaf(dx, dx, dy * dy) <= r2;
+        a[j >> 2] |= ok ? (int)(1u << (8u * (j & 3u))) : 0;
+      }
+      const nm_v4i b = *reinterpret_cast<const nm_v4i*>(
+          bt + (m * NM_MF_ROWS + min(sub, NM_MF_ROWS - 1u)) * NM_GCHUNK + 16u * grp);
+      D[m] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, D[m], 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (++since == 65536u) {
+      nm_mf_fold(D, red, ax, ay, n);
+      ones();
+      since = 0;
+    }
+  }
+  nm_mf_fold(D, red, ax, ay, n);
+}
+
 // Tests the group chunks first, first + step, ... (NM_GCHUNK steps each);
 // adds the lane's accepted sums to (ax, ay, n).
 template <class C>
 __device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds<C>& L, const NmGroups& G,
                                                 uint32_t first, uint32_t step, long long& ax,
                                                 long long& ay, long long& n, uint64_t& steps) {
+  if constexpr (C::MFMA) {
+    nm_group_stream_mf<C>(T, L, G, first, step, ax, ay, n, steps);
+    return;
+  }
   const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / C::GLANES;
   const BtJob& J = *T.J;
   const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2, px = T.px, py = T.py;
@@ -880,7 +1033,7 @@ __device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds<C>& L, co
     return v < G.my_m ? nm_candidate(S, start, pref, G.my_n, v, cur)
                       : make_float4(inf, inf, 0.f, 0.f);
   };
-  float4* tile = &L.tile[w][grp * NM_GSTRIDE];
+  float4* tile = L.tile(w) + grp * NM_GSTRIDE;
   float4 nxt[NM_GCHUNK / C::GLANES];
   if (first < nch) {
 #pragma unroll
@@ -1011,8 +1164,9 @@ __device__ __forceinline__ void nm_store_work(unsigned long long* steps_out,
       b += red[w][1];
     }
     steps_out[blockIdx.x] = a;
-    nbr_out[blockIdx.x] = b;
+    if (nbr_out) nbr_out[blockIdx.x] = b;
   }
+  __syncthreads();  // red is reused by a second call
 }
 
 // 4 waves per SIMD (16 per CU) whatever the workgroup size: at most 128
@@ -1022,7 +1176,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4, 4))) 
 bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split, unsigned long long samples) {
-  typedef NmCfg<GL, W> C;
+  typedef NmCfg<GL, W, GL == 16> C;
   __shared__ NmLds<C> L;
   __shared__ unsigned long long wave_cand[C::WAVES];
   const uint32_t lane = LANE(), w = threadIdx.x >> 6;
@@ -1039,7 +1193,7 @@ bt_neighbor_mean(
   }
   const uint32_t part_steps = max(heavy / 4u, BT_PART_STEPS_MIN);
   unsigned long long cand = 0;
-  uint64_t steps = 0, nbr = 0;
+  uint64_t steps = 0, gsteps = 0, nbr = 0;
   uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
   for (;;) {
     const uint32_t t = uni(claim_work(ctl->xq, total, part, tried));
@@ -1093,7 +1247,7 @@ bt_neighbor_mean(
           ax += G.sx;
           ay += G.sy;
           n += G.sn;
-          nm_group_stream<C>(T, L, G, 0u, 1u, ax, ay, n, steps);
+          nm_group_stream<C>(T, L, G, 0u, 1u, ax, ay, n, gsteps);
           break;
         }
         ux += sx;
@@ -1114,7 +1268,8 @@ bt_neighbor_mean(
     split.wg_cand[blockIdx.x] = s;
   }
   __shared__ unsigned long long work[C::WAVES][2];
-  nm_store_work<C>(split.wg_steps, split.wg_nbr, steps, nbr, work);
+  nm_store_work<C>(split.wg_steps, split.wg_nbr, steps + gsteps, nbr, work);
+  nm_store_work<C>(split.wg_gsteps, nullptr, gsteps, 0, work);
 }
 
 // Runs the parts of heavy tasks: part p tests group chunks p, p + P, p + 2P,
@@ -1129,7 +1284,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
 bt_neighbor_mean_heavy(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split, unsigned long long samples) {
-  typedef NmCfg<GL, W> C;
+  typedef NmCfg<GL, W, GL == 16> C;
   __shared__ NmLds<C> L;
   const uint32_t lane = LANE();
   const uint32_t items = min(uni(__atomic_load_n(&ctl->item_count, __ATOMIC_RELAXED)),

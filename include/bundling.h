@@ -97,6 +97,8 @@ typedef struct bt_info {
   uint64_t test_steps;  /* wave-steps: one candidate tested against a wave's 64 samples */
   uint64_t neighbors;   /* neighbors found for the movable samples, |N(p)| summed */
   uint64_t candidates;  /* candidates in the tasks' union streams (before lane groups) */
+  uint64_t group_steps; /* of test_steps, those of lane-group streams (the rest stream
+                           one candidate to all 64 lanes) */
 } bt_info;
 
 typedef struct bt_ctx bt_ctx;
