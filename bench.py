@@ -29,15 +29,14 @@ HBM_PEAK_GBS = 8000.0
 # = 256 x 4 x 16 lanes x 2 (packed FP32) x 2 (FMA) x 2.4e9)
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 # VALU instructions per wave-step of the neighbor test loops (gfx950
-# assembly). Whole-wave and 32-lane-group streams: 47 per 4 candidates (2
-# sub, 3 mul, 1 fma, 1 compare per test, 3 masked adds folded into
-# add3/addc, one pointer increment per 4). 16-lane-group streams, whose sums
-# are int8 matrix products: 547 per 64 (the same 7 per test, then a select
-# and an OR per test building the 0/1 bytes of A; the 4 MFMAs per chunk run
-# on the matrix units).
+# assembly). Whole-wave streams: 47 per 4 candidates (2 sub, 3 mul, 1 fma,
+# 1 compare per test, 3 masked adds folded into add3/addc, one pointer
+# increment per 4). Lane-group streams, whose sums are int8 matrix
+# products: 547 per 64 (the same 7 per test, then a select and an OR per
+# test building the 0/1 bytes of A; the 4 MFMAs per chunk run on the
+# matrix units).
 VALU_PER_STEP = 47 / 4
 VALU_PER_STEP_MF = 547 / 64
-NM_SHAPE_LANES = (16, 32, 16, 32, 16, 32)  # lanes per group of neighbor-mean shape 0..5
 ITERS = 30
 TUNE_CANDIDATES = (1, 2, 3, 4, 6, 8, 12, 16, 20, 24, 28)
 COPY_PROBE_BYTES = 256 << 20
@@ -588,8 +587,7 @@ def main():
         p = gpu.job_params(i)
         info = gpu.info()
         steps_total += info.test_steps
-        mf = NM_SHAPE_LANES[gpu.shapes(i)[0]] == 16
-        valu = (info.group_steps * (VALU_PER_STEP_MF if mf else VALU_PER_STEP)
+        valu = (info.group_steps * VALU_PER_STEP_MF
                 + (info.test_steps - info.group_steps) * VALU_PER_STEP)
         valu_total += valu
         nm_ms = st["neighbor_mean"]
@@ -620,14 +618,14 @@ def main():
     executed = {"kernel": "bt_neighbor_mean+bt_neighbor_mean_heavy", "bound": "valu",
                 "wave_steps_per_schedule": steps_total,
                 "lane_tests_per_schedule": 64 * steps_total,
-                "valu_per_step": {"16-lane groups (int8 MFMA sums)": VALU_PER_STEP_MF,
-                                  "whole wave and 32-lane groups": VALU_PER_STEP},
+                "valu_per_step": {"lane-group streams (int8 MFMA sums)": VALU_PER_STEP_MF,
+                                  "whole-wave streams": VALU_PER_STEP},
                 "valu_per_schedule": valu_total,
                 "achieved": valu_total / (stage_ms["neighbor_mean"] * 1e-3),
                 "peak": VALU_ISSUE_PEAK, "unit": "wave-instructions/s",
                 "note": "test-loop VALU instructions only (staging, set-up, range walks and "
                         "the density splat are not counted); the int8 MFMAs of the "
-                        "16-lane-group streams run on the matrix units"}
+                        "lane-group streams run on the matrix units"}
     executed["frac"] = executed["achieved"] / executed["peak"]
     iter_bytes = 24 * n + 28 * P
     path = {"bytes_per_iteration": iter_bytes,

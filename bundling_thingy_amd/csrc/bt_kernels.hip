@@ -569,9 +569,11 @@ __device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total
 
 // Kernel shape, a template parameter the autotuner chooses per iteration:
 // lanes per group of the grouped stream (16: 4 groups, 32: 2 groups), waves
-// per workgroup, and whether the grouped stream adds its accepted
-// candidates with matrix instructions (MF, 16-lane groups only). Results do
-// not depend on it (exact sums).
+// per workgroup, and whether the grouped streams add their accepted
+// candidates with matrix instructions (MF; the kernels use it for every
+// shape: a whole-wave stream as matrix products was tried and, with the
+// main kernel's registers spilled further, was slower). Results do not
+// depend on it (exact sums).
 template <int GL, int W, bool MF = false>
 struct NmCfg {
   static constexpr uint32_t GLANES = GL;
@@ -582,12 +584,11 @@ struct NmCfg {
   // MF stream, per wave: the digit planes of B (GROUPS x NM_MF_ROWS x 64
   // bytes), the candidates' positions (GROUPS x 64 float2), and (over both,
   // between chunks) the 4 x 16 x 16 int32 results of the matrix products
-  static constexpr uint32_t MF_BT = GROUPS * NM_MF_ROWS * NM_GCHUNK;
-  static constexpr uint32_t MF_BYTES = MF_BT + GROUPS * NM_GCHUNK * 8u;
+  static constexpr uint32_t MF_BYTES = GROUPS * (NM_MF_ROWS * NM_GCHUNK + NM_GCHUNK * 8u);
   static constexpr uint32_t WAVE_BYTES_0 = TILE * 16u;
   static constexpr uint32_t WAVE_BYTES_1 = MF ? (MF_BYTES > 4096u ? MF_BYTES : 4096u) : 0u;
   static constexpr uint32_t WAVE_BYTES = WAVE_BYTES_0 > WAVE_BYTES_1 ? WAVE_BYTES_0 : WAVE_BYTES_1;
-  static_assert(!MF || GL == 16, "matrix-instruction sums need 16-lane groups");
+
   // record of a heavy task's set-up (BT_REC_WORDS words): per lane group its
   // ranges (start[128], pref[129]), then stream lengths m[GROUPS], range
   // counts n[GROUPS], then per group the inside sums x, y, count (int64)
@@ -905,68 +906,69 @@ __device__ __forceinline__ void nm_mf_fold(nm_v4i (&D)[4], int* red, long long& 
   for (uint32_t m = 0; m < 4u; ++m) D[m] = nm_v4i{0, 0, 0, 0};
 }
 
-// The grouped stream with the accepted sums as matrix products (16-lane
-// groups). Per chunk, group g's 64 candidates are staged as positions and as
-// the 7-bit digits of their fixed-point coordinates plus 2^24 (B, 64 x 9:
-// x digits, y digits, a column of ones; |coordinate| <= 1, so the biased
-// value fits 25 bits); lane l = 16 q + i tests candidates 16 q .. 16 q + 15
-// of every group against the sample of row i of that group (lane 16 g + i),
-// giving the 0/1 bytes of A (16 x 64), and v_mfma_i32_16x16x64_i8 adds
-// A x B to group g's accumulator. Integer products: the sums are exact and
-// equal the VALU stream's. Accumulators fold into the lane sums at the end
-// (and every 2^16 chunks, far below int32 overflow: a chunk adds at most
-// 64 x 127 per entry).
-template <class C>
-__device__ __forceinline__ void nm_group_stream_mf(const NmTask& T, NmLds<C>& L,
-                                                   const NmGroups& G, uint32_t first,
-                                                   uint32_t step, long long& ax, long long& ay,
-                                                   long long& n, uint64_t& steps) {
-  static_assert(C::GLANES == 16, "16-lane groups");
-  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane >> 4, sub = lane & 15u;
+// A candidate stream with the accepted sums as matrix products. The wave's
+// 64 samples form 64 / GL lane groups of GL consecutive lanes, each group
+// with its own stream (GL = 64: one stream for the whole wave). Per chunk,
+// each group's next 64 candidates are staged as positions and as the 7-bit
+// digits of their fixed-point coordinates plus 2^24 (B, 64 x 9: x digits, y
+// digits, a column of ones; |coordinate| <= 1, so the biased value fits 25
+// bits). Matrix product m (m = 0..3) covers the samples of lanes 16 m ..
+// 16 m + 15, whose candidates are those of their group; lane l = 16 q + i
+// tests candidates 16 q .. 16 q + 15 of that group against the sample of
+// lane 16 m + i, giving the 0/1 bytes of A (16 x 64), and
+// v_mfma_i32_16x16x64_i8 adds A x B to accumulator m. Integer products: the
+// sums are exact and equal the VALU stream's. Accumulators fold into the
+// lane sums at the end (and every 2^16 chunks, far below int32 overflow: a
+// chunk adds at most 64 x 127 per entry). Exhausted group streams stage
+// candidates at infinity, which fail the test.
+template <int GL, class C, class Fetch>
+__device__ __forceinline__ void nm_mf_stream(const NmTask& T, NmLds<C>& L, uint32_t nch,
+                                             uint32_t mmax, uint32_t first, uint32_t step,
+                                             Fetch fetch, long long& ax, long long& ay,
+                                             long long& n, uint64_t& steps) {
+  static_assert(GL == 16 || GL == 32 || GL == 64, "lane groups of 16, 32 or 64");
+  static_assert(C::WAVE_BYTES >= (64u / GL) * (NM_MF_ROWS * NM_GCHUNK + NM_GCHUNK * 8u) &&
+                    C::WAVE_BYTES >= 4096u, "matrix-product stream layout");
+  constexpr uint32_t G = 64u / GL;        // groups (streams)
+  constexpr uint32_t PER = NM_GCHUNK / GL;  // candidates each lane stages per chunk
+  constexpr uint32_t RB = GL / 16u;       // products per group
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / GL, sub = lane % GL;
+  const uint32_t q = lane >> 4, i16 = lane & 15u;
   const BtJob& J = *T.J;
   const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2;
-  const float4* S = J.sorted;
-  const uint32_t* start = L.start[w][grp];
-  const uint32_t* pref = L.pref[w][grp];
-  const float inf = __int_as_float(0x7f800000);
-  const uint32_t nch = (G.mmax + NM_GCHUNK - 1u) / NM_GCHUNK;
-  uint32_t cur = 0;
-  auto fetch = [&](uint32_t ch, uint32_t i) -> float4 {
-    const uint32_t v = ch * NM_GCHUNK + i * 16u + sub;
-    return v < G.my_m ? nm_candidate(S, start, pref, G.my_n, v, cur)
-                      : make_float4(inf, inf, 0.f, 0.f);
-  };
-  unsigned char* bt = L.raw[w];                                     // [g][NM_MF_ROWS][64]
-  float2* xy = reinterpret_cast<float2*>(L.raw[w] + C::MF_BT);      // [g][64]
+  unsigned char* bt = L.raw[w];                                             // [G][9][64]
+  float2* xy = reinterpret_cast<float2*>(L.raw[w] + G * NM_MF_ROWS * NM_GCHUNK);  // [G][64]
   int* red = reinterpret_cast<int*>(L.raw[w]);
   auto ones = [&]() {  // the count column: row 8 of each group's B
-    reinterpret_cast<uint32_t*>(bt + (grp * NM_MF_ROWS + 8u) * NM_GCHUNK)[sub] = 0x01010101u;
+    if (lane < 16u * G)
+      reinterpret_cast<uint32_t*>(bt + ((lane >> 4) * NM_MF_ROWS + 8u) * NM_GCHUNK)[i16] =
+          0x01010101u;
   };
   ones();
-  // the samples of this lane's rows (row `sub` of each group)
+  // the samples of this lane's rows (row i16 of each product)
   float qx[4], qy[4];
 #pragma unroll
   for (uint32_t m = 0; m < 4u; ++m) {
-    qx[m] = __shfl(T.px, (int)(16u * m + sub), 64);
-    qy[m] = __shfl(T.py, (int)(16u * m + sub), 64);
+    qx[m] = __shfl(T.px, (int)(16u * m + i16), 64);
+    qy[m] = __shfl(T.py, (int)(16u * m + i16), 64);
   }
   nm_v4i D[4];
 #pragma unroll
   for (uint32_t m = 0; m < 4u; ++m) D[m] = nm_v4i{0, 0, 0, 0};
-  float4 nxt[4];
+  float4 nxt[PER];
   if (first < nch) {
 #pragma unroll
-    for (uint32_t i = 0; i < 4u; ++i) nxt[i] = fetch(first, i);
+    for (uint32_t k = 0; k < PER; ++k) nxt[k] = fetch(first, k);
   }
   uint32_t since = 0;
   for (uint32_t ch = first; ch < nch; ch += step) {
 #pragma unroll
-    for (uint32_t i = 0; i < 4u; ++i) {
-      const uint32_t k = i * 16u + sub;
-      xy[grp * NM_GCHUNK + k] = make_float2(nxt[i].x, nxt[i].y);
-      const uint32_t ux = (uint32_t)(__float_as_int(nxt[i].z) + (1 << 24));
-      const uint32_t uy = (uint32_t)(__float_as_int(nxt[i].w) + (1 << 24));
-      unsigned char* col = bt + grp * NM_MF_ROWS * NM_GCHUNK + k;
+    for (uint32_t k = 0; k < PER; ++k) {
+      const uint32_t c = k * GL + sub;
+      xy[grp * NM_GCHUNK + c] = make_float2(nxt[k].x, nxt[k].y);
+      const uint32_t ux = (uint32_t)(__float_as_int(nxt[k].z) + (1 << 24));
+      const uint32_t uy = (uint32_t)(__float_as_int(nxt[k].w) + (1 << 24));
+      unsigned char* col = bt + grp * NM_MF_ROWS * NM_GCHUNK + c;
       col[0] = (unsigned char)(ux & 127u);
       col[NM_GCHUNK] = (unsigned char)((ux >> 7) & 127u);
       col[2 * NM_GCHUNK] = (unsigned char)((ux >> 14) & 127u);
@@ -979,12 +981,13 @@ __device__ __forceinline__ void nm_group_stream_mf(const NmTask& T, NmLds<C>& L,
     __builtin_amdgcn_wave_barrier();
     if (ch + step < nch) {
 #pragma unroll
-      for (uint32_t i = 0; i < 4u; ++i) nxt[i] = fetch(ch + step, i);
+      for (uint32_t k = 0; k < PER; ++k) nxt[k] = fetch(ch + step, k);
     }
-    steps += min((uint32_t)NM_GCHUNK, G.mmax - ch * (uint32_t)NM_GCHUNK);
+    steps += min((uint32_t)NM_GCHUNK, mmax - ch * (uint32_t)NM_GCHUNK);
 #pragma unroll
     for (uint32_t m = 0; m < 4u; ++m) {
-      const float2* cand = xy + m * NM_GCHUNK + 16u * grp;
+      const uint32_t g = m / RB;
+      const float2* cand = xy + g * NM_GCHUNK + 16u * q;
       nm_v4i a = nm_v4i{0, 0, 0, 0};
 #pragma unroll
       for (uint32_t j = 0; j < 16u; ++j) {
@@ -995,7 +998,7 @@ __device__ __forceinline__ void nm_group_stream_mf(const NmTask& T, NmLds<C>& L,
         a[j >> 2] |= ok ? (int)(1u << (8u * (j & 3u))) : 0;
       }
       const nm_v4i b = *reinterpret_cast<const nm_v4i*>(
-          bt + (m * NM_MF_ROWS + min(sub, NM_MF_ROWS - 1u)) * NM_GCHUNK + 16u * grp);
+          bt + (g * NM_MF_ROWS + min(i16, NM_MF_ROWS - 1u)) * NM_GCHUNK + 16u * q);
       D[m] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, D[m], 0, 0, 0);
     }
     __builtin_amdgcn_wave_barrier();
@@ -1006,6 +1009,28 @@ __device__ __forceinline__ void nm_group_stream_mf(const NmTask& T, NmLds<C>& L,
     }
   }
   nm_mf_fold(D, red, ax, ay, n);
+}
+
+// The grouped stream (C::GLANES-lane groups) as matrix products.
+template <class C>
+__device__ __forceinline__ void nm_group_stream_mf(const NmTask& T, NmLds<C>& L,
+                                                   const NmGroups& G, uint32_t first,
+                                                   uint32_t step, long long& ax, long long& ay,
+                                                   long long& n, uint64_t& steps) {
+  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / C::GLANES;
+  const uint32_t sub = lane % C::GLANES;
+  const float4* S = T.J->sorted;
+  const uint32_t* start = L.start[w][grp];
+  const uint32_t* pref = L.pref[w][grp];
+  const float inf = __int_as_float(0x7f800000);
+  uint32_t cur = 0;
+  auto fetch = [&](uint32_t ch, uint32_t k) -> float4 {
+    const uint32_t v = ch * NM_GCHUNK + k * C::GLANES + sub;
+    return v < G.my_m ? nm_candidate(S, start, pref, G.my_n, v, cur)
+                      : make_float4(inf, inf, 0.f, 0.f);
+  };
+  const uint32_t nch = (G.mmax + NM_GCHUNK - 1u) / NM_GCHUNK;
+  nm_mf_stream<C::GLANES>(T, L, nch, G.mmax, first, step, fetch, ax, ay, n, steps);
 }
 
 // Tests the group chunks first, first + step, ... (NM_GCHUNK steps each);
@@ -1176,7 +1201,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4, 4))) 
 bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split, unsigned long long samples) {
-  typedef NmCfg<GL, W, GL == 16> C;
+  typedef NmCfg<GL, W, true> C;
   __shared__ NmLds<C> L;
   __shared__ unsigned long long wave_cand[C::WAVES];
   const uint32_t lane = LANE(), w = threadIdx.x >> 6;
@@ -1284,7 +1309,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
 bt_neighbor_mean_heavy(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split, unsigned long long samples) {
-  typedef NmCfg<GL, W, GL == 16> C;
+  typedef NmCfg<GL, W, true> C;
   __shared__ NmLds<C> L;
   const uint32_t lane = LANE();
   const uint32_t items = min(uni(__atomic_load_n(&ctl->item_count, __ATOMIC_RELAXED)),
