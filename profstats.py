@@ -25,6 +25,11 @@ arguments below match the base name, every template instance of it.
   python profstats.py traffic_series <fetch.csv> <write.csv> <kernels> <n> <copy_bytes>
       the same counters per dispatch (MB fetched and written per kernel, the
       last n dispatches in order), as a table
+  python profstats.py valu <counters.csv> <main> <heavy> <n>
+      per dispatch (the last n of each of two kernels, in order): duration
+      under counter collection (the same rows' timestamps), SQ_INSTS_VALU and
+      VALU issue utilization = instructions x 4 cycles / (duration x 2.4 GHz
+      x 1,024 SIMDs), with the main kernel's template arguments
   python profstats.py traffic <fetch.csv> <write.csv> <kernels> <n> <copy_bytes>
       HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
       each scaled by what the counter reads for HIP's device copy of
@@ -209,6 +214,39 @@ def traffic_series(fetch_csv, write_csv, kernels, n, copy_bytes):
     return "\n".join(lines)
 
 
+def valu(counter_csv, main, heavy, n):
+    rows = _counter(counter_csv, "SQ_INSTS_VALU")
+    cols = {}
+    for k in (main, heavy):
+        mine = [r for r in rows if _base(r["Kernel_Name"]) == k][-n:]
+        inst = [float(r["Counter_Value"]) for r in mine]
+        dur = [(_short(r["Kernel_Name"]),
+                float(r["End_Timestamp"]) - float(r["Start_Timestamp"])) for r in mine]
+        cols[k] = (inst, dur)
+    peak = 2.4e9 * 1024 / 4.0  # wave64 VALU instructions per second
+    lines = ["| iteration | main shape | main us | main VALU instr | main util | heavy us | "
+             "heavy VALU instr | heavy util |", "|---|---|---|---|---|---|---|---|"]
+    tot = {main: [0.0, 0.0], heavy: [0.0, 0.0]}
+    for i in range(n):
+        cells = []
+        for k in (main, heavy):
+            inst, dur = cols[k]
+            name, t = dur[i]
+            if k == main:
+                cells.append("<" + name.split("<", 1)[1] if "<" in name else "")
+            cells += [f"{t / 1e3:.0f}", f"{inst[i]:.3e}", f"{inst[i] / (t * 1e-9) / peak:.2f}"]
+            tot[k][0] += t
+            tot[k][1] += inst[i]
+        lines.append(f"| {i} | " + " | ".join(cells) + " |")
+    lines.append(f"| all | | {tot[main][0] / 1e3:.0f} | {tot[main][1]:.3e} | "
+                 f"{tot[main][1] / (tot[main][0] * 1e-9) / peak:.2f} | {tot[heavy][0] / 1e3:.0f} | "
+                 f"{tot[heavy][1]:.3e} | {tot[heavy][1] / (tot[heavy][0] * 1e-9) / peak:.2f} |")
+    t_all = tot[main][0] + tot[heavy][0]
+    lines.append(f"stage: {t_all / 1e6:.2f} ms for {n} iterations, VALU issue "
+                 f"{(tot[main][1] + tot[heavy][1]) / (t_all * 1e-9) / peak:.2f} of peak")
+    return "\n".join(lines)
+
+
 def _counter(path, name):
     with open(path) as f:
         return [r for r in csv.DictReader(f) if r["Counter_Name"] == name]
@@ -250,6 +288,8 @@ if __name__ == "__main__":
     elif cmd == "traffic_series":
         print(traffic_series(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]),
                              int(sys.argv[6])))
+    elif cmd == "valu":
+        print(valu(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5])))
     elif cmd == "traffic":
         print(json.dumps(traffic(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]),
                                  int(sys.argv[6])), indent=1))
