@@ -544,15 +544,19 @@ __device__ __forceinline__ int cell_of_fast(float t, float inv_cell, uint32_t g)
 // lines; each only grows, so every wave ends after at most one failed claim
 // per part.
 __device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total, uint32_t& part,
-                                               uint32_t& tried) {
+                                               uint32_t& tried, uint32_t batch = 1u,
+                                               uint32_t* end = nullptr) {
   while (tried < BT_XCDS) {
     const uint32_t lo = (uint32_t)(((uint64_t)total * part) / BT_XCDS);
     const uint32_t hi = (uint32_t)(((uint64_t)total * (part + 1u)) / BT_XCDS);
     if (lo < hi) {
       uint32_t k = 0;
-      if (LANE() == 0) k = atomicAdd(&cursors[part * BT_CTR_STRIDE], 1u);
+      if (LANE() == 0) k = atomicAdd(&cursors[part * BT_CTR_STRIDE], batch);
       k = __builtin_amdgcn_readfirstlane(k);
-      if (lo + k < hi) return lo + k;
+      if (lo + k < hi) {
+        if (end) *end = min(lo + k + batch, hi);
+        return lo + k;
+      }
     }
     part = (part + 1u) % BT_XCDS;
     ++tried;
@@ -566,6 +570,7 @@ __device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total
 #define NM_GSTRIDE 65       // float4 per lane group in a tile (one of padding: no bank conflicts)
 #define NM_GROUP_MIN 2048u  // candidates above which a task streams per lane group
 #define NM_MF_ROWS 9        // B columns of the matrix-instruction sums: 4 + 4 digits, count
+#define NM_BATCH_CAND 256.0 // expected candidates per task below which tasks are claimed 4 at a time
 
 // Kernel shape, a template parameter the autotuner chooses per iteration:
 // lanes per group of the grouped stream (16: 4 groups, 32: 2 groups), waves
@@ -1217,12 +1222,24 @@ bt_neighbor_mean(
                           (unsigned long long)BT_HEAVY_HI);
   }
   const uint32_t part_steps = max(heavy / 4u, BT_PART_STEPS_MIN);
+  // Tasks claimed per work-queue atomic: launches of short tasks (the
+  // small-radius iterations, where each task is a chain of dependent loads
+  // and the claims on the 8 cursors are a visible part of it) take 4 per
+  // claim; longer tasks one, for the balance at the end of the launch.
+  uint32_t batch = 1u;
+  if (prev && total) {
+    const double expect = sprev ? (double)prev * ((double)samples / (double)sprev) : (double)prev;
+    batch = expect / (double)total < NM_BATCH_CAND ? 4u : 1u;
+  }
   unsigned long long cand = 0;
   uint64_t steps = 0, gsteps = 0, nbr = 0;
   uint32_t part = blockIdx.x % BT_XCDS, tried = 0;
   for (;;) {
-    const uint32_t t = uni(claim_work(ctl->xq, total, part, tried));
-    if (t >= total) break;
+    uint32_t tend = 0;
+    const uint32_t t0 = uni(claim_work(ctl->xq, total, part, tried, batch, &tend));
+    if (t0 >= total) break;
+    tend = uni(tend);
+    for (uint32_t t = t0; t < tend; ++t) {
     NmTask T;
     nm_load(T, jobs, tasks, t);
     long long ax = 0, ay = 0, n = 0, ux = 0, uy = 0, un = 0;
@@ -1283,6 +1300,7 @@ bt_neighbor_mean(
       if (deferred) continue;
     }
     nbr += nm_finish(T, ax + ux, ay + uy, n + un);
+    }
   }
   // this workgroup's candidates (every wave leaves the loop above)
   if (lane == 0) wave_cand[threadIdx.x >> 6] = cand;
