@@ -1221,7 +1221,7 @@ bt_neighbor_mean(
     heavy = (uint32_t)min(max(per_wave / 2ull, (unsigned long long)BT_HEAVY_LO),
                           (unsigned long long)BT_HEAVY_HI);
   }
-  const uint32_t part_steps = max(heavy / 4u, BT_PART_STEPS_MIN);
+  const uint32_t part_steps = max(heavy / 2u, BT_PART_STEPS_MIN);
   // Tasks claimed per work-queue atomic: launches of short tasks (the
   // small-radius iterations, where each task is a chain of dependent loads
   // and the claims on the 8 cursors are a visible part of it) take 4 per
