@@ -246,7 +246,7 @@ def render_section(bt, gpu, opts, reps=5):
 # (name, policy, tick_samples, lane_samples, small_cu_mask, large_cu_mask).
 # FIFO for throughput; shortest-first with 1M sample-updates per tick for
 # latency (shorter ticks let small jobs through sooner, at some throughput);
-# two lanes (jobs of at most 256k samples tick on their own stream, larger
+# two lanes (jobs of at most 192k samples tick on their own stream, larger
 # ones on another, overlapping on the GPU), each lane's tick taking the least
 # remaining work first up to 1M sample-updates while no job waits for
 # admission: on any CU, or with the small lane on shader engine 3 of every
@@ -254,8 +254,8 @@ def render_section(bt, gpu, opts, reps=5):
 # lane on every CU while the other has none).
 SE3, SE012 = 0xFF000000, 0x00FFFFFF
 SERVER_POLICIES = (("fifo", "FIFO", 0, 0, 0, 0), ("shortest_1M", "SHORTEST", 1 << 20, 0, 0, 0),
-                   ("lanes_256k_1M", "LANES", 1 << 20, 1 << 18, 0, 0),
-                   ("lanes_256k_1M_se", "LANES", 1 << 20, 1 << 18, SE3, SE012))
+                   ("lanes_192k_1M", "LANES", 1 << 20, 3 << 16, 0, 0),
+                   ("lanes_192k_1M_se", "LANES", 1 << 20, 3 << 16, SE3, SE012))
 # Each server is warmed before its measured run with jobs of another seed (a
 # long-running server has its allocator slabs and pinned result buffers in
 # place); finished jobs are read into two reused host buffers.

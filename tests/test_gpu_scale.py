@@ -218,11 +218,11 @@ SE3, SE012 = 0xFF000000, 0x00FFFFFF  # shader engine 3 / 0-2 of every XCD
 @pytest.mark.parametrize("policy,tick,lane,flags,masks", [
     (0, 0, 0, 0, (0, 0)), (1, 1 << 20, 0, 0, (0, 0)), (0, 0, 0, 4, (0, 0)),
     (1, 1 << 20, 0, 4, (0, 0)), (2, 0, 1 << 18, 0, (0, 0)), (2, 1 << 20, 1 << 18, 4, (0, 0)),
-    (2, 1 << 20, 1 << 18, 4, (SE3, SE012)), (2, 0, 1 << 18, 0, (SE012, SE3))])
+    (2, 1 << 20, 3 << 16, 4, (SE3, SE012)), (2, 0, 1 << 18, 0, (SE012, SE3))])
 def test_served_stream_equals_alone(bt, served_case, policy, tick, lane, flags, masks):
     """Every job of the 100-job served stream, co-scheduled with the others
     under continuous batching (FIFO, shortest-first with 1M sample-updates
-    per tick, and two lanes split at 256k samples, without and with a 1M
+    per tick, and two lanes split at 256k or 192k samples, without and with a 1M
     per-lane tick budget, on any CU or with the lanes on disjoint shader
     engines while both have work), ends with the bits of the job alone,
     whether the jobs compute their density every iteration or only at the
