@@ -32,11 +32,12 @@ VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 # assembly). Whole-wave streams: 47 per 4 candidates (2 sub, 3 mul, 1 fma,
 # 1 compare per test, 3 masked adds folded into add3/addc, one pointer
 # increment per 4). Lane-group streams, whose sums are int8 matrix
-# products: 547 per 64 (the same 7 per test, then a select and an OR per
-# test building the 0/1 bytes of A; the 4 MFMAs per chunk run on the
+# products: 499 per 64 (the same 6 arithmetic instructions per test and an
+# integer subtraction whose sign is the test's outcome; per 4 tests two
+# v_perm_b32 and an OR pack the A bytes; the 4 MFMAs per chunk run on the
 # matrix units).
 VALU_PER_STEP = 47 / 4
-VALU_PER_STEP_MF = 547 / 64
+VALU_PER_STEP_MF = 499 / 64
 ITERS = 30
 TUNE_CANDIDATES = (1, 2, 3, 4, 6, 8, 12, 16, 20, 24, 28)
 COPY_PROBE_BYTES = 256 << 20

@@ -884,11 +884,11 @@ typedef int nm_v4i __attribute__((ext_vector_type(4)));
 
 // Folds the four 16 x 16 matrix-product accumulators of the grouped MF
 // stream into each lane's exact sums. Accumulator m (one per lane group)
-// holds, in lane l = 16 q + c, register r, the column-c total of sample row
-// 4 q + r of group m (the sample of lane 16 m + 4 q + r): columns 0-3 the
-// 7-bit digits of (x fixed point + 2^24), 4-7 those of y, 8 the count. The
-// products pass through LDS (red: 4 x 16 x 16 int32, over the stream's
-// staging area, free between chunks).
+// holds, in lane l = 16 q + c, register r, minus the column-c total of
+// sample row 4 q + r of group m (the sample of lane 16 m + 4 q + r): columns
+// 0-3 the 7-bit digits of (x fixed point + 2^24), 4-7 those of y, 8 the
+// count. The products pass through LDS (red: 4 x 16 x 16 int32, over the
+// stream's staging area, free between chunks).
 __device__ __forceinline__ void nm_mf_fold(nm_v4i (&D)[4], int* red, long long& ax,
                                            long long& ay, long long& n) {
   const uint32_t lane = LANE(), q = lane >> 4, c = lane & 15u;
@@ -897,15 +897,16 @@ __device__ __forceinline__ void nm_mf_fold(nm_v4i (&D)[4], int* red, long long& 
 #pragma unroll
     for (uint32_t r = 0; r < 4u; ++r) red[(m * 16u + 4u * q + r) * 16u + c] = D[m][r];
   __builtin_amdgcn_wave_barrier();
+  // A holds -1 for an accepted candidate: every total is the negated sum
   const int* row = red + lane * 16u;  // this lane's sample: group lane / 16, row lane % 16
-  const long long sx = (long long)row[0] + ((long long)row[1] << 7) + ((long long)row[2] << 14) +
-                       ((long long)row[3] << 21);
-  const long long sy = (long long)row[4] + ((long long)row[5] << 7) + ((long long)row[6] << 14) +
-                       ((long long)row[7] << 21);
-  const long long cnt = row[8];
-  ax += sx - (cnt << 24);
-  ay += sy - (cnt << 24);
-  n += cnt;
+  const long long sx = (long long)row[0] + (long long)row[1] * 128ll +
+                       (long long)row[2] * 16384ll + (long long)row[3] * 2097152ll;
+  const long long sy = (long long)row[4] + (long long)row[5] * 128ll +
+                       (long long)row[6] * 16384ll + (long long)row[7] * 2097152ll;
+  const long long cnt = row[8];  // minus the count
+  ax += cnt * 16777216ll - sx;
+  ay += cnt * 16777216ll - sy;
+  n -= cnt;
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (uint32_t m = 0; m < 4u; ++m) D[m] = nm_v4i{0, 0, 0, 0};
@@ -920,12 +921,20 @@ __device__ __forceinline__ void nm_mf_fold(nm_v4i (&D)[4], int* red, long long& 
 // bits). Matrix product m (m = 0..3) covers the samples of lanes 16 m ..
 // 16 m + 15, whose candidates are those of their group; lane l = 16 q + i
 // tests candidates 16 q .. 16 q + 15 of that group against the sample of
-// lane 16 m + i, giving the 0/1 bytes of A (16 x 64), and
+// lane 16 m + i, giving the bytes of A (16 x 64: -1 accepted, 0 not), and
 // v_mfma_i32_16x16x64_i8 adds A x B to accumulator m. Integer products: the
 // sums are exact and equal the VALU stream's. Accumulators fold into the
 // lane sums at the end (and every 2^16 chunks, far below int32 overflow: a
 // chunk adds at most 64 x 127 per entry). Exhausted group streams stage
 // candidates at infinity, which fail the test.
+//
+// The test is the reference's fmaf(dx, dx, dy * dy) <= r2, bit for bit.
+// d2 = fmaf(...) is +0, positive or +inf, and non-negative floats order as
+// their bit patterns, so d2 <= r2 exactly when bits(d2) - (bits(r2) + 1) <
+// 0 as int32. v_perm_b32's sign-replicating selectors (9: bit 31 of the low
+// source, 11: bit 31 of the high one) turn two such differences into two A
+// bytes of 0xFF or 0: a subtraction per test and three instructions per
+// four tests, where a compare, a select and half an OR took 2.5.
 template <int GL, class C, class Fetch>
 __device__ __forceinline__ void nm_mf_stream(const NmTask& T, NmLds<C>& L, uint32_t nch,
                                              uint32_t mmax, uint32_t first, uint32_t step,
@@ -941,6 +950,8 @@ __device__ __forceinline__ void nm_mf_stream(const NmTask& T, NmLds<C>& L, uint3
   const uint32_t q = lane >> 4, i16 = lane & 15u;
   const BtJob& J = *T.J;
   const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2;
+  // bits(r2) + 1; a NaN radius accepts nothing, as the float compare does
+  const int rb = r2 == r2 ? (int)__float_as_uint(r2) + 1 : 0;
   unsigned char* bt = L.raw[w];                                             // [G][9][64]
   float2* xy = reinterpret_cast<float2*>(L.raw[w] + G * NM_MF_ROWS * NM_GCHUNK);  // [G][64]
   int* red = reinterpret_cast<int*>(L.raw[w]);
@@ -993,15 +1004,19 @@ __device__ __forceinline__ void nm_mf_stream(const NmTask& T, NmLds<C>& L, uint3
     for (uint32_t m = 0; m < 4u; ++m) {
       const uint32_t g = m / RB;
       const float2* cand = xy + g * NM_GCHUNK + 16u * q;
-      nm_v4i a = nm_v4i{0, 0, 0, 0};
+      uint32_t d[16];
 #pragma unroll
       for (uint32_t j = 0; j < 16u; ++j) {
         const float2 c = cand[j];
         const float dx = (c.x - qx[m]) * tsx;
         const float dy = (c.y - qy[m]) * tsy;
-        const bool ok = fmaf(dx, dx, dy * dy) <= r2;
-        a[j >> 2] |= ok ? (int)(1u << (8u * (j & 3u))) : 0;
+        d[j] = (uint32_t)(__float_as_int(fmaf(dx, dx, dy * dy)) - rb);
       }
+      nm_v4i a;
+#pragma unroll
+      for (uint32_t t = 0; t < 4u; ++t)
+        a[t] = (int)(__builtin_amdgcn_perm(d[4 * t + 1], d[4 * t], 0x0c0c0b09u) |
+                     __builtin_amdgcn_perm(d[4 * t + 3], d[4 * t + 2], 0x0b090c0cu));
       const nm_v4i b = *reinterpret_cast<const nm_v4i*>(
           bt + (g * NM_MF_ROWS + min(i16, NM_MF_ROWS - 1u)) * NM_GCHUNK + 16u * q);
       D[m] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, D[m], 0, 0, 0);
