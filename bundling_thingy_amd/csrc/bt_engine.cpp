@@ -233,6 +233,11 @@ Engine::~Engine() {
     }
   stream_ = nullptr;
   if (switch_ev_) (void)hipEventDestroy(switch_ev_);
+  if (side_) {
+    (void)hipStreamSynchronize(side_);
+    (void)hipStreamDestroy(side_);
+  }
+  for (hipEvent_t e : ov_ev_) (void)hipEventDestroy(e);
   for (Pinned& s : pinned_) {
     if (s.ev) (void)hipEventDestroy(s.ev);
     if (s.p) (void)hipHostFree(s.p);
@@ -431,7 +436,7 @@ void Engine::ensure_tasks(const std::vector<Job*>& js) {
 }
 
 void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool density,
-                              hipEvent_t* ev) {
+                              hipEvent_t* ev, const Overlap* ov) {
   const BtJob* J = reinterpret_cast<const BtJob*>(dev);
   const uint32_t nj = p.njobs;
   auto B = [&](int k) {
@@ -441,8 +446,13 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   auto mark = [&](int i) {
     if (ev) HIPCHK(hipEventRecord(ev[i], stream_));
   };
+  // side: this iteration's density stages go to side_ (ov->side, set by the
+  // capture for every iteration with density stages); prev: the previous
+  // iteration's did, and side_ has cleared this iteration's density counts
+  const bool side = ov && ov->side && density && p.grid[BT_K_ROW];
+  const bool prev = ov && ov->prev;
   mark(0);
-  btk::clear(stream_, p.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)nj, ctl, 1);
+  btk::clear(stream_, p.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)nj, ctl, prev ? 1 : 3);
   mark(1);
   btk::bin(stream_, p.grid[BT_K_RANK], J, B(BT_K_RANK), (int)nj);
   mark(2);
@@ -469,18 +479,34 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   split.wg_gsteps = work + 4 * BT_MAX_WG;
   const uint32_t shape = p.kshape & 0xffu;
   last_nm_grid_ = (uint32_t)cus_ * nm_per_cu_[shape];
+  if (prev) HIPCHK(hipStreamWaitEvent(stream_, ov->prev_done, 0));  // the counts are clear
   btk::neighbor_mean(stream_, last_nm_grid_, J, tasks_.as<BtTask>(), ctl, split, p.samples,
                      shape);
   mark(6);
   if (density && p.grid[BT_K_ROW]) {
+    hipStream_t ds = stream_;
+    if (side) {
+      HIPCHK(hipEventRecord(ov->nm_done, stream_));
+      HIPCHK(hipStreamWaitEvent(side_, ov->nm_done, 0));
+      ds = side_;
+    }
     const uint32_t col = (p.kshape >> 16) & 0xffu;
     if (!btk::col_fused(col))
-      btk::blur_rows(stream_, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj, (p.kshape >> 8) & 0xffu);
+      btk::blur_rows(ds, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)nj, (p.kshape >> 8) & 0xffu);
     mark(7);
-    btk::blur_cols(stream_, p.grid[BT_K_COL], J, B(BT_K_COL), (int)nj, col);
+    btk::blur_cols(ds, p.grid[BT_K_COL], J, B(BT_K_COL), (int)nj, col);
     mark(8);
-    btk::normalize(stream_, p.grid[BT_K_NORM], J, B(BT_K_NORM), (int)nj);
+    btk::normalize(ds, p.grid[BT_K_NORM], J, B(BT_K_NORM), (int)nj);
     mark(9);
+    if (side) {
+      // the next iteration's density counts and peak, from its own job table
+      if (ov->next)
+        btk::clear(side_, ov->next->grid[BT_K_CLEAR], reinterpret_cast<const BtJob*>(ov->next_dev),
+                   reinterpret_cast<const uint32_t*>(ov->next_dev + ov->next->base_off) +
+                       (size_t)BT_K_CLEAR * (ov->next->njobs + 1),
+                   (int)ov->next->njobs, ctl, 2);
+      HIPCHK(hipEventRecord(ov->side_done, side_));
+    }
   } else if (ev) {
     mark(7);
     mark(8);
@@ -713,7 +739,7 @@ void Engine::upload_jobs(const std::vector<GraphIn>& gs, std::vector<std::unique
       return reinterpret_cast<const uint32_t*>(dev + p.base_off) + (size_t)k * (n2 + 1);
     };
     btk::upload_setup(st, J, (int)n2, area.as<uint32_t>());
-    btk::clear(st, p.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)n2, ctl, 1);
+    btk::clear(st, p.grid[BT_K_CLEAR], J, B(BT_K_CLEAR), (int)n2, ctl, 3);
     btk::resample_emit(st, p.grid[BT_K_EMIT], J, B(BT_K_EMIT), (int)n2);
     if (p.grid[BT_K_ROW]) {
       btk::blur_rows(st, p.grid[BT_K_ROW], J, B(BT_K_ROW), (int)n2);
@@ -858,9 +884,32 @@ void Engine::iterate(uint32_t n, bool use_graph) {
       put_plan(plans[i], buf->as<unsigned char>() + i * slot);
     sync();
     GraphEntry g;
+    // each iteration's density stages run on side_, beside the next
+    // iteration's binning, scans and task list (which wait for nothing of
+    // them); the next neighbor mean waits for them and for side_'s clear of
+    // its density counts
+    if (!side_) HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    while (ov_ev_.size() < 2 * (size_t)n) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ov_ev_.push_back(e);
+    }
     HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
-    for (uint32_t i = 0; i < n; ++i)
-      launch_iteration(plans[i], buf->as<unsigned char>() + i * slot, density);
+    bool prev = false;
+    for (uint32_t i = 0; i < n; ++i) {
+      Overlap ov;
+      ov.side = true;
+      ov.prev = prev;
+      ov.prev_done = i ? ov_ev_[2 * i - 1] : nullptr;
+      ov.nm_done = ov_ev_[2 * i];
+      ov.side_done = ov_ev_[2 * i + 1];
+      ov.next = i + 1 < n ? &plans[i + 1] : nullptr;
+      ov.next_dev = buf->as<unsigned char>() + (i + 1) * slot;
+      launch_iteration(plans[i], buf->as<unsigned char>() + i * slot, density, nullptr, &ov);
+      prev = density && plans[i].grid[BT_K_ROW] && ov.next;
+      if (density && plans[i].grid[BT_K_ROW] && !ov.next)
+        HIPCHK(hipStreamWaitEvent(stream_, ov.side_done, 0));  // join
+    }
     HIPCHK(hipStreamEndCapture(stream_, &g.graph));
     HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
     g.last_nm_grid = last_nm_grid_;

@@ -185,8 +185,16 @@ class Engine {
   // Copies a plan to device memory through a ring of pinned host slots (the
   // copy is asynchronous; a slot is reused once its previous copy is done).
   void put_plan(const Plan& p, void* dst, hipStream_t st = nullptr);
+  // A captured schedule's iteration whose density stages run on side_.
+  struct Overlap {
+    bool side = false;        // density stages on side_
+    bool prev = false;        // the previous iteration's did (and cleared this one's counts)
+    hipEvent_t prev_done = nullptr, nm_done = nullptr, side_done = nullptr;
+    const Plan* next = nullptr;  // the next iteration (side_ clears its density counts)
+    const unsigned char* next_dev = nullptr;
+  };
   void launch_iteration(const Plan& p, const unsigned char* dev, bool density,
-                        hipEvent_t* ev = nullptr);
+                        hipEvent_t* ev = nullptr, const Overlap* ov = nullptr);
   void ensure_tasks(const std::vector<Job*>& js);
   // Forgets the previous launch's candidate count (the heavy-task threshold
   // falls back to its upper bound).
@@ -204,6 +212,8 @@ class Engine {
   int stream_cus_[2] = {0, 0};
   int active_ = 0;
   hipEvent_t switch_ev_ = nullptr;
+  hipStream_t side_ = nullptr;        // captured schedules' density stages
+  std::vector<hipEvent_t> ov_ev_;     // their fork and join events
   std::vector<std::unique_ptr<Job>> jobs_;
   DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_, retire_plans_;
   struct Pinned {

@@ -129,14 +129,15 @@ __device__ __forceinline__ long long wave_sum64(long long v) {
   const BtJob& J = jobs[j];
 
 // ------------------------------------------------------------------ clear
-// Zeroes the cell counters (n_cells + 1 words), the density counts and the
-// peak of each job, and the launch's control block.
+// mode bit 0: zeroes the cell counters (n_cells + 1 words) of each job and
+// the launch's control block; bit 1: the density counts and the peak of each
+// job (apart, the density clear runs beside the next iteration's binning).
 extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restrict__ jobs,
                                                            const uint32_t* __restrict__ base,
-                                                           int njobs, BtControl* ctl,
-                                                           int clear_density) {
+                                                           int njobs, BtControl* ctl, int mode) {
   JOB_PROLOGUE(BT_K_CLEAR);
-  const uint32_t ncell = J.n_cells + 1u, npix = J.dw * J.dh;
+  const uint32_t ncell = (mode & 1) ? J.n_cells + 1u : 0u;
+  const uint32_t npix = (mode & 2) && J.dens_cnt ? J.dw * J.dh : 0u;
   uint32_t* cells = J.cells;
   uint32_t* dens = J.dens_cnt;
   const uint32_t i0 = lb * BT_CLEAR_ITEMS + threadIdx.x;
@@ -144,11 +145,11 @@ extern "C" __global__ __launch_bounds__(256) void bt_clear(const BtJob* __restri
   for (uint32_t k = 0; k < BT_CLEAR_ITEMS / 256; ++k) {
     uint32_t i = i0 + k * 256u;
     if (i < ncell) cells[i] = 0u;
-    if (clear_density && i < npix) dens[i] = 0u;
+    if (i < npix) dens[i] = 0u;
   }
   if (lb == 0 && threadIdx.x == 0) {
-    if (J.peak) J.peak[0] = 0u;  // no peak buffer when density is off
-    if (j == 0) {
+    if ((mode & 2) && J.peak) J.peak[0] = 0u;  // no peak buffer when density is off
+    if ((mode & 1) && j == 0) {
       ctl->task_total = 0u;
       ctl->heavy_count = 0u;
       ctl->item_count = 0u;
@@ -1986,8 +1987,8 @@ extern "C" __global__ __launch_bounds__(256) void bt_endpoints(const BtJob* __re
 namespace btk {
 #define L256(grid) dim3(grid), dim3(256), 0, s
 void clear(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
-           BtControl* ctl, int clear_density) {
-  hipLaunchKernelGGL(bt_clear, L256(grid), jobs, base, nj, ctl, clear_density);
+           BtControl* ctl, int mode) {
+  hipLaunchKernelGGL(bt_clear, L256(grid), jobs, base, nj, ctl, mode);
 }
 void bin(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj) {
   hipLaunchKernelGGL(bt_bin, L256(grid), jobs, base, nj);

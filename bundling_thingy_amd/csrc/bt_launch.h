@@ -5,8 +5,10 @@
 #include "bt_device.h"
 
 namespace btk {
+// mode bit 0: cell counters and the control block; bit 1: density counts
+// and peak.
 void clear(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
-           BtControl* ctl, int clear_density);
+           BtControl* ctl, int mode);
 void bin(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void cell_scan(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj);
 void tasks(hipStream_t s, uint32_t grid, const BtJob* jobs, const uint32_t* base, int nj,
