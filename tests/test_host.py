@@ -265,3 +265,28 @@ def test_server_refuses_cu_masks_that_leave_an_xcd_empty(bt, small, large):
     with pytest.raises(api.BtError) as e:
         bt.Server(policy=bt.Server.LANES, small_cu_mask=small, large_cu_mask=large)
     assert e.value.code == api.BT_E_INVALID
+
+
+def test_sign_of_bit_difference_is_the_radius_test():
+    """The neighbor-mean matrix-product streams decide fmaf(dx, dx, dy*dy) <= r2
+    as the sign of bits(d2) - (bits(r2) + 1) in int32 (d2 is +0, positive or
+    +inf). Checked against the float compare on random, near-tie, subnormal,
+    zero and infinite squared distances and radii, ties included; shifting the
+    bias by one (a plain bits(d2) - bits(r2)) fails on the ties."""
+    rng = np.random.default_rng(5)
+    f32 = np.float32
+    r2 = np.concatenate([rng.uniform(0, 1e4, 2000), [0.0, 1e-45, 1.0, 900.0, 3.4e38]]).astype(f32)
+    with np.errstate(over="ignore"):  # 2 x 3.4e38 -> +inf, a valid squared distance
+        base = rng.uniform(0, 2, (r2.size, 64)).astype(f32) * r2[:, None]
+    ties = np.repeat(r2[:, None], 4, 1)
+    ulp_up = np.nextafter(ties, f32(np.inf))
+    ulp_dn = np.nextafter(ties, f32(0))
+    special = np.broadcast_to(np.array([0.0, 1e-45, 1e-40, np.inf], dtype=f32), (r2.size, 4))
+    d2 = np.concatenate([base, ties, ulp_up, ulp_dn, special], axis=1).astype(f32)
+    want = d2 <= r2[:, None]
+    bits = d2.view(np.int32).astype(np.int64)
+    rb = r2.view(np.int32).astype(np.int64)[:, None] + 1
+    diff = ((bits - rb + 2**31) % 2**32 - 2**31)  # int32 wrap, as the kernel computes it
+    np.testing.assert_array_equal(diff < 0, want)
+    wrong = ((bits - (rb - 1) + 2**31) % 2**32 - 2**31) < 0
+    assert np.any(wrong != want)
