@@ -568,32 +568,29 @@ __device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total
 #define NM_ROWS 64          // window rows set up per pass (one per lane)
 #define NM_RANGES (2 * NM_ROWS)  // ranges per row pass (two per row)
 #define NM_GCHUNK 64        // candidates per group per chunk of the grouped stream
-#define NM_GSTRIDE 65       // float4 per lane group in a tile (one of padding: no bank conflicts)
 #define NM_GROUP_MIN 2048u  // candidates above which a task streams per lane group
 #define NM_MF_ROWS 9        // B columns of the matrix-instruction sums: 4 + 4 digits, count
 #define NM_BATCH_CAND 256.0 // expected candidates per task below which tasks are claimed 4 at a time
 
 // Kernel shape, a template parameter the autotuner chooses per iteration:
-// lanes per group of the grouped stream (16: 4 groups, 32: 2 groups), waves
-// per workgroup, and whether the grouped streams add their accepted
-// candidates with matrix instructions (MF; the kernels use it for every
-// shape: a whole-wave stream as matrix products was tried and, with the
-// main kernel's registers spilled further, was slower). Results do not
-// depend on it (exact sums).
-template <int GL, int W, bool MF = false>
+// lanes per group of the grouped stream (16: 4 groups, 32: 2 groups) and
+// waves per workgroup. The grouped streams add their accepted candidates
+// with matrix instructions; whole-wave streams (short tasks) with VALU adds
+// (as matrix products they spilled the main kernel's registers further and
+// were slower). Results do not depend on the shape (exact sums).
+template <int GL, int W>
 struct NmCfg {
   static constexpr uint32_t GLANES = GL;
   static constexpr uint32_t GROUPS = 64 / GL;
   static constexpr uint32_t WAVES = W;
-  static constexpr bool MFMA = MF;
-  static constexpr uint32_t TILE = GROUPS * NM_GSTRIDE > 128 ? GROUPS * NM_GSTRIDE : 128;
-  // MF stream, per wave: the digit planes of B (GROUPS x NM_MF_ROWS x 64
-  // bytes), the candidates' positions (GROUPS x 64 float2), and (over both,
-  // between chunks) the 4 x 16 x 16 int32 results of the matrix products
+  // per wave: the whole-wave stream's two chunks of 64 float4 candidates, or
+  // the grouped stream's digit planes of B (GROUPS x NM_MF_ROWS x 64 bytes)
+  // and candidate positions (GROUPS x 64 float2), and (over both, between
+  // chunks) the 4 x 16 x 16 int32 results of the matrix products
+  static constexpr uint32_t TILE_BYTES = 2u * 64u * 16u;
   static constexpr uint32_t MF_BYTES = GROUPS * (NM_MF_ROWS * NM_GCHUNK + NM_GCHUNK * 8u);
-  static constexpr uint32_t WAVE_BYTES_0 = TILE * 16u;
-  static constexpr uint32_t WAVE_BYTES_1 = MF ? (MF_BYTES > 4096u ? MF_BYTES : 4096u) : 0u;
-  static constexpr uint32_t WAVE_BYTES = WAVE_BYTES_0 > WAVE_BYTES_1 ? WAVE_BYTES_0 : WAVE_BYTES_1;
+  static constexpr uint32_t MF_MIN = MF_BYTES > 4096u ? MF_BYTES : 4096u;
+  static constexpr uint32_t WAVE_BYTES = TILE_BYTES > MF_MIN ? TILE_BYTES : MF_MIN;
 
   // record of a heavy task's set-up (BT_REC_WORDS words): per lane group its
   // ranges (start[128], pref[129]), then stream lengths m[GROUPS], range
@@ -607,9 +604,9 @@ struct NmCfg {
   static_assert(GL == 16 || GL == 32, "lane groups of 16 or 32");
 };
 
-// Per wave, `raw` holds either two chunks of 64 candidates (plain stream,
-// double buffered) or one chunk of 64 candidates per lane group (grouped
-// stream), as float4 records, or the matrix-instruction stream's layout.
+// Per wave, `raw` holds either two chunks of 64 candidates (whole-wave
+// stream, double buffered, as float4 records) or the grouped stream's
+// matrix-instruction layout.
 template <class C>
 struct NmLds {
   alignas(16) unsigned char raw[C::WAVES][C::WAVE_BYTES];
@@ -1060,60 +1057,7 @@ template <class C>
 __device__ __forceinline__ void nm_group_stream(const NmTask& T, NmLds<C>& L, const NmGroups& G,
                                                 uint32_t first, uint32_t step, long long& ax,
                                                 long long& ay, long long& n, uint64_t& steps) {
-  if constexpr (C::MFMA) {
-    nm_group_stream_mf<C>(T, L, G, first, step, ax, ay, n, steps);
-    return;
-  }
-  const uint32_t lane = LANE(), w = threadIdx.x >> 6, grp = lane / C::GLANES;
-  const BtJob& J = *T.J;
-  const float tsx = J.ts_x, tsy = J.ts_y, r2 = J.r2, px = T.px, py = T.py;
-  const float4* S = J.sorted;
-  const uint32_t* start = L.start[w][grp];
-  const uint32_t* pref = L.pref[w][grp];
-  const float inf = __int_as_float(0x7f800000);
-  const uint32_t nch = (G.mmax + NM_GCHUNK - 1u) / NM_GCHUNK;
-  const uint32_t sub = lane % C::GLANES;
-  uint32_t cur = 0;
-  auto fetch = [&](uint32_t ch, uint32_t i) -> float4 {
-    const uint32_t v = ch * NM_GCHUNK + i * C::GLANES + sub;
-    return v < G.my_m ? nm_candidate(S, start, pref, G.my_n, v, cur)
-                      : make_float4(inf, inf, 0.f, 0.f);
-  };
-  float4* tile = L.tile(w) + grp * NM_GSTRIDE;
-  float4 nxt[NM_GCHUNK / C::GLANES];
-  if (first < nch) {
-#pragma unroll
-    for (uint32_t i = 0; i < NM_GCHUNK / C::GLANES; ++i) nxt[i] = fetch(first, i);
-  }
-  for (uint32_t ch = first; ch < nch; ch += step) {
-    // one buffer: the previous chunk's reads are complete (wave barrier below)
-#pragma unroll
-    for (uint32_t i = 0; i < NM_GCHUNK / C::GLANES; ++i) tile[i * C::GLANES + sub] = nxt[i];
-    __builtin_amdgcn_wave_barrier();
-    if (ch + step < nch) {
-#pragma unroll
-      for (uint32_t i = 0; i < NM_GCHUNK / C::GLANES; ++i) nxt[i] = fetch(ch + step, i);
-    }
-    const uint32_t cnt = min((uint32_t)NM_GCHUNK, G.mmax - ch * (uint32_t)NM_GCHUNK);
-    steps += cnt;
-    int sx = 0, sy = 0, sc = 0;
-    uint32_t k = 0;
-    for (; k + 4u <= cnt; k += 4u) {
-#pragma unroll
-      for (uint32_t u = 0; u < 4u; ++u) {
-        const float4 s = tile[k + u];
-        NM_TEST(s)
-      }
-    }
-    for (; k < cnt; ++k) {
-      const float4 s = tile[k];
-      NM_TEST(s)
-    }
-    ax += sx;
-    ay += sy;
-    n += sc;
-    __builtin_amdgcn_wave_barrier();
-  }
+  nm_group_stream_mf<C>(T, L, G, first, step, ax, ay, n, steps);
 }
 
 // Stores a heavy task's set-up as its record (NmCfg's REC_ layout).
@@ -1222,7 +1166,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4, 4))) 
 bt_neighbor_mean(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split, unsigned long long samples) {
-  typedef NmCfg<GL, W, true> C;
+  typedef NmCfg<GL, W> C;
   __shared__ NmLds<C> L;
   __shared__ unsigned long long wave_cand[C::WAVES];
   const uint32_t lane = LANE(), w = threadIdx.x >> 6;
@@ -1343,7 +1287,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
 bt_neighbor_mean_heavy(
     const BtJob* __restrict__ jobs, const BtTask* __restrict__ tasks, BtControl* ctl,
     BtSplit split, unsigned long long samples) {
-  typedef NmCfg<GL, W, true> C;
+  typedef NmCfg<GL, W> C;
   __shared__ NmLds<C> L;
   const uint32_t lane = LANE();
   const uint32_t items = min(uni(__atomic_load_n(&ctl->item_count, __ATOMIC_RELAXED)),
