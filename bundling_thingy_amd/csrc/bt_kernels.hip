@@ -571,6 +571,7 @@ __device__ __forceinline__ uint32_t claim_work(uint32_t* cursors, uint32_t total
 #define NM_GROUP_MIN 2048u  // candidates above which a task streams per lane group
 #define NM_MF_ROWS 9        // B columns of the matrix-instruction sums: 4 + 4 digits, count
 #define NM_BATCH_CAND 256.0 // expected candidates per task below which tasks are claimed 4 at a time
+#define NM_BATCH8_CAND 64.0 // ... and below which 8 at a time
 
 // Kernel shape, a template parameter the autotuner chooses per iteration:
 // lanes per group of the grouped stream (16: 4 groups, 32: 2 groups) and
@@ -1185,11 +1186,13 @@ bt_neighbor_mean(
   // Tasks claimed per work-queue atomic: launches of short tasks (the
   // small-radius iterations, where each task is a chain of dependent loads
   // and the claims on the 8 cursors are a visible part of it) take 4 per
-  // claim; longer tasks one, for the balance at the end of the launch.
+  // claim, 8 when a task expects under 64 candidates; longer tasks one, for
+  // the balance at the end of the launch.
   uint32_t batch = 1u;
   if (prev && total) {
     const double expect = sprev ? (double)prev * ((double)samples / (double)sprev) : (double)prev;
-    batch = expect / (double)total < NM_BATCH_CAND ? 4u : 1u;
+    const double per_task = expect / (double)total;
+    batch = per_task < NM_BATCH8_CAND ? 8u : per_task < NM_BATCH_CAND ? 4u : 1u;
   }
   unsigned long long cand = 0;
   uint64_t steps = 0, gsteps = 0, nbr = 0;
