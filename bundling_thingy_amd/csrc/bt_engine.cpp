@@ -233,10 +233,11 @@ Engine::~Engine() {
     }
   stream_ = nullptr;
   if (switch_ev_) (void)hipEventDestroy(switch_ev_);
-  if (side_) {
-    (void)hipStreamSynchronize(side_);
-    (void)hipStreamDestroy(side_);
-  }
+  for (hipStream_t st : {side_, tasks_st_})
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
   for (hipEvent_t e : ov_ev_) (void)hipEventDestroy(e);
   for (Pinned& s : pinned_) {
     if (s.ev) (void)hipEventDestroy(s.ev);
@@ -458,10 +459,20 @@ void Engine::launch_iteration(const Plan& p, const unsigned char* dev, bool dens
   mark(2);
   btk::cell_scan(stream_, p.grid[BT_K_SCAN], J, B(BT_K_SCAN), (int)nj);
   mark(3);
-  btk::tasks(stream_, p.grid[BT_K_TASK], J, B(BT_K_TASK), (int)nj, ctl, tasks_.as<BtTask>());
+  if (ov) {
+    // the task list (it reads only the cell offsets) beside the scatter and
+    // the prefix scans
+    HIPCHK(hipEventRecord(ov->scan_done, stream_));
+    HIPCHK(hipStreamWaitEvent(tasks_st_, ov->scan_done, 0));
+    btk::tasks(tasks_st_, p.grid[BT_K_TASK], J, B(BT_K_TASK), (int)nj, ctl, tasks_.as<BtTask>());
+    HIPCHK(hipEventRecord(ov->tasks_done, tasks_st_));
+  } else {
+    btk::tasks(stream_, p.grid[BT_K_TASK], J, B(BT_K_TASK), (int)nj, ctl, tasks_.as<BtTask>());
+  }
   mark(4);
   btk::scatter(stream_, p.grid[BT_K_BIN], J, B(BT_K_BIN), (int)nj);
   btk::pre_scan(stream_, p.grid[BT_K_PTS], J, B(BT_K_PTS), (int)nj);
+  if (ov) HIPCHK(hipStreamWaitEvent(stream_, ov->tasks_done, 0));
   mark(5);
   // Persistent grid: 4 workgroups (16 waves) per CU, the kernel's occupancy
   // at its register count; a multiple of the XCD count.
@@ -889,7 +900,8 @@ void Engine::iterate(uint32_t n, bool use_graph) {
     // them); the next neighbor mean waits for them and for side_'s clear of
     // its density counts
     if (!side_) HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
-    while (ov_ev_.size() < 2 * (size_t)n) {
+    if (!tasks_st_) HIPCHK(hipStreamCreateWithFlags(&tasks_st_, hipStreamNonBlocking));
+    while (ov_ev_.size() < 4 * (size_t)n) {
       hipEvent_t e;
       HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       ov_ev_.push_back(e);
@@ -903,6 +915,8 @@ void Engine::iterate(uint32_t n, bool use_graph) {
       ov.prev_done = i ? ov_ev_[2 * i - 1] : nullptr;
       ov.nm_done = ov_ev_[2 * i];
       ov.side_done = ov_ev_[2 * i + 1];
+      ov.scan_done = ov_ev_[2 * n + 2 * i];
+      ov.tasks_done = ov_ev_[2 * n + 2 * i + 1];
       ov.next = i + 1 < n ? &plans[i + 1] : nullptr;
       ov.next_dev = buf->as<unsigned char>() + (i + 1) * slot;
       launch_iteration(plans[i], buf->as<unsigned char>() + i * slot, density, nullptr, &ov);

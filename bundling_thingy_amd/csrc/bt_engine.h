@@ -185,11 +185,13 @@ class Engine {
   // Copies a plan to device memory through a ring of pinned host slots (the
   // copy is asynchronous; a slot is reused once its previous copy is done).
   void put_plan(const Plan& p, void* dst, hipStream_t st = nullptr);
-  // A captured schedule's iteration whose density stages run on side_.
+  // A captured schedule's iteration: its task list runs on tasks_st_ and
+  // (side) its density stages on side_.
   struct Overlap {
     bool side = false;        // density stages on side_
     bool prev = false;        // the previous iteration's did (and cleared this one's counts)
     hipEvent_t prev_done = nullptr, nm_done = nullptr, side_done = nullptr;
+    hipEvent_t scan_done = nullptr, tasks_done = nullptr;  // the task list on tasks_st_
     const Plan* next = nullptr;  // the next iteration (side_ clears its density counts)
     const unsigned char* next_dev = nullptr;
   };
@@ -213,6 +215,7 @@ class Engine {
   int active_ = 0;
   hipEvent_t switch_ev_ = nullptr;
   hipStream_t side_ = nullptr;        // captured schedules' density stages
+  hipStream_t tasks_st_ = nullptr;    // ... and task lists
   std::vector<hipEvent_t> ov_ev_;     // their fork and join events
   std::vector<std::unique_ptr<Job>> jobs_;
   DevBuf ctl_, tasks_, plans_, graph_plans_, staging_, staging2_, split_, retire_plans_;
