@@ -91,6 +91,33 @@ def test_server_density_policies_equal_alone(bt):
         srv.close()
 
 
+def test_batched_graph_mixed_density_policies_equal_alone(bt):
+    """One batch whose jobs compute the density every iteration or only when
+    read, iterated as a captured schedule (the density stages and the next
+    iteration's density clear on a second stream, the task list on a third)
+    and replayed after a rewind: every job's points and density equal the
+    job alone iterated eagerly, after each run."""
+    graphs = workloads.batched_graphs(count=6, lo=60, hi=1500)
+    kw = dict(spacing=0.004, radius=(0.05, 0.02), sigma=(4.0, 2.0), density=256, schedule_len=5)
+    opts = [bt.Options(density_on_read=(k % 2 == 1), **kw) for k in range(6)]
+    alone = []
+    for g, o in zip(graphs, opts):
+        with bt.Gpu(o) as gpu:
+            gpu.upload(g)
+            gpu.iterate(5)
+            alone.append((gpu.points()[0], gpu.density(3)))
+    with bt.Gpu(opts[0]) as gpu:
+        gpu.upload_batch(graphs, opts)
+        for label in ("captured", "replayed"):
+            gpu.iterate(5, graph=True)
+            for j, (p_ref, d_ref) in enumerate(alone):
+                np.testing.assert_array_equal(_bits(gpu.points(j)[0]), _bits(p_ref),
+                                              f"{label} job {j} points")
+                np.testing.assert_array_equal(_bits(gpu.density(3, j)), _bits(d_ref),
+                                              f"{label} job {j} density")
+            gpu.rewind()
+
+
 def test_server_memory_budget_holds(bt):
     """A budget of under half the ten jobs' footprints: at every tick the bytes the
     admitted jobs' buffers hold stay within the footprint charged for them,
