@@ -907,6 +907,17 @@ void Engine::iterate(uint32_t n, bool use_graph) {
       ov_ev_.push_back(e);
     }
     HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+    // a launch that throws mid-capture must not leave the stream capturing
+    struct CaptureGuard {
+      hipStream_t s;
+      bool open = true;
+      ~CaptureGuard() {
+        if (!open) return;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+      }
+    } guard{stream_};
     bool prev = false;
     for (uint32_t i = 0; i < n; ++i) {
       Overlap ov;
@@ -924,8 +935,13 @@ void Engine::iterate(uint32_t n, bool use_graph) {
       if (density && plans[i].grid[BT_K_ROW] && !ov.next)
         HIPCHK(hipStreamWaitEvent(stream_, ov.side_done, 0));  // join
     }
+    guard.open = false;
     HIPCHK(hipStreamEndCapture(stream_, &g.graph));
-    HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    const hipError_t ie = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g.graph);
+      HIPCHK(ie);
+    }
     g.last_nm_grid = last_nm_grid_;
     found = graphs_.emplace(key, g).first;
     graph_bufs_.emplace(key, std::move(buf));
